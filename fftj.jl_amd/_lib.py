@@ -1,0 +1,86 @@
+"""ctypes binding of libfftjh.so (include/fftjh.h).
+
+The library is built in-tree by fftj.jl_amd/build.py.  There is no fallback:
+if the library is missing, loading fails with an error that says how to
+build it.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfftjh.so")
+
+OK, ERR_ARGUMENT, ERR_DIMENSION, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+F32, F64 = 0, 1
+KIND_C2C, KIND_R2C, KIND_C2R = 0, 1, 2
+
+# every entry point include/fftjh.h declares
+EXPORTS = (
+    "fftjh_plan_c2c", "fftjh_plan_r2c", "fftjh_plan_c2r", "fftjh_execute", "fftjh_execute_checked",
+    "fftjh_plan_info", "fftjh_plan_describe", "fftjh_describe_request", "fftjh_plan_destroy",
+    "fftjh_last_error", "fftjh_version", "fftjh_profile_launches",
+)
+
+
+class ArgumentError(ValueError):
+    """Invalid plan parameters (the reference's ArgumentError)."""
+
+
+class DimensionMismatch(ValueError):
+    """Array lengths or shapes that do not match the plan (DimensionMismatch)."""
+
+
+class HipError(RuntimeError):
+    """A HIP runtime call inside the library failed."""
+
+
+class UnsupportedError(NotImplementedError):
+    """A valid request this build does not plan."""
+
+
+_ERRS = {ERR_ARGUMENT: ArgumentError, ERR_DIMENSION: DimensionMismatch, ERR_HIP: HipError,
+         ERR_UNSUPPORTED: UnsupportedError}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python fftj.jl_amd/build.py` "
+                              "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        i64, vp, ci = ctypes.c_int64, ctypes.c_void_p, ctypes.c_int
+        P64 = ctypes.POINTER(i64)
+        L.fftjh_plan_c2c.argtypes = [ctypes.POINTER(vp), ci, ci, P64, i64, ci, ctypes.c_double, ctypes.c_uint]
+        L.fftjh_plan_r2c.argtypes = [ctypes.POINTER(vp), ci, i64, i64, ctypes.c_uint]
+        L.fftjh_plan_c2r.argtypes = [ctypes.POINTER(vp), ci, i64, i64, ci, ctypes.c_uint]
+        L.fftjh_execute.argtypes = [vp, vp, vp, vp]
+        L.fftjh_execute_checked.argtypes = [vp, vp, i64, vp, i64, vp]
+        L.fftjh_plan_info.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci),
+                                      ctypes.POINTER(ci), P64, P64, ctypes.POINTER(ctypes.c_double), P64, P64]
+        L.fftjh_plan_describe.argtypes = [vp, ctypes.c_char_p, i64]
+        L.fftjh_describe_request.argtypes = [ci, ci, ci, P64, i64, ci, ctypes.c_char_p, i64]
+        L.fftjh_plan_destroy.argtypes = [vp]
+        L.fftjh_last_error.restype = ctypes.c_char_p
+        L.fftjh_profile_launches.argtypes = [vp, vp, vp, vp, ci, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ci)]
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().fftjh_last_error().decode()
+        raise _ERRS.get(rc, RuntimeError)(msg)
+
+
+def dims_array(dims):
+    return (ctypes.c_int64 * max(1, len(dims)))(*dims)
+
+
+def describe_request(kind, dtype, dims, batch=1, sign=-1):
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib().fftjh_describe_request(kind, dtype, len(dims), dims_array(dims), batch, sign, buf, len(buf)))
+    return buf.value.decode()

@@ -1,0 +1,477 @@
+"""Plan/apply API of the GPU engine, mirroring FFTJ.jl's (api.jl, plan.jl,
+real.jl, ndplan.jl) so a GPU plan stands in for a CPU one.
+
+    Julia (reference)                 here
+    plan_fft(T, n)                    plan_fft(torch.float32, n)       plan.jl:207-213
+    plan_fft(x::Array)                plan_fft(x)                      api.jl:135-140
+    p(y, x) / p(x) / p * x            p(y, x) / p(x) / p * x           plan.jl:359-398, api.jl:61-63
+    mul!(y, p, x)                     mul_(y, p, x)                    api.jl:62-82
+    plan_fft!(T, n) / InplacePlan     plan_fft_(T, n) / InplacePlan    api.jl:43-52
+    inv(p) -> Scaled(p, s)            inv(p) -> Scaled(p, s)           api.jl:100-122
+    size(p)                           size(p)                          api.jl:85-88, 194-196
+    fft, ifft, bfft, fft!, rfft, ...  fft, ifft, bfft, fft_, rfft, ...  api.jl:124-127, 209-231
+
+Data are torch tensors on the GPU (complex64/complex128 for complex sides,
+float32/float64 for real sides), contiguous.  torch is only the memory and
+stream plumbing: every transform runs in the HIP kernels of libfftjh.so on
+torch's current stream.  N-d arrays: a row-major tensor of shape
+(s0, ..., s_{N-1}) is the column-major array of dims (s_{N-1}, ..., s0), so
+"transform all dims" is the same operation; size(p) reports the tensor shape.
+Python regions are 0-based.  An extension over the reference: 1-D plans take
+batch=B for B transforms stored back to back.  nthreads and FFTW planner
+flags are accepted and ignored (the reference ignores flags, api.jl:107-108;
+the GPU has no thread count).
+"""
+import ctypes
+import math
+
+import torch
+
+from . import _lib
+from ._lib import ArgumentError, DimensionMismatch, UnsupportedError, check, dims_array, lib
+
+F32, F64 = _lib.F32, _lib.F64
+_CPLX = {F32: torch.complex64, F64: torch.complex128}
+_REAL = {F32: torch.float32, F64: torch.float64}
+
+
+def _code(T):
+    """Precision code of a type argument (the reference's T)."""
+    if T in (torch.float32, torch.complex64, "f32", "float32", "Float32"):
+        return F32
+    if T in (torch.float64, torch.complex128, "f64", "float64", "Float64"):
+        return F64
+    raise ArgumentError(f"element type must be float32 or float64 (got {T})")
+
+
+def _code_of_tensor(x):
+    if x.dtype in (torch.complex64, torch.float32):
+        return F32
+    if x.dtype in (torch.complex128, torch.float64):
+        return F64
+    raise ArgumentError(f"unsupported element type {x.dtype}")
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _require(x, dtype, what):
+    if not isinstance(x, torch.Tensor):
+        raise ArgumentError(f"{what} must be a torch tensor")
+    if x.dtype != dtype:
+        raise TypeError(f"{what} has element type {x.dtype}, plan needs {dtype}")
+    if not x.is_cuda:
+        raise ArgumentError(f"{what} must be on the GPU")
+    if not x.is_contiguous():
+        raise ArgumentError(f"{what} must be contiguous")
+
+
+class _Handle:
+    __slots__ = ("ptr",)
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def __del__(self):
+        if self.ptr and _lib._LIB is not None:
+            _lib._LIB.fftjh_plan_destroy(self.ptr)
+            self.ptr = None
+
+
+class _PlanBase:
+    def _exec(self, y, x):
+        check(lib().fftjh_execute(self._h.ptr, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                  _stream()))
+        return y
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(lib().fftjh_plan_describe(self._h.ptr, buf, len(buf)))
+        return buf.value.decode()
+
+    def profile_launches(self, y, x, iters=20):
+        """Mean milliseconds of each kernel launch of this plan over `iters` runs."""
+        cap = ctypes.c_int(64)
+        ms = (ctypes.c_double * 64)()
+        check(lib().fftjh_profile_launches(self._h.ptr, ctypes.c_void_p(y.data_ptr()),
+                                           ctypes.c_void_p(x.data_ptr()), _stream(), iters, ms, ctypes.byref(cap)))
+        return [ms[i] for i in range(cap.value)]
+
+
+class FFTPlan(_PlanBase):
+    """Complex plan: 1-D (the reference's FFTPlan/FourStepPlan/ResamplePlan,
+    chosen by the GPU planner) or N-d (NdPlan).  Fields n, sign, scale."""
+
+    def __init__(self, dtype, dims, sign, scale, batch=1, shape=None):
+        self.dtype_code = dtype
+        self.dims = tuple(int(d) for d in dims)  # column-major, dims[0] fastest
+        self.sign = int(sign)
+        self.scale = float(scale)
+        self.batch = int(batch)
+        self.n = math.prod(self.dims)
+        self.shape = tuple(shape) if shape is not None else (
+            (self.n,) if len(self.dims) == 1 else tuple(reversed(self.dims)))
+        h = ctypes.c_void_p()
+        check(lib().fftjh_plan_c2c(ctypes.byref(h), dtype, len(self.dims), dims_array(self.dims), self.batch,
+                                   self.sign, self.scale, 0))
+        self._h = _Handle(h.value)
+        self._scaled = {}
+
+    @property
+    def dtype(self):
+        return _CPLX[self.dtype_code]
+
+    def _len_check(self, x, what):
+        want = self.n * self.batch
+        if x.numel() != want:
+            raise DimensionMismatch(f"plan length {want}, {what} length {x.numel()}")
+        if len(self.dims) > 1 and self.batch == 1 and x.dim() == len(self.dims) and tuple(x.shape) != self.shape:
+            raise DimensionMismatch(f"plan dims {self.shape}, {what} dims {tuple(x.shape)}")
+
+    def __call__(self, y, x=None):
+        if x is None:
+            x, y = y, None
+        _require(x, self.dtype, "input")
+        self._len_check(x, "input")
+        if y is None:
+            y = torch.empty_like(x)
+        _require(y, self.dtype, "output")
+        self._len_check(y, "output")
+        return self._exec(y, x)
+
+    def __mul__(self, x):
+        return self(x)
+
+    def _with_scale(self, s):
+        """The same transform with its scale multiplied by s (Scaled applies this)."""
+        key = float(s)
+        if key not in self._scaled:
+            self._scaled[key] = FFTPlan(self.dtype_code, self.dims, self.sign, self.scale * key, self.batch,
+                                        self.shape)
+        return self._scaled[key]
+
+    def _sibling(self, sign, scale):
+        return FFTPlan(self.dtype_code, self.dims, sign, scale, self.batch, self.shape)
+
+    def __repr__(self):
+        return f"FFTPlan(dims={self.dims}, sign={self.sign}, scale={self.scale}, batch={self.batch})"
+
+
+class RfftPlan(_PlanBase):
+    """Real forward plan (real.jl:6-13): n reals -> n//2+1 complex."""
+
+    def __init__(self, dtype, n, batch=1):
+        self.dtype_code = dtype
+        self.n = int(n)
+        self.batch = int(batch)
+        h = ctypes.c_void_p()
+        check(lib().fftjh_plan_r2c(ctypes.byref(h), dtype, self.n, self.batch, 0))
+        self._h = _Handle(h.value)
+
+    def __call__(self, y, x=None):
+        if x is None:
+            x, y = y, None
+        _require(x, _REAL[self.dtype_code], "input")
+        if x.numel() != self.n * self.batch:
+            raise DimensionMismatch(f"plan length {self.n * self.batch}, input length {x.numel()}")
+        nout = (self.n // 2 + 1) * self.batch
+        if y is None:
+            shape = (self.n // 2 + 1,) if self.batch == 1 else (self.batch, self.n // 2 + 1)
+            y = torch.empty(shape, dtype=_CPLX[self.dtype_code], device=x.device)
+        _require(y, _CPLX[self.dtype_code], "output")
+        if y.numel() != nout:
+            raise DimensionMismatch(f"rfft output must have length {nout}, got {y.numel()}")
+        return self._exec(y, x)
+
+    def __mul__(self, x):
+        return self(x)
+
+
+class IrfftPlan(_PlanBase):
+    """Real inverse plan (real.jl:15-23): n//2+1 complex -> n reals;
+    normalized=True is irfft, False is brfft."""
+
+    def __init__(self, dtype, n, normalized=True, batch=1):
+        self.dtype_code = dtype
+        self.n = int(n)
+        self.batch = int(batch)
+        self.normalized = bool(normalized)
+        h = ctypes.c_void_p()
+        check(lib().fftjh_plan_c2r(ctypes.byref(h), dtype, self.n, self.batch, int(self.normalized), 0))
+        self._h = _Handle(h.value)
+
+    def __call__(self, y, X=None):
+        if X is None:
+            X, y = y, None
+        _require(X, _CPLX[self.dtype_code], "input")
+        nin = (self.n // 2 + 1) * self.batch
+        if X.numel() != nin:
+            raise DimensionMismatch(f"irfft input must have length {nin}, got {X.numel()}")
+        if y is None:
+            shape = (self.n,) if self.batch == 1 else (self.batch, self.n)
+            y = torch.empty(shape, dtype=_REAL[self.dtype_code], device=X.device)
+        _require(y, _REAL[self.dtype_code], "output")
+        if y.numel() != self.n * self.batch:
+            raise DimensionMismatch(f"plan length {self.n * self.batch}, output length {y.numel()}")
+        return self._exec(y, X)
+
+    def __mul__(self, X):
+        return self(X)
+
+
+class InplacePlan:
+    """In-place wrapper (api.jl:43-52): ip * x transforms x in place."""
+
+    def __init__(self, p):
+        self.p = p
+
+    def __mul__(self, x):
+        self.p(x, x)
+        return x
+
+    def __call__(self, y, x=None):
+        if x is None:
+            return self * y
+        self.p(y, x)
+        return y
+
+
+class Scaled:
+    """A plan times a constant, as inv returns (api.jl:54-59); .p and .scale
+    are public and .p stays the unnormalized plan.  Application runs one
+    fused plan whose kernels fold the combined scale into their stores."""
+
+    def __init__(self, p, scale):
+        self.p = p
+        self.scale = scale
+
+    def _fused(self):
+        inner = self.p.p if isinstance(self.p, InplacePlan) else self.p
+        return inner._with_scale(self.scale)
+
+    def __mul__(self, x):
+        f = self._fused()
+        if isinstance(self.p, InplacePlan):
+            f(x, x)
+            return x
+        return f(x)
+
+    def __call__(self, y, x=None):
+        if x is None:
+            return self * y
+        self._fused()(y, x)
+        return y
+
+
+def mul_(y, p, x):
+    """mul!(y, p, x): apply p to x into y and return y (api.jl:61-82)."""
+    if isinstance(p, (InplacePlan, Scaled)):
+        p(y, x)
+        return y
+    return p(y, x)
+
+
+def inv(p):
+    """inv(p) (api.jl:100-122, 198-207): forward -> Scaled(bfft, 1/n) (so
+    inv(p).p is the unnormalized backward plan); ifft -> plain fft;
+    bfft -> Scaled(fft, 1/n); inv(c*P) = inv(P)/c."""
+    if isinstance(p, InplacePlan):
+        s = inv(p.p)
+        return Scaled(InplacePlan(s.p), s.scale) if isinstance(s, Scaled) else InplacePlan(s)
+    if isinstance(p, Scaled):
+        q = inv(p.p)
+        return Scaled(q.p, q.scale / p.scale) if isinstance(q, Scaled) else Scaled(q, 1.0 / p.scale)
+    if isinstance(p, FFTPlan):
+        if p.sign == -1:
+            return Scaled(p._sibling(+1, 1.0), 1.0 / p.n)
+        if p.scale == 1:
+            return Scaled(p._sibling(-1, 1.0), 1.0 / p.n)
+        return p._sibling(-1, 1.0)
+    raise ArgumentError(f"inv is not defined for {type(p).__name__}")
+
+
+def size(p):
+    """size(p) (api.jl:85-88, 194-196)."""
+    if isinstance(p, (InplacePlan,)):
+        return size(p.p)
+    if isinstance(p, Scaled):
+        return size(p.p)
+    if isinstance(p, FFTPlan):
+        return p.shape
+    if isinstance(p, RfftPlan):
+        return (p.n,)
+    if isinstance(p, IrfftPlan):
+        return (p.n // 2 + 1,)
+    raise ArgumentError(f"size is not defined for {type(p).__name__}")
+
+
+# ------------------------------------------------------------ constructors
+
+def _check_region(region, N):
+    # _checkregion_nd (api.jl:131-133): N-d plans transform all dims
+    if region is None:
+        return
+    r = tuple(region) if not isinstance(region, int) else (region,)
+    if r != tuple(range(N)):
+        raise ArgumentError(f"FFTJ N-d plans transform all dims (expected region {tuple(range(N))}, got {r})")
+
+
+def _positive_n(n):
+    n = int(n)
+    if n < 1:
+        raise ArgumentError(f"transform length must be positive (got {n})")
+    return n
+
+
+def _plan_c2c(T_or_x, n, sign, scale_kind, region, batch):
+    if isinstance(T_or_x, torch.Tensor):
+        x = T_or_x
+        if not x.is_complex():
+            raise ArgumentError("complex plans need a complex array")
+        code = _code_of_tensor(x)
+        N = x.dim()
+        _check_region(region, N)
+        shape = tuple(x.shape)
+        dims = (x.numel(),) if N <= 1 else tuple(reversed(shape))
+        if any(d < 1 for d in dims):
+            raise ArgumentError(f"dimensions must be positive (got {shape})")
+        length = math.prod(dims)
+        scale = 1.0 / length if scale_kind == "inv" else 1.0
+        return FFTPlan(code, dims, sign, scale, 1, shape if N > 1 else (length,))
+    code = _code(T_or_x)
+    n = _positive_n(n)
+    scale = 1.0 / n if scale_kind == "inv" else 1.0
+    return FFTPlan(code, (n,), sign, scale, batch)
+
+
+def plan_fft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, **flags):
+    """plan_fft(T, n) (plan.jl:207-208) or plan_fft(x[, region]) (api.jl:135-140)."""
+    return _plan_c2c(T_or_x, n, -1, "one", region, batch)
+
+
+def plan_ifft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, **flags):
+    """plan_ifft (plan.jl:209-210, api.jl:147-152): inverse scaled by 1/n."""
+    return _plan_c2c(T_or_x, n, +1, "inv", region, batch)
+
+
+def plan_bfft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, **flags):
+    """plan_bfft (plan.jl:212-213, api.jl:141-146): unnormalized inverse."""
+    return _plan_c2c(T_or_x, n, +1, "one", region, batch)
+
+
+def plan_fft_(T_or_x, n=None, region=None, **kw):
+    """plan_fft! (api.jl:47-48, 153-154)."""
+    return InplacePlan(plan_fft(T_or_x, n, region, **kw))
+
+
+def plan_ifft_(T_or_x, n=None, region=None, **kw):
+    return InplacePlan(plan_ifft(T_or_x, n, region, **kw))
+
+
+def plan_bfft_(T_or_x, n=None, region=None, **kw):
+    return InplacePlan(plan_bfft(T_or_x, n, region, **kw))
+
+
+def plan_rfft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, **flags):
+    """plan_rfft(T, n) (real.jl:41-52) or plan_rfft(x) (api.jl:159-164)."""
+    if isinstance(T_or_x, torch.Tensor):
+        x = T_or_x
+        if x.is_complex():
+            raise ArgumentError("rfft plans need a real array")
+        _check_region(region, x.dim())
+        if x.dim() > 1:
+            raise UnsupportedError("N-d real plans are outside this build")
+        return RfftPlan(_code_of_tensor(x), _positive_n(x.numel()), 1)
+    return RfftPlan(_code(T_or_x), _positive_n(n), batch)
+
+
+def _plan_c2r(T_or_X, n, normalized, region, batch):
+    if isinstance(T_or_X, torch.Tensor):
+        X = T_or_X
+        _check_region(region, X.dim())
+        if X.dim() > 1:
+            raise UnsupportedError("N-d real plans are outside this build")
+        n = int(n)
+        if X.shape[0] != n // 2 + 1:
+            what = "irfft" if normalized else "brfft"
+            raise DimensionMismatch(f"{what} of leading length {n} needs {n // 2 + 1} rows, got {X.shape[0]}")
+        return IrfftPlan(_code_of_tensor(X), _positive_n(n), normalized, 1)
+    return IrfftPlan(_code(T_or_X), _positive_n(n), normalized, batch)
+
+
+def plan_irfft(T_or_X, n, region=None, *, batch=1, nthreads=None, **flags):
+    """plan_irfft(T, n) (real.jl:57-70) or plan_irfft(X, n1) (api.jl:174-182)."""
+    return _plan_c2r(T_or_X, n, True, region, batch)
+
+
+def plan_brfft(T_or_X, n, region=None, *, batch=1, nthreads=None, **flags):
+    """plan_brfft(T, n) (real.jl:72-73) or plan_brfft(X, n1) (api.jl:165-173)."""
+    return _plan_c2r(T_or_X, n, False, region, batch)
+
+
+# ---------------------------------------------------------------- one-shots
+
+def _as_complex(x):
+    # promoting fallbacks (api.jl:220-231): integers and reals -> float complex
+    if not x.is_complex():
+        if not x.is_floating_point():
+            x = x.to(torch.float64)
+        x = x.to(torch.complex64 if x.dtype == torch.float32 else torch.complex128)
+    return x.contiguous()
+
+
+def _as_real(x):
+    if not x.is_floating_point():
+        x = x.to(torch.float64)
+    return x.contiguous()
+
+
+def fft(x):
+    x = _as_complex(x)
+    return plan_fft(x) * x
+
+
+def ifft(x):
+    x = _as_complex(x)
+    return plan_ifft(x) * x
+
+
+def bfft(x):
+    x = _as_complex(x)
+    return plan_bfft(x) * x
+
+
+def fft_(x):
+    """fft!(x) (api.jl:125): in place."""
+    return plan_fft_(x) * x
+
+
+def ifft_(x):
+    return plan_ifft_(x) * x
+
+
+def bfft_(x):
+    return plan_bfft_(x) * x
+
+
+def rfft(x):
+    x = _as_real(x)
+    return plan_rfft(x) * x
+
+
+def irfft(X, n):
+    X = _as_complex(X)
+    return plan_irfft(X, n) * X
+
+
+def brfft(X, n):
+    X = _as_complex(X)
+    return plan_brfft(X, n) * X
+
+
+# backend-neutral utilities the reference re-exports (FFTJ.jl:8)
+fftshift = torch.fft.fftshift
+ifftshift = torch.fft.ifftshift
+fftfreq = torch.fft.fftfreq
+rfftfreq = torch.fft.rfftfreq

@@ -1,0 +1,71 @@
+// Descriptors shared by the host planner and the HIP kernels.
+//
+// One "pass" is one kernel launch that runs whole length-n transforms, a group
+// of C of them per workgroup, entirely in LDS/registers.  Every plan (direct,
+// four-step, N-d axis passes, the m-length transform inside resampling, the
+// half-length transform inside real plans) is a short list of passes.
+#pragma once
+#include <stdint.h>
+
+namespace fftjh {
+
+constexpr int kMaxStages = 16;
+// LDS per workgroup on gfx950 (160 KiB, verified by compiling a 160 KiB array)
+constexpr int kLdsBytes = 160 * 1024;
+constexpr int kMaxThreads = 1024;
+// complex elements held in registers per thread inside one stage
+constexpr int kElemsPerThread = 16;
+
+// One Stockham radix stage (autosort form, twiddles before the butterfly):
+// for butterfly j < n/R with k = j mod ns,
+//   a_r = in[j + r*(n/R)] * w^{r*k},   w = exp(sign*2*pi*i/(ns*R))
+//   out[(j-k)*R + k + r*ns] = DFT_R(a)_r
+// ns is the product of the radices of the earlier stages.
+struct StageDesc {
+  int32_t radix;
+  int32_t ns;
+  int32_t tw_off;     // complex offset of this stage's table: entry (r-1)*ns + k
+  uint32_t ns_magic;  // floor(2^32/ns)+1: j/ns == umulhi(j, magic) for j*ns < 2^32
+};
+
+// element i of transform b lives at  (b % Bi)*s_in + (b / Bi)*s_out + i*stride
+struct Geometry {
+  int64_t stride;
+  int64_t s_in;
+  int64_t s_out;
+};
+
+struct PassDesc {
+  int32_t n;
+  int32_t nstages;
+  StageDesc st[kMaxStages];
+  int64_t batch;     // transforms in the launch
+  int64_t Bi;        // inner batch count of the two-level batch index
+  int32_t C;         // transforms per workgroup (power of two)
+  int32_t logC;
+  int32_t nthreads;  // threads per workgroup (multiple of 64 and of C)
+  int32_t ngroups;   // workgroups in the launch
+  Geometry in;
+  Geometry out;
+  // four-step twiddle at the store: output k of transform b is multiplied by
+  // w_N^{(b % Bi) * k} = hi[e >> lo_bits] * lo[e & (2^lo_bits - 1)]
+  int32_t post_tw;
+  int32_t lo_bits;
+  double scale;      // applied at the store
+  int32_t xcd_remap;  // map consecutive logical groups onto one XCD
+  int32_t pad_;
+};
+
+// LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)
+inline int lds_elems(int cn) { return cn + (cn >> 4) + 1; }
+
+// Work item of the ragged persistent kernel (one whole transform).
+struct RaggedItem {
+  int64_t offset;    // complex offset of the transform in the packed buffers
+  int32_t n;
+  int32_t kind;      // 0 = direct LDS chain, 1 = two-level (four-step in one workgroup), 2 = resample
+  int32_t sched;     // index into the schedule table
+  int32_t pad_;
+};
+
+}  // namespace fftjh

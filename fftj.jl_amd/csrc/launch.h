@@ -1,0 +1,39 @@
+// Host-side launchers of the HIP kernels (one per kernel family).
+// dtype: 0 = float32, 1 = float64.  All launches are asynchronous on `stream`.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "common.h"
+
+namespace fftjh {
+
+// d: host copy (launch geometry); dev_desc: the same descriptor in device memory
+hipError_t launch_pass(int dtype, int dir, const PassDesc& d, const PassDesc* dev_desc, const void* x,
+                       void* y, const void* tw, const void* thi, const void* tlo, hipStream_t stream);
+// largest workgroup the pass kernel is compiled for
+int max_threads(int dtype);
+
+hipError_t launch_tw_stage(int dtype, void* out, int ns, int R, int dir, hipStream_t stream);
+hipError_t launch_tw_pow(int dtype, void* out, int64_t count, int64_t mult, int64_t L, int dir,
+                         hipStream_t stream);
+
+hipError_t launch_r2c_twist(int dtype, const void* Z, void* y, const void* w, int64_t h, int64_t batch,
+                            int64_t zs, int64_t ys, hipStream_t st);
+hipError_t launch_c2r_pre(int dtype, const void* X, void* Z, const void* w, int64_t h, int64_t batch,
+                          int64_t xs, int64_t zs, double pre, hipStream_t st);
+hipError_t launch_real_to_complex(int dtype, const void* x, void* z, int64_t n, int64_t batch,
+                                  hipStream_t st);
+hipError_t launch_copy_prefix(int dtype, const void* z, void* y, int64_t n, int64_t keep, int64_t batch,
+                              hipStream_t st);
+hipError_t launch_hermitian_fill(int dtype, const void* X, void* z, int64_t n, int64_t batch,
+                                 hipStream_t st);
+hipError_t launch_take_real(int dtype, const void* z, void* y, int64_t count, hipStream_t st);
+
+hipError_t launch_rs_corr(int dtype, void* corr, int64_t n, int64_t m, int w, double beta, const double* zq,
+                          const double* wq, int nq, double scale, hipStream_t st);
+hipError_t launch_rs_deconv(int dtype, const void* x, void* g, const void* corr, int64_t n, int64_t m,
+                            int64_t batch, hipStream_t st);
+hipError_t launch_rs_interp(int dtype, const void* G, void* y, int64_t n, int64_t m, int w, double beta,
+                            int64_t batch, hipStream_t st);
+
+}  // namespace fftjh

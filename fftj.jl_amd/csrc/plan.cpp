@@ -1,0 +1,875 @@
+// Host planner and C-ABI entry points (include/fftjh.h).
+//
+// A plan is a list of steps (kernel launches) over five buffer roles: the
+// caller's x and y and up to four scratch buffers owned by the plan.
+// Planning runs in two phases: layout() decides every launch on the host
+// alone (it also serves fftjh_describe_request without a GPU), then
+// materialize() generates the twiddle/correction tables on the GPU (FP64,
+// shared through a process-wide cache like the reference's _STAGE_CACHE,
+// plan.jl:153-167), allocates scratch and uploads the pass descriptors.
+// Execution only launches kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/fftjh.h"
+#include "common.h"
+#include "launch.h"
+
+namespace fftjh {
+
+// ---------------------------------------------------------------- errors --
+
+static thread_local std::string g_err;
+
+struct Fail {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] static void fail(int code, const std::string& msg) { throw Fail{code, msg}; }
+
+static void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(FFTJH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------- device buffers --
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  explicit DevBuf(size_t b) : bytes(b) {
+    if (b) hip_check(hipMalloc(&p, b), "hipMalloc");
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+using DevBufPtr = std::shared_ptr<DevBuf>;
+
+// Immutable tables shared across plans: key -> weak reference, so a table
+// lives while some plan uses it (the reference keeps its entries for the
+// session; here device memory is returned when the last plan goes).
+static std::mutex g_cache_mu;
+static std::map<std::string, std::weak_ptr<DevBuf>> g_cache;
+
+template <typename F>
+static DevBufPtr cached(const std::string& key, size_t bytes, F&& fill) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_cache.find(key);
+  if (it != g_cache.end()) {
+    if (auto sp = it->second.lock()) return sp;
+  }
+  auto buf = std::make_shared<DevBuf>(bytes);
+  fill(buf->p);
+  hip_check(hipDeviceSynchronize(), "table generation");
+  g_cache[key] = buf;
+  return buf;
+}
+
+// ------------------------------------------------------- factorization --
+
+static const int kOddRadices[] = {3, 5, 7, 11, 13};
+
+// n = 2^a * prod p^c over {3,5,7,11,13} (plan.jl:20-34 uses the same prime set)
+static bool factor13(int64_t n, int& a, int cnt[5]) {
+  if (n < 1) return false;
+  a = 0;
+  while ((n & 1) == 0) {
+    n >>= 1;
+    ++a;
+  }
+  for (int i = 0; i < 5; ++i) {
+    cnt[i] = 0;
+    while (n % kOddRadices[i] == 0) {
+      n /= kOddRadices[i];
+      ++cnt[i];
+    }
+  }
+  return n == 1;
+}
+
+// default radix chain for one pass: the power-of-two part in radix-16
+// stages (remainder 2/4/8 first), then the odd primes ascending
+static std::vector<int> default_radices(int64_t n) {
+  int a, cnt[5];
+  std::vector<int> r;
+  if (!factor13(n, a, cnt)) return r;
+  if (a % 4) r.push_back(1 << (a % 4));
+  for (int i = 0; i < a / 4; ++i) r.push_back(16);
+  for (int i = 0; i < 5; ++i)
+    for (int k = 0; k < cnt[i]; ++k) r.push_back(kOddRadices[i]);
+  return r;
+}
+
+static int esize(int dtype) { return dtype == FFTJH_F32 ? 8 : 16; }
+
+// threads one workgroup needs for C transforms of length n with `radices`
+static int pass_threads(int n, int C, const std::vector<int>& radices) {
+  int t = 64;
+  for (int R : radices) {
+    const int bpt = kElemsPerThread / R;
+    const int64_t nb = static_cast<int64_t>(C) * (n / R);
+    t = std::max<int64_t>(t, (nb + bpt - 1) / bpt);
+  }
+  t = (t + 63) / 64 * 64;
+  return std::max(t, C);
+}
+
+static bool pass_fits(int dtype, int n, int C, const std::vector<int>& radices, int* threads) {
+  if (static_cast<int64_t>(lds_elems(C * n)) * esize(dtype) > kLdsBytes) return false;
+  const int t = pass_threads(n, C, radices);
+  if (t > max_threads(dtype)) return false;
+  if (threads) *threads = t;
+  return true;
+}
+
+static bool single_pass_ok(int dtype, int64_t n) {
+  if (n == 1) return true;
+  if (n > (1 << 16)) return false;
+  auto r = default_radices(n);
+  if (r.empty()) return false;
+  return pass_fits(dtype, static_cast<int>(n), 1, r, nullptr);
+}
+
+// ---------------------------------------------------------------- steps --
+
+enum Role { RX = 0, RY = 1, RS0 = 2, RS1 = 3, RS2 = 4, RS3 = 5, kRoles = 6 };
+
+enum StepKind {
+  ST_PASS,
+  ST_R2C_TWIST,
+  ST_C2R_PRE,
+  ST_REAL_TO_COMPLEX,
+  ST_COPY_PREFIX,
+  ST_HERMITIAN_FILL,
+  ST_TAKE_REAL,
+  ST_RS_DECONV,
+  ST_RS_INTERP,
+};
+
+static const char* kStepNames[] = {"pass",        "r2c_twist",  "c2r_pre",   "real_to_complex", "copy_prefix",
+                                   "hermitian_fill", "take_real", "rs_deconv", "rs_interp"};
+
+struct Step {
+  int kind = ST_PASS;
+  int src = RX, dst = RY;
+  int dir = -1;
+  // ST_PASS
+  PassDesc d{};
+  std::vector<int> radices;
+  int64_t post_n = 0;  // four-step length (post twiddle)
+  // other kernels
+  int64_t n = 0, m = 0, h = 0, batch = 0, s1 = 0, s2 = 0, keep = 0;
+  int w = 0;
+  double beta = 0.0, pre = 1.0, scale = 1.0;
+  // device resources
+  DevBufPtr tw, thi, tlo, desc, aux;
+};
+
+struct Layout {
+  int dtype = 0, kind = FFTJH_KIND_C2C, sign = -1, rank = 1;
+  int64_t dims[8] = {0};
+  int64_t batch = 1;
+  double scale = 1.0;
+  int64_t in_elems = 0, out_elems = 0;
+  std::vector<Step> steps;
+  int64_t scratch_elems[4] = {0, 0, 0, 0};  // complex elements per scratch role
+};
+
+static void need_scratch(Layout& L, int role, int64_t elems) {
+  if (role >= RS0) L.scratch_elems[role - RS0] = std::max(L.scratch_elems[role - RS0], elems);
+}
+
+static uint32_t magic_for(int ns) {
+  return ns > 1 ? static_cast<uint32_t>((uint64_t(1) << 32) / static_cast<uint64_t>(ns) + 1) : 0u;
+}
+
+// one launch of the pass kernel
+static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geometry in, Geometry out,
+                     double scale, int src, int dst, int64_t post_n) {
+  Step s;
+  s.kind = ST_PASS;
+  s.src = src;
+  s.dst = dst;
+  s.dir = dir;
+  s.post_n = post_n;
+  PassDesc& d = s.d;
+  d.n = n;
+  d.batch = batch;
+  d.Bi = Bi;
+  d.in = in;
+  d.out = out;
+  d.scale = scale;
+  s.radices = (n == 1) ? std::vector<int>{} : default_radices(n);
+  if (n > 1 && s.radices.empty()) fail(FFTJH_ERR_UNSUPPORTED, "pass length has a prime factor above 13");
+  if (static_cast<int>(s.radices.size()) > kMaxStages) fail(FFTJH_ERR_UNSUPPORTED, "too many stages");
+  // transforms per workgroup: strided passes take adjacent columns for
+  // contiguous HBM runs; row passes group short transforms to fill a workgroup
+  const bool colmode = in.stride > 1 || out.stride > 1;
+  int bestC = 0, bestT = 0;
+  for (int C = colmode ? 16 : 64; C >= 1; C >>= 1) {
+    int t;
+    if (!pass_fits(L.dtype, n, C, s.radices, &t)) continue;
+    if (C > 1 && C > batch) continue;
+    if (!colmode && t > 256 && C > 1) continue;
+    bestC = C;
+    bestT = t;
+    break;
+  }
+  if (!bestC) fail(FFTJH_ERR_UNSUPPORTED, "pass does not fit one workgroup");
+  d.C = bestC;
+  d.logC = 0;
+  while ((1 << d.logC) < bestC) ++d.logC;
+  d.nthreads = bestT;
+  const int64_t groups = (batch + bestC - 1) / bestC;
+  if (groups > 0x7fffffff) fail(FFTJH_ERR_UNSUPPORTED, "batch too large for one launch");
+  d.ngroups = static_cast<int>(groups);
+  d.xcd_remap = colmode ? 1 : 0;
+  int ns = 1, off = 0;
+  d.nstages = static_cast<int>(s.radices.size());
+  for (size_t i = 0; i < s.radices.size(); ++i) {
+    const int R = s.radices[i];
+    d.st[i].radix = R;
+    d.st[i].ns = ns;
+    d.st[i].tw_off = off;
+    d.st[i].ns_magic = magic_for(ns);
+    off += (R - 1) * ns;
+    ns *= R;
+  }
+  if (post_n) {
+    int lb = 0;
+    while ((int64_t(1) << (2 * lb)) < post_n) ++lb;  // 2^lb >= sqrt(N)
+    d.post_tw = 1;
+    d.lo_bits = lb;
+  }
+  L.steps.push_back(s);
+}
+
+static Geometry geo(int64_t stride, int64_t s_in, int64_t s_out) { return Geometry{stride, s_in, s_out}; }
+
+// four-step split n = N1*N2 (N2 = pass-A column length) with both factors a
+// single pass, nearest to sqrt(n) (the role of _fourstep_split/_fs_pick,
+// fourstep.jl:166-199: nearest divisor to sqrt(n) with buildable factors)
+static bool fourstep_split(int dtype, int64_t n, int64_t& N1, int64_t& N2) {
+  double best = 1e300;
+  N1 = N2 = 0;
+  const double rt = std::sqrt(static_cast<double>(n));
+  for (int64_t q = 2; q * q <= n; ++q) {
+    if (n % q) continue;
+    const int64_t cands[2] = {q, n / q};
+    for (int64_t a : cands) {
+      const int64_t b = n / a;
+      if (!single_pass_ok(dtype, a) || !single_pass_ok(dtype, b)) continue;
+      const double dist = std::fabs(std::log(static_cast<double>(a) / rt));
+      if (dist < best - 1e-12 || (std::fabs(dist - best) <= 1e-12 && a < N2)) {
+        best = dist;
+        N2 = a;
+        N1 = b;
+      }
+    }
+  }
+  return N2 != 0;
+}
+
+// ---- resampling parameters (resample.jl:12-21, 33-53, 202-256) ----
+
+static int64_t nextsmooth(int64_t N) {
+  static const int64_t odd[] = {1, 3, 5, 7, 9, 15, 21};
+  int64_t best = INT64_MAX;
+  for (int64_t o : odd) {
+    int64_t q = (N + o - 1) / o - 1;
+    int k = 0;
+    while (q) {
+      q >>= 1;
+      ++k;
+    }
+    k = std::max(k, 6);
+    best = std::min(best, o << k);
+  }
+  return best;
+}
+
+struct RsParams {
+  int64_t m;
+  int w;
+  double beta;
+};
+
+static RsParams rs_params(int dtype, int64_t n) {
+  RsParams p;
+  p.m = nextsmooth(2 * n);
+  const double eps = dtype == FFTJH_F32 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
+  const double tol = 10.0 * eps;
+  const double ratio = std::max(static_cast<double>(n) / p.m, 0.5);
+  const int w0 = static_cast<int>(std::ceil(std::log(1.0 / tol) / (M_PI * std::sqrt(1.0 - ratio))));
+  p.w = 4 * ((w0 + 3) / 4);
+  p.beta = 0.97 * M_PI * p.w * (1.0 - static_cast<double>(n) / (2.0 * p.m));
+  return p;
+}
+
+// Gauss-Legendre nodes/weights on [-1, 1] by Newton on the three-term
+// recurrence (the reference's _gausslegendre, resample.jl:33-53)
+static void gauss_legendre(int nq, std::vector<double>& z, std::vector<double>& wq) {
+  z.assign(nq, 0.0);
+  wq.assign(nq, 0.0);
+  for (int i = 1; i <= nq; ++i) {
+    double x = std::cos(M_PI * (i - 0.25) / (nq + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 20; ++it) {
+      double p0 = 1.0, p1 = x;
+      for (int k = 2; k <= nq; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = nq * (x * p1 - p0) / (x * x - 1.0);
+      const double dx = p1 / dp;
+      x -= dx;
+      if (std::fabs(dx) < 4 * 2.220446049250313e-16) break;
+    }
+    z[i - 1] = x;
+    wq[i - 1] = 2.0 / ((1.0 - x * x) * dp * dp);
+  }
+}
+
+// Complex transform of `batch` contiguous length-n rows, src -> dst, using
+// the scratch roles in `free` (front first).
+static void build_c2c_rows(Layout& L, int64_t n, int64_t batch, int dir, double scale, int src, int dst,
+                           std::vector<int> free) {
+  if (single_pass_ok(L.dtype, n)) {
+    add_pass(L, dir, static_cast<int>(n), batch, 1, geo(1, 0, n), geo(1, 0, n), scale, src, dst, 0);
+    return;
+  }
+  int a, cnt[5];
+  if (factor13(n, a, cnt)) {
+    int64_t N1, N2;
+    if (!fourstep_split(L.dtype, n, N1, N2))
+      fail(FFTJH_ERR_UNSUPPORTED, "no two-pass split of length " + std::to_string(n));
+    if (free.empty()) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for four-step");
+    const int z = free[0];
+    need_scratch(L, z, n * batch);
+    // pass A: columns j1 over j2 (stride N1), twiddle w_n^{j1*k2}, z[j1 + N1*k2]
+    add_pass(L, dir, static_cast<int>(N2), batch * N1, N1, geo(N1, 1, n), geo(N1, 1, n), 1.0, src, z, n);
+    // pass B: rows k2 of z over j1, y[k2 + N2*k1]
+    add_pass(L, dir, static_cast<int>(N1), batch * N2, N2, geo(1, N1, n), geo(N2, 1, n), scale, z, dst, 0);
+    return;
+  }
+  // not 13-smooth: resample onto a smooth m (resample.jl:258-276)
+  if (free.size() < 2) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for resampling");
+  const RsParams rp = rs_params(L.dtype, n);
+  const int g = free[0];
+  need_scratch(L, g, rp.m * batch);
+  Step dc;
+  dc.kind = ST_RS_DECONV;
+  dc.src = src;
+  dc.dst = g;
+  dc.dir = dir;
+  dc.n = n;
+  dc.m = rp.m;
+  dc.batch = batch;
+  dc.w = rp.w;
+  dc.beta = rp.beta;
+  dc.scale = scale;
+  L.steps.push_back(dc);
+  std::vector<int> rest(free.begin() + 1, free.end());
+  build_c2c_rows(L, rp.m, batch, dir, 1.0, g, g, rest);
+  Step ip;
+  ip.kind = ST_RS_INTERP;
+  ip.src = g;
+  ip.dst = dst;
+  ip.dir = dir;
+  ip.n = n;
+  ip.m = rp.m;
+  ip.batch = batch;
+  ip.w = rp.w;
+  ip.beta = rp.beta;
+  L.steps.push_back(ip);
+}
+
+static void check_dtype(int dtype) {
+  if (dtype != FFTJH_F32 && dtype != FFTJH_F64)
+    fail(FFTJH_ERR_ARGUMENT, "dtype must be FFTJH_F32 or FFTJH_F64");
+}
+
+static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch, int sign, double scale) {
+  check_dtype(dtype);
+  if (rank < 1 || rank > 8) fail(FFTJH_ERR_ARGUMENT, "rank must be in 1..8");
+  if (!dims) fail(FFTJH_ERR_ARGUMENT, "dims is NULL");
+  if (batch < 1) fail(FFTJH_ERR_ARGUMENT, "batch must be positive (got " + std::to_string(batch) + ")");
+  if (sign != -1 && sign != 1) fail(FFTJH_ERR_ARGUMENT, "sign must be -1 or +1");
+  int64_t len = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] < 1)
+      fail(FFTJH_ERR_ARGUMENT, rank == 1 ? "transform length must be positive (got " + std::to_string(dims[i]) + ")"
+                                         : "dimensions must be positive");
+    len *= dims[i];
+  }
+  Layout L;
+  L.dtype = dtype;
+  L.kind = FFTJH_KIND_C2C;
+  L.sign = sign;
+  L.rank = rank;
+  for (int i = 0; i < rank; ++i) L.dims[i] = dims[i];
+  L.batch = batch;
+  L.scale = scale;
+  L.in_elems = L.out_elems = len * batch;
+  if (rank == 1) {
+    build_c2c_rows(L, dims[0], batch, sign, scale, RX, RY, {RS0, RS1, RS2, RS3});
+    return L;
+  }
+  // N-d: one strided axis pass per dim of length > 1 (ndplan.jl:57-118,
+  // _nd_core! :270-327), x -> y then in place on y; scale on the last pass
+  std::vector<int> axes;
+  for (int i = 0; i < rank; ++i)
+    if (dims[i] > 1) axes.push_back(i);
+  if (axes.empty()) {
+    add_pass(L, sign, 1, batch, 1, geo(1, 0, 1), geo(1, 0, 1), scale, RX, RY, 0);
+    return L;
+  }
+  int src = RX;
+  for (size_t ai = 0; ai < axes.size(); ++ai) {
+    const int ax = axes[ai];
+    int64_t Bi = 1, Bo = batch;
+    for (int i = 0; i < ax; ++i) Bi *= dims[i];
+    for (int i = ax + 1; i < rank; ++i) Bo *= dims[i];
+    const int64_t nd = dims[ax];
+    const double sc = (ai + 1 == axes.size()) ? scale : 1.0;
+    if (!single_pass_ok(dtype, nd))
+      fail(FFTJH_ERR_UNSUPPORTED, "N-d axis length " + std::to_string(nd) +
+                                      " needs a single-pass 13-smooth length in this build");
+    add_pass(L, sign, static_cast<int>(nd), Bi * Bo, Bi, geo(Bi, 1, Bi * nd), geo(Bi, 1, Bi * nd), sc, src, RY, 0);
+    src = RY;
+  }
+  return L;
+}
+
+static Layout layout_r2c(int dtype, int64_t n, int64_t batch) {
+  check_dtype(dtype);
+  if (n < 1) fail(FFTJH_ERR_ARGUMENT, "transform length must be positive (got " + std::to_string(n) + ")");
+  if (batch < 1) fail(FFTJH_ERR_ARGUMENT, "batch must be positive");
+  Layout L;
+  L.dtype = dtype;
+  L.kind = FFTJH_KIND_R2C;
+  L.sign = -1;
+  L.rank = 1;
+  L.dims[0] = n;
+  L.batch = batch;
+  L.in_elems = n * batch;
+  L.out_elems = (n / 2 + 1) * batch;
+  if (n % 2 == 0) {
+    const int64_t h = n / 2;
+    // x read as h complex points per row (real.jl:88), transform into S0
+    need_scratch(L, RS0, h * batch);
+    build_c2c_rows(L, h, batch, -1, 1.0, RX, RS0, {RS1, RS2, RS3});
+    Step t;
+    t.kind = ST_R2C_TWIST;
+    t.src = RS0;
+    t.dst = RY;
+    t.dir = -1;
+    t.n = n;
+    t.h = h;
+    t.batch = batch;
+    t.s1 = h;
+    t.s2 = h + 1;
+    L.steps.push_back(t);
+  } else {
+    need_scratch(L, RS0, n * batch);
+    need_scratch(L, RS1, n * batch);
+    Step e;
+    e.kind = ST_REAL_TO_COMPLEX;
+    e.src = RX;
+    e.dst = RS0;
+    e.n = n;
+    e.batch = batch;
+    L.steps.push_back(e);
+    build_c2c_rows(L, n, batch, -1, 1.0, RS0, RS1, {RS2, RS3});
+    Step c;
+    c.kind = ST_COPY_PREFIX;
+    c.src = RS1;
+    c.dst = RY;
+    c.n = n;
+    c.keep = n / 2 + 1;
+    c.batch = batch;
+    L.steps.push_back(c);
+  }
+  return L;
+}
+
+static Layout layout_c2r(int dtype, int64_t n, int64_t batch, int normalized) {
+  check_dtype(dtype);
+  if (n < 1) fail(FFTJH_ERR_ARGUMENT, "transform length must be positive (got " + std::to_string(n) + ")");
+  if (batch < 1) fail(FFTJH_ERR_ARGUMENT, "batch must be positive");
+  Layout L;
+  L.dtype = dtype;
+  L.kind = FFTJH_KIND_C2R;
+  L.sign = 1;
+  L.rank = 1;
+  L.dims[0] = n;
+  L.batch = batch;
+  L.scale = normalized ? 1.0 / static_cast<double>(n) : 1.0;
+  L.in_elems = (n / 2 + 1) * batch;
+  L.out_elems = n * batch;
+  if (n % 2 == 0) {
+    const int64_t h = n / 2;
+    need_scratch(L, RS0, h * batch);
+    Step p;
+    p.kind = ST_C2R_PRE;
+    p.src = RX;
+    p.dst = RS0;
+    p.dir = 1;
+    p.n = n;
+    p.h = h;
+    p.batch = batch;
+    p.s1 = h + 1;
+    p.s2 = h;
+    p.pre = normalized ? 0.5 : 1.0;
+    L.steps.push_back(p);
+    // inner inverse of length h: scale 1/h (irfft) or 1 (brfft), real.jl:64-65
+    build_c2c_rows(L, h, batch, 1, normalized ? 1.0 / static_cast<double>(h) : 1.0, RS0, RY, {RS1, RS2, RS3});
+  } else {
+    need_scratch(L, RS0, n * batch);
+    need_scratch(L, RS1, n * batch);
+    Step f;
+    f.kind = ST_HERMITIAN_FILL;
+    f.src = RX;
+    f.dst = RS0;
+    f.n = n;
+    f.batch = batch;
+    L.steps.push_back(f);
+    build_c2c_rows(L, n, batch, 1, L.scale, RS0, RS1, {RS2, RS3});
+    Step t;
+    t.kind = ST_TAKE_REAL;
+    t.src = RS1;
+    t.dst = RY;
+    t.n = n;
+    t.batch = batch;
+    L.steps.push_back(t);
+  }
+  return L;
+}
+
+static std::string describe(const Layout& L) {
+  static const char* roles[] = {"x", "y", "s0", "s1", "s2", "s3"};
+  std::ostringstream o;
+  o << (L.kind == FFTJH_KIND_C2C ? "c2c" : L.kind == FFTJH_KIND_R2C ? "r2c" : "c2r")
+    << (L.dtype == FFTJH_F32 ? " f32" : " f64") << " dims=";
+  for (int i = 0; i < L.rank; ++i) o << (i ? "x" : "") << L.dims[i];
+  o << " batch=" << L.batch << " sign=" << L.sign << " launches=" << L.steps.size() << "\n";
+  for (const Step& s : L.steps) {
+    o << "  " << kStepNames[s.kind] << " " << roles[s.src] << "->" << roles[s.dst];
+    if (s.kind == ST_PASS) {
+      o << " n=" << s.d.n << " radices=";
+      for (size_t i = 0; i < s.radices.size(); ++i) o << (i ? "," : "") << s.radices[i];
+      o << " batch=" << s.d.batch << " C=" << s.d.C << " threads=" << s.d.nthreads << " groups=" << s.d.ngroups
+        << " in_stride=" << s.d.in.stride << " out_stride=" << s.d.out.stride;
+      if (s.d.post_tw) o << " twiddle=w_" << s.post_n;
+      if (s.d.scale != 1.0) o << " scale=" << s.d.scale;
+    } else if (s.kind == ST_RS_DECONV || s.kind == ST_RS_INTERP) {
+      o << " n=" << s.n << " m=" << s.m << " w=" << s.w << " beta=" << s.beta;
+    } else {
+      o << " n=" << s.n << " batch=" << s.batch;
+    }
+    o << "\n";
+  }
+  return o.str();
+}
+
+// ------------------------------------------------------------ the plan --
+
+}  // namespace fftjh
+
+struct fftjh_plan_s {
+  fftjh::Layout L;
+  fftjh::DevBufPtr scratch[4];
+  int device = 0;
+};
+
+namespace fftjh {
+
+static std::string key_of(const char* tag, std::initializer_list<double> vals) {
+  std::ostringstream o;
+  o.precision(17);
+  o << tag;
+  for (double v : vals) o << ":" << v;
+  return o.str();
+}
+
+static void materialize(fftjh_plan_s& P) {
+  Layout& L = P.L;
+  hip_check(hipGetDevice(&P.device), "hipGetDevice");
+  const int dt = L.dtype, es = esize(dt);
+  for (Step& s : L.steps) {
+    const double dev = P.device;
+    if (s.kind == ST_PASS) {
+      // stage twiddle table, shared by every pass with the same chain
+      std::ostringstream k;
+      k << "stage:" << dev << ":" << dt << ":" << s.dir;
+      for (int r : s.radices) k << ":" << r;
+      int64_t total = 0, ns = 1;
+      for (int R : s.radices) {
+        total += (R - 1) * ns;
+        ns *= R;
+      }
+      const std::vector<int> rad = s.radices;
+      const int dir = s.dir;
+      s.tw = cached(k.str(), std::max<int64_t>(total, 1) * es, [&](void* p) {
+        int64_t off = 0, nsl = 1;
+        for (int R : rad) {
+          hip_check(launch_tw_stage(dt, static_cast<char*>(p) + off * es, static_cast<int>(nsl), R, dir, 0),
+                    "twiddle kernel");
+          off += (R - 1) * nsl;
+          nsl *= R;
+        }
+      });
+      if (s.d.post_tw) {
+        const int64_t N = s.post_n, K = int64_t(1) << s.d.lo_bits, nh = (N + K - 1) / K;
+        s.tlo = cached(key_of("fs_lo", {dev, double(dt), double(dir), double(N)}), K * es, [&](void* p) {
+          hip_check(launch_tw_pow(dt, p, K, 1, N, dir, 0), "twiddle kernel");
+        });
+        s.thi = cached(key_of("fs_hi", {dev, double(dt), double(dir), double(N), double(K)}), nh * es,
+                       [&](void* p) { hip_check(launch_tw_pow(dt, p, nh, K, N, dir, 0), "twiddle kernel"); });
+      }
+      s.desc = std::make_shared<DevBuf>(sizeof(PassDesc));
+      hip_check(hipMemcpy(s.desc->p, &s.d, sizeof(PassDesc), hipMemcpyHostToDevice), "descriptor upload");
+    } else if (s.kind == ST_R2C_TWIST || s.kind == ST_C2R_PRE) {
+      // w_k = exp(dir*2*pi*i*k/n), k = 0..h (real.jl:26-39)
+      const int64_t cnt = s.h + 1, n = s.n;
+      const int dir = s.dir;
+      s.aux = cached(key_of("real", {dev, double(dt), double(dir), double(n)}), cnt * es,
+                     [&](void* p) { hip_check(launch_tw_pow(dt, p, cnt, 1, n, dir, 0), "twiddle kernel"); });
+    } else if (s.kind == ST_RS_DECONV) {
+      const int64_t n = s.n, m = s.m;
+      const int w = s.w;
+      const double beta = s.beta, scale = s.scale;
+      s.aux = cached(key_of("rs_corr", {dev, double(dt), double(n), double(m), double(w), beta, scale}),
+                     n * (es / 2), [&](void* p) {
+                       std::vector<double> zq, wq;
+                       gauss_legendre(2 * w + 20, zq, wq);
+                       DevBuf nodes(zq.size() * 2 * sizeof(double));
+                       hip_check(hipMemcpy(nodes.p, zq.data(), zq.size() * sizeof(double), hipMemcpyHostToDevice),
+                                 "node upload");
+                       double* wqd = static_cast<double*>(nodes.p) + zq.size();
+                       hip_check(hipMemcpy(wqd, wq.data(), wq.size() * sizeof(double), hipMemcpyHostToDevice),
+                                 "node upload");
+                       hip_check(launch_rs_corr(dt, p, n, m, w, beta, static_cast<const double*>(nodes.p), wqd,
+                                                static_cast<int>(zq.size()), scale, 0),
+                                 "corr kernel");
+                       hip_check(hipDeviceSynchronize(), "corr kernel");
+                     });
+    }
+  }
+  for (int i = 0; i < 4; ++i)
+    if (L.scratch_elems[i]) P.scratch[i] = std::make_shared<DevBuf>(L.scratch_elems[i] * es);
+}
+
+static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const* ptr, hipStream_t st) {
+  const int dt = P.L.dtype;
+  switch (s.kind) {
+    case ST_PASS:
+      return launch_pass(dt, s.dir, s.d, static_cast<const PassDesc*>(s.desc->p), ptr[s.src], ptr[s.dst], s.tw->p,
+                         s.thi ? s.thi->p : nullptr, s.tlo ? s.tlo->p : nullptr, st);
+    case ST_R2C_TWIST:
+      return launch_r2c_twist(dt, ptr[s.src], ptr[s.dst], s.aux->p, s.h, s.batch, s.s1, s.s2, st);
+    case ST_C2R_PRE:
+      return launch_c2r_pre(dt, ptr[s.src], ptr[s.dst], s.aux->p, s.h, s.batch, s.s1, s.s2, s.pre, st);
+    case ST_REAL_TO_COMPLEX:
+      return launch_real_to_complex(dt, ptr[s.src], ptr[s.dst], s.n, s.batch, st);
+    case ST_COPY_PREFIX:
+      return launch_copy_prefix(dt, ptr[s.src], ptr[s.dst], s.n, s.keep, s.batch, st);
+    case ST_HERMITIAN_FILL:
+      return launch_hermitian_fill(dt, ptr[s.src], ptr[s.dst], s.n, s.batch, st);
+    case ST_TAKE_REAL:
+      return launch_take_real(dt, ptr[s.src], ptr[s.dst], s.n * s.batch, st);
+    case ST_RS_DECONV:
+      return launch_rs_deconv(dt, ptr[s.src], ptr[s.dst], s.aux->p, s.n, s.m, s.batch, st);
+    case ST_RS_INTERP:
+      return launch_rs_interp(dt, ptr[s.src], ptr[s.dst], s.n, s.m, s.w, s.beta, s.batch, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+static void run(const fftjh_plan_s& P, void* y, const void* x, hipStream_t st, hipEvent_t* ev) {
+  void* ptr[kRoles] = {const_cast<void*>(x), y, nullptr, nullptr, nullptr, nullptr};
+  for (int i = 0; i < 4; ++i) ptr[RS0 + i] = P.scratch[i] ? P.scratch[i]->p : nullptr;
+  for (size_t i = 0; i < P.L.steps.size(); ++i) {
+    if (ev) hip_check(hipEventRecord(ev[2 * i], st), "hipEventRecord");
+    hip_check(launch_step(P, P.L.steps[i], ptr, st), kStepNames[P.L.steps[i].kind]);
+    if (ev) hip_check(hipEventRecord(ev[2 * i + 1], st), "hipEventRecord");
+  }
+}
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return FFTJH_OK;
+  } catch (const Fail& e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "out of host memory";
+    return FFTJH_ERR_HIP;
+  }
+}
+
+static int make_plan(fftjh_plan* out, Layout&& L) {
+  if (!out) fail(FFTJH_ERR_ARGUMENT, "plan output pointer is NULL");
+  auto P = std::make_unique<fftjh_plan_s>();
+  P->L = std::move(L);
+  materialize(*P);
+  *out = P.release();
+  return 0;
+}
+
+}  // namespace fftjh
+
+using namespace fftjh;
+
+extern "C" {
+
+int fftjh_plan_c2c(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
+                   double scale, unsigned flags) {
+  return guarded([&] {
+    if (plan) *plan = nullptr;
+    if (flags) fail(FFTJH_ERR_ARGUMENT, "flags must be 0");
+    make_plan(plan, layout_c2c(dtype, rank, dims, batch, sign, scale));
+  });
+}
+
+int fftjh_plan_r2c(fftjh_plan* plan, int dtype, int64_t n, int64_t batch, unsigned flags) {
+  return guarded([&] {
+    if (plan) *plan = nullptr;
+    if (flags) fail(FFTJH_ERR_ARGUMENT, "flags must be 0");
+    make_plan(plan, layout_r2c(dtype, n, batch));
+  });
+}
+
+int fftjh_plan_c2r(fftjh_plan* plan, int dtype, int64_t n, int64_t batch, int normalized, unsigned flags) {
+  return guarded([&] {
+    if (plan) *plan = nullptr;
+    if (flags) fail(FFTJH_ERR_ARGUMENT, "flags must be 0");
+    make_plan(plan, layout_c2r(dtype, n, batch, normalized));
+  });
+}
+
+int fftjh_execute(fftjh_plan plan, void* y, const void* x, fftjh_stream stream) {
+  return guarded([&] {
+    if (!plan) fail(FFTJH_ERR_ARGUMENT, "plan is NULL");
+    if (!x || !y) fail(FFTJH_ERR_ARGUMENT, "x and y must be device pointers");
+    run(*plan, y, x, static_cast<hipStream_t>(stream), nullptr);
+  });
+}
+
+int fftjh_execute_checked(fftjh_plan plan, void* y, int64_t y_elems, const void* x, int64_t x_elems,
+                          fftjh_stream stream) {
+  return guarded([&] {
+    if (!plan) fail(FFTJH_ERR_ARGUMENT, "plan is NULL");
+    const Layout& L = plan->L;
+    if (x_elems != L.in_elems)
+      fail(FFTJH_ERR_DIMENSION, "plan input length " + std::to_string(L.in_elems) + ", input length " +
+                                    std::to_string(x_elems));
+    if (y_elems != L.out_elems)
+      fail(FFTJH_ERR_DIMENSION, "plan output length " + std::to_string(L.out_elems) + ", output length " +
+                                    std::to_string(y_elems));
+    if (!x || !y) fail(FFTJH_ERR_ARGUMENT, "x and y must be device pointers");
+    run(*plan, y, x, static_cast<hipStream_t>(stream), nullptr);
+  });
+}
+
+int fftjh_plan_info(fftjh_plan plan, int* kind, int* dtype, int* sign, int* rank, int64_t* dims, int64_t* batch,
+                    double* scale, int64_t* in_elems, int64_t* out_elems) {
+  return guarded([&] {
+    if (!plan) fail(FFTJH_ERR_ARGUMENT, "plan is NULL");
+    const Layout& L = plan->L;
+    if (kind) *kind = L.kind;
+    if (dtype) *dtype = L.dtype;
+    if (sign) *sign = L.sign;
+    if (rank) *rank = L.rank;
+    if (dims)
+      for (int i = 0; i < 8; ++i) dims[i] = i < L.rank ? L.dims[i] : 0;
+    if (batch) *batch = L.batch;
+    if (scale) *scale = L.scale;
+    if (in_elems) *in_elems = L.in_elems;
+    if (out_elems) *out_elems = L.out_elems;
+  });
+}
+
+static void copy_out(const std::string& s, char* buf, int64_t len) {
+  if (!buf || len <= 0) return;
+  const size_t n = std::min<size_t>(s.size(), static_cast<size_t>(len - 1));
+  std::memcpy(buf, s.data(), n);
+  buf[n] = 0;
+}
+
+int fftjh_plan_describe(fftjh_plan plan, char* buf, int64_t len) {
+  return guarded([&] {
+    if (!plan) fail(FFTJH_ERR_ARGUMENT, "plan is NULL");
+    copy_out(describe(plan->L), buf, len);
+  });
+}
+
+int fftjh_describe_request(int kind, int dtype, int rank, const int64_t* dims, int64_t batch, int sign, char* buf,
+                           int64_t len) {
+  return guarded([&] {
+    Layout L;
+    if (kind == FFTJH_KIND_C2C) L = layout_c2c(dtype, rank, dims, batch, sign, sign < 0 ? 1.0 : 1.0);
+    else if (kind == FFTJH_KIND_R2C || kind == FFTJH_KIND_C2R) {
+      if (rank != 1 || !dims) fail(FFTJH_ERR_ARGUMENT, "real plans are 1-D");
+      L = kind == FFTJH_KIND_R2C ? layout_r2c(dtype, dims[0], batch) : layout_c2r(dtype, dims[0], batch, 1);
+    } else fail(FFTJH_ERR_ARGUMENT, "unknown kind");
+    copy_out(describe(L), buf, len);
+  });
+}
+
+int fftjh_plan_destroy(fftjh_plan plan) {
+  return guarded([&] { delete plan; });
+}
+
+const char* fftjh_last_error(void) { return g_err.c_str(); }
+
+int fftjh_version(void) { return 100; }
+
+int fftjh_profile_launches(fftjh_plan plan, void* y, const void* x, fftjh_stream stream, int iters, double* ms,
+                           int* nlaunch) {
+  return guarded([&] {
+    if (!plan) fail(FFTJH_ERR_ARGUMENT, "plan is NULL");
+    if (iters < 1) fail(FFTJH_ERR_ARGUMENT, "iters must be positive");
+    const size_t ns = plan->L.steps.size();
+    const int cap = nlaunch ? *nlaunch : 0;
+    if (nlaunch) *nlaunch = static_cast<int>(ns);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::vector<hipEvent_t> ev(2 * ns * iters);
+    for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    std::vector<double> acc(ns, 0.0);
+    try {
+      for (int it = 0; it < iters; ++it) run(*plan, y, x, st, ev.data() + 2 * ns * it);
+      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+      for (int it = 0; it < iters; ++it)
+        for (size_t i = 0; i < ns; ++i) {
+          float t = 0.f;
+          hip_check(hipEventElapsedTime(&t, ev[2 * ns * it + 2 * i], ev[2 * ns * it + 2 * i + 1]), "elapsed");
+          acc[i] += t;
+        }
+    } catch (...) {
+      for (auto& e : ev) (void)hipEventDestroy(e);
+      throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (size_t i = 0; i < ns && static_cast<int>(i) < cap; ++i) ms[i] = acc[i] / iters;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+/* fftjh -- C-ABI of the MI355X (gfx950) FFT engine that stands in for the
+ * plan/apply API of FFTJ.jl.
+ *
+ * The reference has no FFI: its "operator API" is the set of Julia plan
+ * constructors and plan application methods.  Each entry point below names
+ * the reference interface it replaces; a Julia package binds them with
+ * ccall (the binding is shown in INTEGRATION.md of this repository).
+ *
+ * Conventions (all mirror the reference):
+ *   - sign: -1 forward, +1 inverse (the reference's D, kernels.jl:3);
+ *   - scale: multiplied into the output; the reference's defaults are 1 for
+ *     fft and bfft and 1/n for ifft (plan.jl:97, 207-213);
+ *   - data: interleaved complex (re, im) of the plan's precision; N-d arrays
+ *     column-major, dims[0] fastest (ndplan.jl:7-31);
+ *   - execution is alias safe: y == x is an in-place transform
+ *     (api.jl:39-40);
+ *   - errors are reported before any work is queued; codes 1 and 2 map to
+ *     Julia's ArgumentError and DimensionMismatch (plan.jl:171, 361-362).
+ *
+ * Pointers passed to execute are device pointers (hipMalloc'd memory on the
+ * plan's device).  Execution is asynchronous on the given stream, allocates
+ * nothing, and may be captured into a hipGraph.  A plan owns its scratch, so
+ * one plan must not run on two streams at once (the reference clones plans
+ * for the same reason, fourstep.jl:332-336); distinct plans may.
+ */
+#ifndef FFTJH_H
+#define FFTJH_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFTJH_OK 0
+#define FFTJH_ERR_ARGUMENT 1    /* Julia ArgumentError */
+#define FFTJH_ERR_DIMENSION 2   /* Julia DimensionMismatch */
+#define FFTJH_ERR_HIP 3         /* a HIP runtime call failed */
+#define FFTJH_ERR_UNSUPPORTED 4 /* valid request outside what this build plans */
+
+#define FFTJH_F32 0 /* Float32 / ComplexF32 */
+#define FFTJH_F64 1 /* Float64 / ComplexF64 */
+
+#define FFTJH_KIND_C2C 0
+#define FFTJH_KIND_R2C 1
+#define FFTJH_KIND_C2R 2
+
+typedef struct fftjh_plan_s* fftjh_plan;
+/* hipStream_t, passed as an opaque pointer (NULL = the null stream) */
+typedef void* fftjh_stream;
+
+/* Complex plan.  rank 1 with dims[0] = n replaces
+ *   plan_fft / plan_ifft / plan_bfft(::Type{T}, n)        plan.jl:207-213
+ * rank >= 2 replaces the array-based N-d constructors
+ *   plan_fft / plan_ifft / plan_bfft(x::Array{Complex{T},N})  api.jl:135-152
+ * (all dims transformed, dims[0] fastest).  batch > 1 runs that many
+ * independent transforms stored back to back (an extension: the reference
+ * has no batch API and loops over 1-D plans).  flags must be 0 (the
+ * reference accepts and ignores FFTW planner flags, api.jl:107-108). */
+int fftjh_plan_c2c(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
+                   double scale, unsigned flags);
+
+/* Real-to-complex forward plan: n reals -> n/2+1 complex per transform.
+ * Replaces plan_rfft(::Type{T}, n)                        real.jl:41-52 */
+int fftjh_plan_r2c(fftjh_plan* plan, int dtype, int64_t n, int64_t batch, unsigned flags);
+
+/* Complex-to-real inverse plan: n/2+1 complex -> n reals per transform.
+ * normalized != 0 replaces plan_irfft(::Type{T}, n)        real.jl:57-70
+ * normalized == 0 replaces plan_brfft(::Type{T}, n)        real.jl:72-73 */
+int fftjh_plan_c2r(fftjh_plan* plan, int dtype, int64_t n, int64_t batch, int normalized, unsigned flags);
+
+/* Apply the plan: y = P x.  Replaces p(y, x) / mul!(y, p, x)
+ * (plan.jl:359-396, fourstep.jl:338-356, real.jl:80-211, ndplan.jl:329-335,
+ * api.jl:61-69).  No length checks (see fftjh_execute_checked). */
+int fftjh_execute(fftjh_plan plan, void* y, const void* x, fftjh_stream stream);
+
+/* As fftjh_execute, after checking the element counts of y and x against
+ * the plan (complex elements for complex sides, reals for real sides);
+ * returns FFTJH_ERR_DIMENSION before queuing anything on a mismatch, as
+ * the reference's call methods do (plan.jl:361-362, real.jl:82-84). */
+int fftjh_execute_checked(fftjh_plan plan, void* y, int64_t y_elems, const void* x, int64_t x_elems,
+                          fftjh_stream stream);
+
+/* Shape queries.  Replaces size(p) (api.jl:85-88, 194-196): kind, dtype,
+ * sign, rank, dims, batch, scale and the element counts of one whole call's
+ * input and output. */
+int fftjh_plan_info(fftjh_plan plan, int* kind, int* dtype, int* sign, int* rank, int64_t* dims /* [8] */,
+                    int64_t* batch, double* scale, int64_t* in_elems, int64_t* out_elems);
+
+/* Human-readable schedule of the plan (kernel launches, radices, groups). */
+int fftjh_plan_describe(fftjh_plan plan, char* buf, int64_t len);
+
+/* Same description for a plan request, computed on the host only (no GPU
+ * needed); kind is FFTJH_KIND_*; for C2R, normalized is taken as 1. */
+int fftjh_describe_request(int kind, int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
+                           char* buf, int64_t len);
+
+/* Release the plan's scratch and its hold on the shared tables. */
+int fftjh_plan_destroy(fftjh_plan plan);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char* fftjh_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int fftjh_version(void);
+
+/* Timing support for the benchmark: runs `iters` applications of the plan
+ * on `stream` with HIP events around every kernel launch, and writes the
+ * mean duration in milliseconds of launch i into ms[i] (up to *nlaunch
+ * entries; *nlaunch is set to the plan's launch count).  Host-synchronous. */
+int fftjh_profile_launches(fftjh_plan plan, void* y, const void* x, fftjh_stream stream, int iters,
+                           double* ms, int* nlaunch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FFTJH_H */

@@ -1,0 +1,655 @@
+/* Type-generic core of the CPU oracle, included once per element type with
+ *   T    the real type (float, double, long double)
+ *   SUF  the name suffix (_f32, _f64, _ld)
+ *   TW   the reference's pick_width(T) on an AVX-512 host (kernels.jl:11-22)
+ * ORACLE ONLY: test infrastructure, never a product path.
+ */
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define F(name) CAT(name, SUF)
+
+/* ------------------------------------------------------------------ chain */
+
+typedef struct {
+  int n;
+  int nst;
+  int rad[40];
+  T* twr[40];
+  T* twi[40];
+} F(Chain);
+
+/* make_stage (plan.jl:66-95), access-order layout: entry (r-1)*p + j - 1 is
+ * w^{p*j}, w = exp(D*2*pi*i/n), angle from an exact integer numerator */
+static void F(make_twiddles)(int r, int n, int D, T** twr, T** twi) {
+  int m = n / r;
+  *twr = (T*)malloc(sizeof(T) * (size_t)((r - 1) * m + 1));
+  *twi = (T*)malloc(sizeof(T) * (size_t)((r - 1) * m + 1));
+  for (int p = 0; p < m; ++p)
+    for (int j = 1; j < r; ++j) {
+      long double c, s;
+      expi_exact((int64_t)p * j, n, D, &c, &s);
+      (*twr)[(r - 1) * p + j - 1] = (T)c;
+      (*twi)[(r - 1) * p + j - 1] = (T)s;
+    }
+}
+
+static int F(chain_init)(F(Chain) * ch, int64_t n, int D) {
+  int rad[64];
+  int nst = build_chain(n, TW, rad);
+  if (nst < 0) return -1;
+  ch->n = (int)n;
+  ch->nst = nst;
+  int ncur = (int)n;
+  for (int i = 0; i < nst; ++i) {
+    ch->rad[i] = rad[i];
+    F(make_twiddles)(rad[i], ncur, D, &ch->twr[i], &ch->twi[i]);
+    ncur /= rad[i];
+  }
+  return 0;
+}
+
+static void F(chain_free)(F(Chain) * ch) {
+  for (int i = 0; i < ch->nst; ++i) {
+    free(ch->twr[i]);
+    free(ch->twi[i]);
+  }
+  ch->nst = 0;
+}
+
+/* One DIF Stockham stage (stockham.jl:26-130, 351-428): n = current length,
+ * s = stride (lanes), m = n/r butterflies; for p < m, q < s
+ *   a_j = x[s*p + j*m*s + q],  y[r*s*p + k*s + q] = w^{p*k} * DFT_r(a)_k     */
+static void F(stage)(T* yre, T* yim, const T* xre, const T* xim, int n, int64_t s, int r, const T* twr,
+                     const T* twi, int D) {
+  int m = n / r;
+  int64_t ms = (int64_t)m * s;
+  int h = (r - 1) / 2;
+  T cc[64], ss[64];
+  if (r % 2 == 1) { /* bfp_consts (kernels.jl:158-163) */
+    for (int k = 1; k <= h; ++k)
+      for (int j = 1; j <= h; ++j) {
+        long double c, sn;
+        expi_exact((int64_t)j * k, r, 1, &c, &sn);
+        cc[(k - 1) * h + j - 1] = (T)c;
+        ss[(k - 1) * h + j - 1] = (T)(D * sn);
+      }
+  }
+  const T c8 = (T)0.70710678118654752440084436210484903928L;
+  for (int p = 0; p < m; ++p) {
+    int64_t xb = s * p, yb = (int64_t)r * s * p;
+    for (int64_t q = 0; q < s; ++q) {
+      T ar[16], ai[16], yr[16], yi[16];
+      for (int j = 0; j < r; ++j) {
+        ar[j] = xre[xb + j * ms + q];
+        ai[j] = xim[xb + j * ms + q];
+      }
+      if (r == 2) { /* bf2 (kernels.jl:93-98) */
+        yr[0] = ar[0] + ar[1];
+        yi[0] = ai[0] + ai[1];
+        yr[1] = ar[0] - ar[1];
+        yi[1] = ai[0] - ai[1];
+      } else if (r == 4) {
+        F(bf4_core)(ar[0], ai[0], ar[1], ai[1], ar[2], ai[2], ar[3], ai[3], D, yr, yi);
+      } else if (r == 8) { /* bf8_core (kernels.jl:124-144) */
+        T er[4], ei[4], or_[4], oi[4];
+        F(bf4_core)(ar[0], ai[0], ar[2], ai[2], ar[4], ai[4], ar[6], ai[6], D, er, ei);
+        F(bf4_core)(ar[1], ai[1], ar[3], ai[3], ar[5], ai[5], ar[7], ai[7], D, or_, oi);
+        T t1r, t1i, t2r, t2i, t3r, t3i;
+        if (D == -1) {
+          t1r = c8 * (or_[1] + oi[1]);
+          t1i = c8 * (oi[1] - or_[1]);
+          t2r = oi[2];
+          t2i = -or_[2];
+          t3r = c8 * (oi[3] - or_[3]);
+          t3i = -c8 * (or_[3] + oi[3]);
+        } else {
+          t1r = c8 * (or_[1] - oi[1]);
+          t1i = c8 * (oi[1] + or_[1]);
+          t2r = -oi[2];
+          t2i = or_[2];
+          t3r = -c8 * (or_[3] + oi[3]);
+          t3i = c8 * (or_[3] - oi[3]);
+        }
+        yr[0] = er[0] + or_[0]; yi[0] = ei[0] + oi[0];
+        yr[1] = er[1] + t1r;    yi[1] = ei[1] + t1i;
+        yr[2] = er[2] + t2r;    yi[2] = ei[2] + t2i;
+        yr[3] = er[3] + t3r;    yi[3] = ei[3] + t3i;
+        yr[4] = er[0] - or_[0]; yi[4] = ei[0] - oi[0];
+        yr[5] = er[1] - t1r;    yi[5] = ei[1] - t1i;
+        yr[6] = er[2] - t2r;    yi[6] = ei[2] - t2i;
+        yr[7] = er[3] - t3r;    yi[7] = ei[3] - t3i;
+      } else { /* bfp (kernels.jl:174-228) */
+        T tr[8], ti[8], dr[8], di[8];
+        T y0r = ar[0], y0i = ai[0];
+        for (int j = 1; j <= h; ++j) {
+          tr[j] = ar[j] + ar[r - j];
+          ti[j] = ai[j] + ai[r - j];
+          dr[j] = ar[j] - ar[r - j];
+          di[j] = ai[j] - ai[r - j];
+          y0r += tr[j];
+          y0i += ti[j];
+        }
+        yr[0] = y0r;
+        yi[0] = y0i;
+        for (int k = 1; k <= h; ++k) {
+          T ur = ar[0], ui = ai[0], vr = 0, vi = 0;
+          for (int j = h; j >= 1; --j) {
+            ur += cc[(k - 1) * h + j - 1] * tr[j];
+            ui += cc[(k - 1) * h + j - 1] * ti[j];
+            vr += ss[(k - 1) * h + j - 1] * dr[j];
+            vi += ss[(k - 1) * h + j - 1] * di[j];
+          }
+          yr[k] = ur - vi;
+          yi[k] = ui + vr;
+          yr[r - k] = ur + vi;
+          yi[r - k] = ui - vr;
+        }
+      }
+      yre[yb + q] = yr[0];
+      yim[yb + q] = yi[0];
+      for (int k = 1; k < r; ++k) {
+        T wr = twr[(r - 1) * p + k - 1], wi = twi[(r - 1) * p + k - 1];
+        yre[yb + k * s + q] = wr * yr[k] - wi * yi[k];
+        yim[yb + k * s + q] = wr * yi[k] + wi * yr[k];
+      }
+    }
+  }
+}
+
+/* run_chain! (plan.jl:311-321): ping-pong, stride s*B with B inner lanes;
+ * returns 0 if the result is in (are, aim), 1 if in (bre, bim) */
+static int F(chain_run)(const F(Chain) * ch, T* are, T* aim, T* bre, T* bim, int64_t B, int D) {
+  int ncur = ch->n;
+  int64_t s = 1;
+  T *cr = are, *ci = aim, *orr = bre, *oi = bim;
+  for (int i = 0; i < ch->nst; ++i) {
+    F(stage)(orr, oi, cr, ci, ncur, s * B, ch->rad[i], ch->twr[i], ch->twi[i], D);
+    T* t = cr; cr = orr; orr = t;
+    t = ci; ci = oi; oi = t;
+    s *= ch->rad[i];
+    ncur /= ch->rad[i];
+  }
+  return cr == are ? 0 : 1;
+}
+
+/* FFTPlan call (plan.jl:359-396): deinterleave, chain, scaled interleave */
+static int F(fft_chain)(int64_t n, int D, T scale, const T* x, T* y) {
+  F(Chain) ch;
+  if (F(chain_init)(&ch, n, D)) return -1;
+  T* buf = (T*)malloc(sizeof(T) * 4 * (size_t)n);
+  T *are = buf, *aim = buf + n, *bre = buf + 2 * n, *bim = buf + 3 * n;
+  for (int64_t j = 0; j < n; ++j) {
+    are[j] = x[2 * j];
+    aim[j] = x[2 * j + 1];
+  }
+  int which = F(chain_run)(&ch, are, aim, bre, bim, 1, D);
+  T* rr = which ? bre : are;
+  T* ri = which ? bim : aim;
+  for (int64_t j = 0; j < n; ++j) {
+    y[2 * j] = rr[j] * scale;
+    y[2 * j + 1] = ri[j] * scale;
+  }
+  free(buf);
+  F(chain_free)(&ch);
+  return 0;
+}
+
+/* -------------------------------------------------------------- four-step */
+
+/* FourStepPlan (fourstep.jl:272-495), one task: j = j1 + N1*j2, k = k2 + N2*k1.
+ * Pass A: strips of S columns, chain over j2 with S lanes, twiddle
+ * w^{j1*k2} from the two-level table hi[e/K]*lo[e%K] (K = isqrt(n-1)+1,
+ * fourstep.jl:297-308), store z[j1 + N1*k2].  Pass B: groups of G columns
+ * k2 of z transposed into lanes, chain over j1, scaled natural-order rows.
+ * The reference's further lane factor b1*b2 and the nested inner split are
+ * register/cache details and are not restated.
+ * nt > 1 runs strips and groups on nt threads (tforeach_chunks). */
+typedef struct {
+  int64_t n, N1, N2, S, K;
+  int D;
+  T scale;
+  const T* x;
+  T* y;
+  T *zre, *zim;
+  T *hir, *hii, *lor, *loi;
+  F(Chain) chA, chB;
+} F(FsCtx);
+
+static void F(fs_pass_a)(F(FsCtx) * c, int64_t slo, int64_t shi) {
+  int64_t N1 = c->N1, N2 = c->N2, S = c->S;
+  T* buf = (T*)malloc(sizeof(T) * 4 * (size_t)(S * N2));
+  T *are = buf, *aim = buf + S * N2, *bre = buf + 2 * S * N2, *bim = buf + 3 * S * N2;
+  for (int64_t st = slo; st < shi; ++st) {
+    int64_t c0 = st * S;
+    int64_t lim = N1 - c0 < S ? N1 - c0 : S;
+    for (int64_t j2 = 0; j2 < N2; ++j2)
+      for (int64_t l = 0; l < S; ++l) {
+        int64_t e = c0 + l + N1 * j2;
+        are[S * j2 + l] = l < lim ? c->x[2 * e] : 0;
+        aim[S * j2 + l] = l < lim ? c->x[2 * e + 1] : 0;
+      }
+    int which = F(chain_run)(&c->chA, are, aim, bre, bim, S, c->D);
+    T* rr = which ? bre : are;
+    T* ri = which ? bim : aim;
+    for (int64_t k2 = 0; k2 < N2; ++k2)
+      for (int64_t l = 0; l < lim; ++l) {
+        int64_t j1 = c0 + l;
+        int64_t e = j1 * k2;
+        int64_t hi = e / c->K, lo = e % c->K;
+        T tr = c->hir[hi] * c->lor[lo] - c->hii[hi] * c->loi[lo];
+        T ti = c->hir[hi] * c->loi[lo] + c->hii[hi] * c->lor[lo];
+        T vr = rr[S * k2 + l], vi = ri[S * k2 + l];
+        c->zre[j1 + N1 * k2] = vr * tr - vi * ti;
+        c->zim[j1 + N1 * k2] = vr * ti + vi * tr;
+      }
+  }
+  free(buf);
+}
+
+static void F(fs_pass_b)(F(FsCtx) * c, int64_t glo, int64_t ghi) {
+  int64_t N1 = c->N1, N2 = c->N2, G = c->S;
+  T* buf = (T*)malloc(sizeof(T) * 4 * (size_t)(G * N1));
+  T *are = buf, *aim = buf + G * N1, *bre = buf + 2 * G * N1, *bim = buf + 3 * G * N1;
+  for (int64_t gr = glo; gr < ghi; ++gr) {
+    int64_t c2 = gr * G;
+    int64_t lim = N2 - c2 < G ? N2 - c2 : G;
+    for (int64_t g = 0; g < G; ++g)
+      for (int64_t j1 = 0; j1 < N1; ++j1) {
+        are[G * j1 + g] = g < lim ? c->zre[j1 + N1 * (c2 + g)] : 0;
+        aim[G * j1 + g] = g < lim ? c->zim[j1 + N1 * (c2 + g)] : 0;
+      }
+    int which = F(chain_run)(&c->chB, are, aim, bre, bim, G, c->D);
+    T* rr = which ? bre : are;
+    T* ri = which ? bim : aim;
+    for (int64_t k1 = 0; k1 < N1; ++k1)
+      for (int64_t g = 0; g < lim; ++g) {
+        int64_t e = c2 + g + N2 * k1;
+        c->y[2 * e] = rr[G * k1 + g] * c->scale;
+        c->y[2 * e + 1] = ri[G * k1 + g] * c->scale;
+      }
+  }
+  free(buf);
+}
+
+typedef struct {
+  F(FsCtx) * c;
+  int pass;
+  int64_t lo, hi;
+} F(FsJob);
+
+static void* F(fs_worker)(void* arg) {
+  F(FsJob)* j = (F(FsJob)*)arg;
+  if (j->pass == 0) F(fs_pass_a)(j->c, j->lo, j->hi);
+  else F(fs_pass_b)(j->c, j->lo, j->hi);
+  return NULL;
+}
+
+static void F(fs_run_pass)(F(FsCtx) * c, int pass, int64_t count, int nt) {
+  if (nt <= 1) {
+    if (pass == 0) F(fs_pass_a)(c, 0, count);
+    else F(fs_pass_b)(c, 0, count);
+    return;
+  }
+  pthread_t th[256];
+  F(FsJob) jobs[256];
+  if (nt > 256) nt = 256;
+  int used = 0;
+  for (int i = 0; i < nt; ++i) {
+    int64_t lo, hi;
+    chunk_range(count, nt, i, 1, &lo, &hi);
+    if (lo >= hi) continue;
+    jobs[used].c = c;
+    jobs[used].pass = pass;
+    jobs[used].lo = lo;
+    jobs[used].hi = hi;
+    pthread_create(&th[used], NULL, F(fs_worker), &jobs[used]);
+    ++used;
+  }
+  for (int i = 0; i < used; ++i) pthread_join(th[i], NULL);
+}
+
+static int F(fourstep)(int64_t n, int64_t N1, int64_t N2, int64_t S, int D, T scale, const T* x, T* y, int nt) {
+  if (N1 * N2 != n || S < 1) return -1;
+  F(FsCtx) c;
+  memset(&c, 0, sizeof(c));
+  c.n = n;
+  c.N1 = N1;
+  c.N2 = N2;
+  c.S = S;
+  c.D = D;
+  c.scale = scale;
+  c.x = x;
+  c.y = y;
+  if (F(chain_init)(&c.chA, N2, D) || F(chain_init)(&c.chB, N1, D)) return -1;
+  int64_t K = isqrt64(n - 1) + 1, nh = (n + K - 1) / K;
+  c.K = K;
+  c.hir = (T*)malloc(sizeof(T) * (size_t)nh);
+  c.hii = (T*)malloc(sizeof(T) * (size_t)nh);
+  c.lor = (T*)malloc(sizeof(T) * (size_t)K);
+  c.loi = (T*)malloc(sizeof(T) * (size_t)K);
+  for (int64_t i = 0; i < nh; ++i) {
+    long double cc, ss;
+    expi_exact(i * K, n, D, &cc, &ss);
+    c.hir[i] = (T)cc;
+    c.hii[i] = (T)ss;
+  }
+  for (int64_t i = 0; i < K; ++i) {
+    long double cc, ss;
+    expi_exact(i, n, D, &cc, &ss);
+    c.lor[i] = (T)cc;
+    c.loi[i] = (T)ss;
+  }
+  c.zre = (T*)malloc(sizeof(T) * (size_t)n);
+  c.zim = (T*)malloc(sizeof(T) * (size_t)n);
+  F(fs_run_pass)(&c, 0, (N1 + S - 1) / S, nt);
+  F(fs_run_pass)(&c, 1, (N2 + S - 1) / S, nt);
+  free(c.zre);
+  free(c.zim);
+  free(c.hir);
+  free(c.hii);
+  free(c.lor);
+  free(c.loi);
+  F(chain_free)(&c.chA);
+  F(chain_free)(&c.chB);
+  return 0;
+}
+
+/* -------------------------------------------------------------- resample */
+
+/* ResamplePlan (resample.jl:175-276) with table weights (:232-245) */
+static int F(resample)(int64_t n, int D, T scale, const T* x, T* y, long double tol) {
+  int64_t m = nextsmooth(2 * n);
+  double ratio = (double)n / (double)m;
+  if (ratio < 0.5) ratio = 0.5;
+  int w0 = (int)ceil(log(1.0 / (double)tol) / (M_PI * sqrt(1.0 - ratio)));
+  int w = 4 * ((w0 + 3) / 4);
+  int h = w / 2;
+  double beta = 0.97 * M_PI * w * (1.0 - (double)n / (2.0 * m));
+  int nq = 2 * w + 20;
+  double zq[128], wq[128];
+  gauss_legendre(nq, zq, wq);
+  int64_t hp = (n + 1) / 2;
+  T* corr = (T*)malloc(sizeof(T) * (size_t)n);
+  for (int64_t j = 0; j < n; ++j) {
+    int64_t jp = j < hp ? j : j - n;
+    double nu = (double)jp / (double)m;
+    double phi = 0.0;
+    for (int q = 0; q < nq; ++q)
+      if (zq[q] > 0) phi += wq[q] * es_kernel(zq[q], beta) * cos(M_PI * nu * w * zq[q]);
+    corr[j] = (T)((double)scale / (w * phi));
+  }
+  F(Chain) ch;
+  if (F(chain_init)(&ch, m, D)) {
+    free(corr);
+    return -1;
+  }
+  T* buf = (T*)malloc(sizeof(T) * 4 * (size_t)m);
+  T *are = buf, *aim = buf + m, *bre = buf + 2 * m, *bim = buf + 3 * m;
+  for (int64_t j = 0; j < m; ++j) are[j] = aim[j] = 0;
+  for (int64_t j = 0; j < hp; ++j) {
+    are[j] = x[2 * j] * corr[j];
+    aim[j] = x[2 * j + 1] * corr[j];
+  }
+  for (int64_t j = hp; j < n; ++j) {
+    int64_t o = m - n + j;
+    are[o] = x[2 * j] * corr[j];
+    aim[o] = x[2 * j + 1] * corr[j];
+  }
+  int which = F(chain_run)(&ch, are, aim, bre, bim, 1, D);
+  T* gr = which ? bre : are;
+  T* gi = which ? bim : aim;
+  for (int64_t k = 0; k < n; ++k) {
+    int64_t km = k * m, q = km / n, r = km % n;
+    int64_t l0 = q - h + 1;
+    double delta = (double)r / (double)n;
+    T re = 0, im = 0;
+    for (int i = 0; i < w; ++i) {
+      T wt = (T)es_kernel((delta + h - 1 - i) / h, beta);
+      int64_t l = ((l0 + i) % m + m) % m;
+      re += wt * gr[l];
+      im += wt * gi[l];
+    }
+    y[2 * k] = re;
+    y[2 * k + 1] = im;
+  }
+  free(buf);
+  free(corr);
+  F(chain_free)(&ch);
+  return 0;
+}
+
+/* 1-D routing for the oracle: a smooth length runs its stage chain, any
+ * other length the resampling path (plan.jl:192-206 without the CPU
+ * speed heuristics, which do not change the result beyond rounding) */
+int F(oracle_fft)(int64_t n, int D, T scale, const T* x, T* y) {
+  if (n < 1) return -1;
+  if (is_smooth13(n)) return F(fft_chain)(n, D, scale, x, y);
+  return F(resample)(n, D, scale, x, y, F(resample_tol)());
+}
+
+int F(oracle_fft_batch)(int64_t n, int64_t batch, int D, T scale, const T* x, T* y) {
+  for (int64_t b = 0; b < batch; ++b)
+    if (F(oracle_fft)(n, D, scale, x + 2 * n * b, y + 2 * n * b)) return -1;
+  return 0;
+}
+
+int F(oracle_fourstep)(int64_t n, int64_t N1, int64_t N2, int D, T scale, const T* x, T* y, int nt) {
+  return F(fourstep)(n, N1, N2, TW, D, scale, x, y, nt);
+}
+
+int F(oracle_resample)(int64_t n, int D, T scale, const T* x, T* y) {
+  return F(resample)(n, D, scale, x, y, F(resample_tol)());
+}
+
+/* ------------------------------------------------------------------ real */
+
+/* rfft (real.jl:80-142): even n packs to n/2 complex + twist, odd n full */
+int F(oracle_rfft)(int64_t n, const T* x, T* y) {
+  if (n < 1) return -1;
+  if (n % 2) {
+    T* xc = (T*)calloc(2 * (size_t)n, sizeof(T));
+    T* yc = (T*)malloc(sizeof(T) * 2 * (size_t)n);
+    for (int64_t j = 0; j < n; ++j) xc[2 * j] = x[j];
+    int rc = F(oracle_fft)(n, -1, (T)1, xc, yc);
+    memcpy(y, yc, sizeof(T) * 2 * (size_t)(n / 2 + 1));
+    free(xc);
+    free(yc);
+    return rc;
+  }
+  int64_t h = n / 2;
+  T* Z = (T*)malloc(sizeof(T) * 2 * (size_t)h);
+  if (F(oracle_fft)(h, -1, (T)1, x, Z)) {
+    free(Z);
+    return -1;
+  }
+  y[0] = Z[0] + Z[1];
+  y[1] = 0;
+  y[2 * h] = Z[0] - Z[1];
+  y[2 * h + 1] = 0;
+  for (int64_t k = 1; k < h; ++k) {
+    long double c, s;
+    expi_exact(k, n, -1, &c, &s);
+    T wr = (T)c, wi = (T)s;
+    T ar = Z[2 * k], ai = Z[2 * k + 1];
+    T br = Z[2 * (h - k)], bi = -Z[2 * (h - k) + 1];
+    T xer = (ar + br) * (T)0.5, xei = (ai + bi) * (T)0.5;
+    T xor_ = (ar - br) * (T)0.5, xoi = (ai - bi) * (T)0.5;
+    /* xe + w*(-i*xo), -i*xo = (xoi, -xor) */
+    T tr = xoi, ti = -xor_;
+    y[2 * k] = xer + (wr * tr - wi * ti);
+    y[2 * k + 1] = xei + (wr * ti + wi * tr);
+  }
+  free(Z);
+  return 0;
+}
+
+/* irfft / brfft (real.jl:144-211) */
+int F(oracle_irfft)(int64_t n, const T* X, T* y, int normalized) {
+  if (n < 1) return -1;
+  if (n % 2) {
+    int64_t h = n / 2;
+    T* buf = (T*)malloc(sizeof(T) * 2 * (size_t)n);
+    T* out = (T*)malloc(sizeof(T) * 2 * (size_t)n);
+    for (int64_t k = 0; k <= h; ++k) {
+      buf[2 * k] = X[2 * k];
+      buf[2 * k + 1] = X[2 * k + 1];
+    }
+    for (int64_t k = 1; k <= h; ++k) {
+      buf[2 * (n - k)] = X[2 * k];
+      buf[2 * (n - k) + 1] = -X[2 * k + 1];
+    }
+    int rc = F(oracle_fft)(n, 1, normalized ? (T)(1.0L / n) : (T)1, buf, out);
+    for (int64_t j = 0; j < n; ++j) y[j] = out[2 * j];
+    free(buf);
+    free(out);
+    return rc;
+  }
+  int64_t h = n / 2;
+  T pre = normalized ? (T)0.5 : (T)1;
+  T* Z = (T*)malloc(sizeof(T) * 2 * (size_t)h);
+  for (int64_t k = 0; k < h; ++k) {
+    long double c, s;
+    expi_exact(k, n, 1, &c, &s);
+    T wr = (T)c, wi = (T)s;
+    T ar = X[2 * k], ai = X[2 * k + 1];
+    T br = X[2 * (h - k)], bi = -X[2 * (h - k) + 1];
+    T xer = (ar + br) * pre, xei = (ai + bi) * pre;
+    T xdr = (ar - br) * pre, xdi = (ai - bi) * pre;
+    T tr = wr * xdr - wi * xdi, ti = wr * xdi + wi * xdr;
+    Z[2 * k] = xer - ti;
+    Z[2 * k + 1] = xei + tr;
+  }
+  int rc = F(oracle_fft)(h, 1, normalized ? (T)(1.0L / h) : (T)1, Z, y);
+  free(Z);
+  return rc;
+}
+
+/* ------------------------------------------------------------------- N-d */
+
+/* NdPlan (ndplan.jl:57-118, 270-327): one pass per dim of length > 1, each
+ * pencil transformed by the 1-D route; column-major, dims[0] fastest;
+ * the scale is applied on the last pass */
+int F(oracle_ndfft)(int rank, const int64_t* dims, int D, T scale, const T* x, T* y) {
+  int64_t len = 1;
+  for (int i = 0; i < rank; ++i) len *= dims[i];
+  memcpy(y, x, sizeof(T) * 2 * (size_t)len);
+  int last = -1;
+  for (int i = 0; i < rank; ++i)
+    if (dims[i] > 1) last = i;
+  if (last < 0) {
+    for (int64_t j = 0; j < 2 * len; ++j) y[j] *= scale;
+    return 0;
+  }
+  for (int ax = 0; ax < rank; ++ax) {
+    int64_t nd = dims[ax];
+    if (nd == 1) continue;
+    int64_t Bi = 1, Bo = 1;
+    for (int i = 0; i < ax; ++i) Bi *= dims[i];
+    for (int i = ax + 1; i < rank; ++i) Bo *= dims[i];
+    T* pin = (T*)malloc(sizeof(T) * 2 * (size_t)nd);
+    T* pout = (T*)malloc(sizeof(T) * 2 * (size_t)nd);
+    T sc = ax == last ? scale : (T)1;
+    for (int64_t bo = 0; bo < Bo; ++bo)
+      for (int64_t bi = 0; bi < Bi; ++bi) {
+        int64_t base = bi + bo * Bi * nd;
+        for (int64_t i = 0; i < nd; ++i) {
+          pin[2 * i] = y[2 * (base + i * Bi)];
+          pin[2 * i + 1] = y[2 * (base + i * Bi) + 1];
+        }
+        if (F(oracle_fft)(nd, D, sc, pin, pout)) {
+          free(pin);
+          free(pout);
+          return -1;
+        }
+        for (int64_t i = 0; i < nd; ++i) {
+          y[2 * (base + i * Bi)] = pout[2 * i];
+          y[2 * (base + i * Bi) + 1] = pout[2 * i + 1];
+        }
+      }
+    free(pin);
+    free(pout);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------- naive reference */
+
+/* X_k = sum_j x_j exp(D*2*pi*i*jk/n), exact integer angle reduction */
+int F(dft_naive)(int64_t n, int D, const T* x, T* y) {
+  for (int64_t k = 0; k < n; ++k) {
+    long double sr = 0, si = 0;
+    for (int64_t j = 0; j < n; ++j) {
+      long double c, s;
+      expi_exact((j * k) % n, n, D, &c, &s);
+      sr += c * x[2 * j] - s * x[2 * j + 1];
+      si += c * x[2 * j + 1] + s * x[2 * j];
+    }
+    y[2 * k] = (T)sr;
+    y[2 * k + 1] = (T)si;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------- threaded CPU baseline */
+
+/* the reference's loop of 1-D plans over a batch, spread over nt threads in
+ * contiguous chunks (threads.jl:15-38); one chain (and its twiddle tables)
+ * per thread, reused across that thread's transforms */
+typedef struct {
+  int64_t n, lo, hi;
+  int D;
+  T scale;
+  const T* x;
+  T* y;
+} F(BatchJob);
+
+static void* F(batch_worker)(void* arg) {
+  F(BatchJob)* j = (F(BatchJob)*)arg;
+  F(Chain) ch;
+  if (F(chain_init)(&ch, j->n, j->D)) return NULL;
+  int64_t n = j->n;
+  T* buf = (T*)malloc(sizeof(T) * 4 * (size_t)n);
+  T *are = buf, *aim = buf + n, *bre = buf + 2 * n, *bim = buf + 3 * n;
+  for (int64_t b = j->lo; b < j->hi; ++b) {
+    const T* x = j->x + 2 * n * b;
+    T* y = j->y + 2 * n * b;
+    for (int64_t i = 0; i < n; ++i) {
+      are[i] = x[2 * i];
+      aim[i] = x[2 * i + 1];
+    }
+    int which = F(chain_run)(&ch, are, aim, bre, bim, 1, j->D);
+    T* rr = which ? bre : are;
+    T* ri = which ? bim : aim;
+    for (int64_t i = 0; i < n; ++i) {
+      y[2 * i] = rr[i] * j->scale;
+      y[2 * i + 1] = ri[i] * j->scale;
+    }
+  }
+  free(buf);
+  F(chain_free)(&ch);
+  return NULL;
+}
+
+int F(baseline_fft_batch)(int64_t n, int64_t batch, int D, T scale, const T* x, T* y, int nt) {
+  if (!is_smooth13(n)) return -1;
+  if (nt < 1) nt = 1;
+  if (nt > 256) nt = 256;
+  pthread_t th[256];
+  F(BatchJob) jobs[256];
+  int used = 0;
+  for (int i = 0; i < nt; ++i) {
+    int64_t lo, hi;
+    chunk_range(batch, nt, i, 1, &lo, &hi);
+    if (lo >= hi) continue;
+    F(BatchJob) j = {n, lo, hi, D, scale, x, y};
+    jobs[used] = j;
+    pthread_create(&th[used], NULL, F(batch_worker), &jobs[used]);
+    ++used;
+  }
+  for (int i = 0; i < used; ++i) pthread_join(th[i], NULL);
+  return 0;
+}
+
+#undef F
+#undef CAT
+#undef CAT2

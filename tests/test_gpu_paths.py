@@ -1,0 +1,107 @@
+"""GPU parity: prime (resampling) plans, real plans, N-d plans, against the
+FP64 oracle on the same seeded inputs (runtests.jl:57-83, 233-262, 330-389),
+plus the BASELINE configuration 3 shapes at full size."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from conftest import F, cplx_np, inputs_c, real_np, rel_l2, to_dev, to_host, tol_for, torch_dtype
+
+pytestmark = pytest.mark.gpu
+
+# ------------------------------------------------------------- resampling (A9)
+
+PRIMES = [17, 19, 23, 34, 38, 51, 97, 101, 289, 331, 361, 997, 1026, 1234, 2050, 8191, 10006, 10007, 65542]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", PRIMES)
+def test_resampled(prec, n):
+    x = inputs_c(n, n, prec)
+    xd = to_dev(x)
+    y = to_host(F.plan_fft(torch_dtype(prec), n)(xd))
+    ref = O.fft(x.astype(np.complex128))
+    assert rel_l2(y, ref) <= tol_for(prec, n)
+    yi = to_host(F.plan_ifft(torch_dtype(prec), n)(to_dev(y)))
+    assert rel_l2(yi, x) <= 2 * tol_for(prec, n)
+    yb = to_host(F.plan_bfft(torch_dtype(prec), n)(to_dev(y)))
+    assert rel_l2(yb, n * x.astype(np.complex128)) <= 2 * tol_for(prec, n)
+
+
+@pytest.mark.parametrize("n", [65537, 1000003])
+def test_config3_primes(n):
+    batch = 4
+    x = inputs_c(0, n * batch, "f32").reshape(batch, n)
+    y = to_host(F.plan_fft(torch.float32, n, batch=batch)(to_dev(x))).reshape(batch, n)
+    rows = [0, 3]
+    ref = O.fft(x[rows].astype(np.complex128))
+    assert rel_l2(y[rows], ref) <= tol_for("f32", n)
+
+
+# ------------------------------------------------------------------ real (A10)
+
+REAL_N = [1, 2, 3, 4, 5, 6, 8, 11, 16, 22, 34, 64, 100, 255, 256, 257, 1024, 768, 1000, 4096, 2**14,
+          3 * 2**10, 7 * 2**8, 2**18 + 2, 10007 * 2]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", REAL_N)
+def test_rfft_irfft(prec, n):
+    x = F.rng.normal(n, n).astype(real_np(prec))
+    T = torch_dtype(prec)
+    y = to_host(F.plan_rfft(T, n)(to_dev(x)))
+    assert y.shape == (n // 2 + 1,)
+    ref = O.rfft(x.astype(np.float64))
+    assert rel_l2(y, ref) <= tol_for(prec, n)
+    xr = to_host(F.plan_irfft(T, n)(to_dev(y)))
+    assert rel_l2(xr, x) <= 2 * tol_for(prec, n)
+    yb = to_host(F.plan_brfft(T, n)(to_dev(ref.astype(cplx_np(prec)))))
+    assert rel_l2(yb, O.irfft(ref, n, normalized=False)) <= tol_for(prec, n)
+
+
+def test_config3_rfft_2pow22_x4():
+    n, batch = 1 << 22, 4
+    x = F.rng.normal(0, n * batch).astype(np.float32).reshape(batch, n)
+    p = F.plan_rfft(torch.float32, n, batch=batch)
+    yd = p(to_dev(x))
+    y = to_host(yd)
+    assert y.shape == (batch, n // 2 + 1)
+    ref = O.rfft(x[0].astype(np.float64))
+    assert rel_l2(y[0], ref) <= tol_for("f32", n)
+    back = to_host(F.plan_irfft(torch.float32, n, batch=batch)(yd))
+    assert rel_l2(back, x) <= 2 * tol_for("f32", n)
+
+
+# ------------------------------------------------------------------- N-d (A11)
+
+SHAPES = [(64, 48), (256, 256), (33, 17), (16, 11), (11, 16), (8, 1), (1, 8), (128, 3, 5), (16, 16, 16),
+          (12, 10, 14), (32, 64), (60, 50), (24, 40, 3), (1024, 512), (4, 2**12), (2**12, 4), (256, 33, 17)]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("dims", SHAPES)
+def test_nd(prec, dims):
+    n = int(np.prod(dims))
+    x = inputs_c(n, n, prec).reshape(tuple(reversed(dims)))
+    xd = to_dev(x)
+    p = F.plan_fft(xd)
+    assert F.size(p) == tuple(reversed(dims))
+    y = to_host(p * xd)
+    ref = O.ndfft(x.astype(np.complex128).ravel(), list(dims)).reshape(x.shape)
+    tol = 1e-13 if prec == "f64" else 200 * np.finfo(np.float32).eps   # runtests.jl:332 tier
+    assert rel_l2(y, ref) <= tol
+    back = to_host(F.plan_ifft(xd) * to_dev(y))
+    assert rel_l2(back, x) <= 2 * tol
+    yb = to_host(F.plan_bfft(xd) * to_dev(y))
+    assert rel_l2(yb, n * x.astype(np.complex128)) <= 2 * tol
+
+
+def test_config3_nd_256cubed():
+    dims = (256, 256, 256)
+    n = 256 ** 3
+    x = inputs_c(0, n, "f32").reshape(dims)
+    xd = to_dev(x)
+    y = to_host(F.fft(xd))
+    ref = O.ndfft(x.astype(np.complex128).ravel(), list(dims)).reshape(dims)
+    assert rel_l2(y, ref) <= 200 * np.finfo(np.float32).eps

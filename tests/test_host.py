@@ -1,0 +1,100 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every entry point include/fftjh.h declares, argument errors come back as
+the reference's exception types before any GPU work, and the planner's
+schedules for the benchmark configurations."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT, F
+
+_lib = F._lib
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "fftjh.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fftjh_\w+)\s*\(", src, re.M)))
+
+
+def test_exports_every_declared_symbol():
+    names = _declared()
+    assert len(names) >= 12
+    L = _lib.lib()
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(_lib.EXPORTS)
+    assert L.fftjh_version() >= 100
+
+
+def test_argument_errors_before_gpu_work():
+    # n = 0 -> ArgumentError (runtests.jl:424), raised on the host
+    with pytest.raises(F.ArgumentError, match="positive"):
+        F.plan_fft("f64", 0)
+    with pytest.raises(F.ArgumentError):
+        F.plan_rfft("f32", -3)
+    with pytest.raises(F.ArgumentError):
+        F.plan_fft("f16", 8)
+    with pytest.raises(F.ArgumentError, match="positive"):
+        _lib.describe_request(_lib.KIND_C2C, _lib.F32, [0])
+    with pytest.raises(F.ArgumentError):
+        _lib.describe_request(_lib.KIND_C2C, _lib.F32, [8], batch=0)
+    with pytest.raises(F.ArgumentError):
+        _lib.describe_request(_lib.KIND_C2C, 7, [8])
+
+
+def test_execute_errors_without_a_plan():
+    L = _lib.lib()
+    assert L.fftjh_execute(None, None, None, None) == _lib.ERR_ARGUMENT
+    assert "plan is NULL" in L.fftjh_last_error().decode()
+    assert L.fftjh_plan_destroy(None) == _lib.OK
+
+
+def _passes(desc):
+    return [ln.strip() for ln in desc.splitlines()[1:] if ln.strip()]
+
+
+def test_schedule_config1_single_launch():
+    d = _lib.describe_request(_lib.KIND_C2C, _lib.F64, [1024])
+    steps = _passes(d)
+    assert len(steps) == 1 and "n=1024" in steps[0] and "radices=4,16,16" in steps[0]
+
+
+def test_schedule_config2():
+    # (a) 4096 x 16384 contiguous: one launch, one transform per workgroup row
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [4096], batch=16384))
+    assert len(s) == 1 and "batch=16384" in s[0]
+    # (b) 2^24: four-step 4096 x 4096, twiddle fused into pass A
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1 << 24]))
+    assert len(s) == 2
+    assert "n=4096" in s[0] and "twiddle=w_16777216" in s[0] and "in_stride=4096" in s[0]
+    assert "n=4096" in s[1] and "in_stride=1" in s[1] and "out_stride=4096" in s[1]
+    # (c) 10^6 = 1000 x 1000
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [10**6], batch=8))
+    assert "n=1000" in s[0] and "n=1000" in s[1]
+
+
+def test_schedule_config3():
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [65537], batch=4))
+    assert s[0].startswith("rs_deconv") and "m=147456" in s[0] and s[-1].startswith("rs_interp")
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1000003], batch=4))
+    assert "m=2097152" in s[0]
+    s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F32, [1 << 22], batch=4))
+    assert s[-1].startswith("r2c_twist")
+    s = _passes(_lib.describe_request(_lib.KIND_C2R, _lib.F32, [1 << 22], batch=4))
+    assert s[0].startswith("c2r_pre")
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [256, 256, 256]))
+    assert len(s) == 3 and "in_stride=1 " in s[0] and "in_stride=256 " in s[1] and "in_stride=65536 " in s[2]
+
+
+def test_nd_unsupported_axis_is_reported():
+    with pytest.raises(F.UnsupportedError):
+        _lib.describe_request(_lib.KIND_C2C, _lib.F32, [1 << 20, 2])
+
+
+def test_region_check():
+    import torch
+    x = torch.zeros(4, 4, dtype=torch.complex64)
+    with pytest.raises(F.ArgumentError, match="all dims"):
+        F.plan_fft(x, region=(1,))
