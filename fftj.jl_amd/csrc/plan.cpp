@@ -157,10 +157,11 @@ enum StepKind {
   ST_TAKE_REAL,
   ST_RS_DECONV,
   ST_RS_INTERP,
+  ST_STRIDED_COPY,
 };
 
-static const char* kStepNames[] = {"pass",        "r2c_twist",  "c2r_pre",   "real_to_complex", "copy_prefix",
-                                   "hermitian_fill", "take_real", "rs_deconv", "rs_interp"};
+static const char* kStepNames[] = {"pass",           "r2c_twist", "c2r_pre",   "real_to_complex", "copy_prefix",
+                                   "hermitian_fill", "take_real", "rs_deconv", "rs_interp",       "strided_copy"};
 
 struct Step {
   int kind = ST_PASS;
@@ -174,6 +175,9 @@ struct Step {
   int64_t n = 0, m = 0, h = 0, batch = 0, s1 = 0, s2 = 0, keep = 0;
   int w = 0;
   double beta = 0.0, pre = 1.0, scale = 1.0;
+  // strided sides of rs_deconv (ga), rs_interp (gb), strided_copy (ga -> gb)
+  int64_t Bi = 1;
+  Geometry ga{1, 0, 0}, gb{1, 0, 0};
   // device resources
   DevBufPtr tw, thi, tlo, desc, aux;
 };
@@ -344,16 +348,44 @@ static void gauss_legendre(int nq, std::vector<double>& z, std::vector<double>& 
   }
 }
 
-// Complex transform of `batch` contiguous length-n rows, src -> dst, using
+static bool is_rows(const Geometry& g, int64_t n, int64_t Bi) { return Bi == 1 && g.stride == 1 && g.s_out == n; }
+
+// Complex transform of `batch` length-n transforms laid out at geometry gin
+// (read from src) and gout (written to dst), batch index split by Bi, using
 // the scratch roles in `free` (front first).
-static void build_c2c_rows(Layout& L, int64_t n, int64_t batch, int dir, double scale, int src, int dst,
-                           std::vector<int> free) {
+static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry gin, Geometry gout, int dir,
+                      double scale, int src, int dst, std::vector<int> free) {
   if (single_pass_ok(L.dtype, n)) {
-    add_pass(L, dir, static_cast<int>(n), batch, 1, geo(1, 0, n), geo(1, 0, n), scale, src, dst, 0);
+    add_pass(L, dir, static_cast<int>(n), batch, Bi, gin, gout, scale, src, dst, 0);
     return;
   }
   int a, cnt[5];
+  const Geometry rows = geo(1, 0, n);
   if (factor13(n, a, cnt)) {
+    if (!is_rows(gin, n, Bi) || !is_rows(gout, n, Bi)) {
+      // a long strided axis: gather into rows, transform, scatter back
+      if (free.size() < 2) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for a strided long axis");
+      const int r = free[0];
+      need_scratch(L, r, n * batch);
+      Step g;
+      g.kind = ST_STRIDED_COPY;
+      g.src = src;
+      g.dst = r;
+      g.n = n;
+      g.batch = batch;
+      g.Bi = Bi;
+      g.ga = gin;
+      g.gb = rows;
+      L.steps.push_back(g);
+      build_c2c(L, n, batch, 1, rows, rows, dir, scale, r, r, std::vector<int>(free.begin() + 1, free.end()));
+      Step s2 = g;
+      s2.src = r;
+      s2.dst = dst;
+      s2.ga = rows;
+      s2.gb = gout;
+      L.steps.push_back(s2);
+      return;
+    }
     int64_t N1, N2;
     if (!fourstep_split(L.dtype, n, N1, N2))
       fail(FFTJH_ERR_UNSUPPORTED, "no two-pass split of length " + std::to_string(n));
@@ -382,9 +414,11 @@ static void build_c2c_rows(Layout& L, int64_t n, int64_t batch, int dir, double 
   dc.w = rp.w;
   dc.beta = rp.beta;
   dc.scale = scale;
+  dc.Bi = Bi;
+  dc.ga = gin;
   L.steps.push_back(dc);
   std::vector<int> rest(free.begin() + 1, free.end());
-  build_c2c_rows(L, rp.m, batch, dir, 1.0, g, g, rest);
+  build_c2c(L, rp.m, batch, 1, geo(1, 0, rp.m), geo(1, 0, rp.m), dir, 1.0, g, g, rest);
   Step ip;
   ip.kind = ST_RS_INTERP;
   ip.src = g;
@@ -395,7 +429,14 @@ static void build_c2c_rows(Layout& L, int64_t n, int64_t batch, int dir, double 
   ip.batch = batch;
   ip.w = rp.w;
   ip.beta = rp.beta;
+  ip.Bi = Bi;
+  ip.gb = gout;
   L.steps.push_back(ip);
+}
+
+static void build_c2c_rows(Layout& L, int64_t n, int64_t batch, int dir, double scale, int src, int dst,
+                           std::vector<int> free) {
+  build_c2c(L, n, batch, 1, geo(1, 0, n), geo(1, 0, n), dir, scale, src, dst, std::move(free));
 }
 
 static void check_dtype(int dtype) {
@@ -446,10 +487,8 @@ static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch
     for (int i = ax + 1; i < rank; ++i) Bo *= dims[i];
     const int64_t nd = dims[ax];
     const double sc = (ai + 1 == axes.size()) ? scale : 1.0;
-    if (!single_pass_ok(dtype, nd))
-      fail(FFTJH_ERR_UNSUPPORTED, "N-d axis length " + std::to_string(nd) +
-                                      " needs a single-pass 13-smooth length in this build");
-    add_pass(L, sign, static_cast<int>(nd), Bi * Bo, Bi, geo(Bi, 1, Bi * nd), geo(Bi, 1, Bi * nd), sc, src, RY, 0);
+    build_c2c(L, nd, Bi * Bo, Bi, geo(Bi, 1, Bi * nd), geo(Bi, 1, Bi * nd), sign, sc, src, RY,
+              {RS0, RS1, RS2, RS3});
     src = RY;
   }
   return L;
@@ -693,9 +732,11 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
     case ST_TAKE_REAL:
       return launch_take_real(dt, ptr[s.src], ptr[s.dst], s.n * s.batch, st);
     case ST_RS_DECONV:
-      return launch_rs_deconv(dt, ptr[s.src], ptr[s.dst], s.aux->p, s.n, s.m, s.batch, st);
+      return launch_rs_deconv(dt, ptr[s.src], ptr[s.dst], s.aux->p, s.n, s.m, s.batch, s.Bi, s.ga, st);
     case ST_RS_INTERP:
-      return launch_rs_interp(dt, ptr[s.src], ptr[s.dst], s.n, s.m, s.w, s.beta, s.batch, st);
+      return launch_rs_interp(dt, ptr[s.src], ptr[s.dst], s.n, s.m, s.w, s.beta, s.batch, s.Bi, s.gb, st);
+    case ST_STRIDED_COPY:
+      return launch_strided_copy(dt, ptr[s.src], ptr[s.dst], s.n, s.batch, s.Bi, s.ga, s.gb, st);
   }
   return hipErrorInvalidValue;
 }

@@ -49,9 +49,15 @@ __global__ void k_rs_corr(T* corr, int64_t n, int64_t m, int w, double beta, con
   }
 }
 
+// transform b of a strided batch starts at (b % Bi)*s_in + (b / Bi)*s_out
+__device__ __forceinline__ int64_t geo_base(const Geometry& g, int64_t Bi, int64_t b) {
+  return (b % Bi) * g.s_in + (b / Bi) * g.s_out;
+}
+
+// x (transforms at geometry gx) -> g (contiguous rows of m)
 template <typename T>
 __global__ void k_rs_deconv(const V2<T>* x, V2<T>* g, const T* __restrict__ corr, int64_t n, int64_t m,
-                            int64_t batch) {
+                            int64_t batch, int64_t Bi, Geometry gx) {
   const int64_t hp = (n + 1) / 2;
   const int64_t hi0 = m - n + hp;
   GRID_STRIDE(idx, batch * m) {
@@ -60,14 +66,15 @@ __global__ void k_rs_deconv(const V2<T>* x, V2<T>* g, const T* __restrict__ corr
     if (j < hp) src = j;
     else if (j >= hi0) src = j - (m - n);
     V2<T> v = mk<T>(T(0), T(0));
-    if (src >= 0) v = cscale<T>(x[b * n + src], corr[src]);
+    if (src >= 0) v = cscale<T>(x[geo_base(gx, Bi, b) + src * gx.stride], corr[src]);
     g[idx] = v;
   }
 }
 
+// G (contiguous rows of m) -> y (transforms at geometry gy)
 template <typename T>
 __global__ void k_rs_interp(const V2<T>* G, V2<T>* y, int64_t n, int64_t m, int w, double beta,
-                            int64_t batch) {
+                            int64_t batch, int64_t Bi, Geometry gy) {
   const int h = w / 2;
   GRID_STRIDE(idx, batch * n) {
     const int64_t b = idx / n, k = idx - b * n;
@@ -86,7 +93,18 @@ __global__ void k_rs_interp(const V2<T>* G, V2<T>* y, int64_t n, int64_t m, int 
       acc.x = fma(wt, gv.x, acc.x);
       acc.y = fma(wt, gv.y, acc.y);
     }
-    y[idx] = acc;
+    y[geo_base(gy, Bi, b) + k * gy.stride] = acc;
+  }
+}
+
+// dst[transform b, element i] = src[transform b, element i] between two
+// strided geometries (gather a strided axis into rows and back)
+template <typename T>
+__global__ void k_strided_copy(const V2<T>* src, V2<T>* dst, int64_t n, int64_t batch, int64_t Bi, Geometry gs,
+                               Geometry gd) {
+  GRID_STRIDE(idx, batch * n) {
+    const int64_t b = idx / n, i = idx - b * n;
+    dst[geo_base(gd, Bi, b) + i * gd.stride] = src[geo_base(gs, Bi, b) + i * gs.stride];
   }
 }
 
@@ -103,26 +121,38 @@ hipError_t launch_rs_corr(int dtype, void* corr, int64_t n, int64_t m, int w, do
 }
 
 hipError_t launch_rs_deconv(int dtype, const void* x, void* g, const void* corr, int64_t n, int64_t m,
-                            int64_t batch, hipStream_t st) {
+                            int64_t batch, int64_t Bi, Geometry gx, hipStream_t st) {
   const dim3 gr = grid_1d(batch * m);
   if (dtype == 0)
     hipLaunchKernelGGL(k_rs_deconv<float>, gr, dim3(256), 0, st, static_cast<const float2*>(x),
-                       static_cast<float2*>(g), static_cast<const float*>(corr), n, m, batch);
+                       static_cast<float2*>(g), static_cast<const float*>(corr), n, m, batch, Bi, gx);
   else
     hipLaunchKernelGGL(k_rs_deconv<double>, gr, dim3(256), 0, st, static_cast<const double2*>(x),
-                       static_cast<double2*>(g), static_cast<const double*>(corr), n, m, batch);
+                       static_cast<double2*>(g), static_cast<const double*>(corr), n, m, batch, Bi, gx);
   return hipGetLastError();
 }
 
 hipError_t launch_rs_interp(int dtype, const void* G, void* y, int64_t n, int64_t m, int w, double beta,
-                            int64_t batch, hipStream_t st) {
+                            int64_t batch, int64_t Bi, Geometry gy, hipStream_t st) {
   const dim3 gr = grid_1d(batch * n);
   if (dtype == 0)
     hipLaunchKernelGGL(k_rs_interp<float>, gr, dim3(256), 0, st, static_cast<const float2*>(G),
-                       static_cast<float2*>(y), n, m, w, beta, batch);
+                       static_cast<float2*>(y), n, m, w, beta, batch, Bi, gy);
   else
     hipLaunchKernelGGL(k_rs_interp<double>, gr, dim3(256), 0, st, static_cast<const double2*>(G),
-                       static_cast<double2*>(y), n, m, w, beta, batch);
+                       static_cast<double2*>(y), n, m, w, beta, batch, Bi, gy);
+  return hipGetLastError();
+}
+
+hipError_t launch_strided_copy(int dtype, const void* src, void* dst, int64_t n, int64_t batch, int64_t Bi,
+                               Geometry gs, Geometry gd, hipStream_t st) {
+  const dim3 gr = grid_1d(batch * n);
+  if (dtype == 0)
+    hipLaunchKernelGGL(k_strided_copy<float>, gr, dim3(256), 0, st, static_cast<const float2*>(src),
+                       static_cast<float2*>(dst), n, batch, Bi, gs, gd);
+  else
+    hipLaunchKernelGGL(k_strided_copy<double>, gr, dim3(256), 0, st, static_cast<const double2*>(src),
+                       static_cast<double2*>(dst), n, batch, Bi, gs, gd);
   return hipGetLastError();
 }
 

@@ -110,7 +110,8 @@ def test_config2a_4096x16384():
     back = to_host(pi(yd))
     assert rel_l2(back, x) <= 2 * tol_for("f32", n)
     # Parseval over the whole batch
-    assert abs(np.vdot(y, y).real / (n * np.vdot(x, x).real) - 1) < 1e-5
+    y64, x64 = y.astype(np.complex128), x.astype(np.complex128)
+    assert abs(np.vdot(y64, y64).real / (n * np.vdot(x64, x64).real) - 1) < 1e-5
 
 
 def test_config2b_2pow24():

@@ -88,9 +88,13 @@ def test_schedule_config3():
     assert len(s) == 3 and "in_stride=1 " in s[0] and "in_stride=256 " in s[1] and "in_stride=65536 " in s[2]
 
 
-def test_nd_unsupported_axis_is_reported():
-    with pytest.raises(F.UnsupportedError):
-        _lib.describe_request(_lib.KIND_C2C, _lib.F32, [1 << 20, 2])
+def test_nd_long_and_nonsmooth_axes():
+    # a long strided axis is gathered into rows, transformed, scattered back
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [2, 1 << 20]))
+    assert s[1].startswith("strided_copy") and s[-1].startswith("strided_copy")
+    # a non-smooth axis resamples with strided deconvolution / interpolation
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [33, 17]))
+    assert s[1].startswith("rs_deconv") and s[-1].startswith("rs_interp")
 
 
 def test_region_check():
