@@ -18,8 +18,31 @@ KIND_C2C, KIND_R2C, KIND_C2R = 0, 1, 2
 EXPORTS = (
     "fftjh_plan_c2c", "fftjh_plan_r2c", "fftjh_plan_c2r", "fftjh_execute", "fftjh_execute_checked",
     "fftjh_plan_info", "fftjh_plan_describe", "fftjh_describe_request", "fftjh_plan_destroy",
-    "fftjh_last_error", "fftjh_version", "fftjh_profile_launches",
+    "fftjh_last_error", "fftjh_version", "fftjh_profile_launches", "fftjh_plan_c2c_tuned",
 )
+
+
+class PassTuning(ctypes.Structure):
+    _fields_ = [("C", ctypes.c_int32), ("nradix", ctypes.c_int32), ("radices", ctypes.c_int32 * 16),
+                ("xcd_remap", ctypes.c_int32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("split", ctypes.c_int64), ("pass_", PassTuning * 4)]
+
+
+def make_tuning(split=0, passes=()):
+    """passes: up to 4 dicts with optional keys C, radices, xcd_remap (0 planner, 1 off, 2 on)."""
+    t = Tuning()
+    t.split = int(split)
+    for i, p in enumerate(passes):
+        t.pass_[i].C = int(p.get("C", 0))
+        rad = p.get("radices") or []
+        t.pass_[i].nradix = len(rad)
+        for j, r in enumerate(rad):
+            t.pass_[i].radices[j] = int(r)
+        t.pass_[i].xcd_remap = int(p.get("xcd_remap", 0))
+    return t
 
 
 class ArgumentError(ValueError):
@@ -54,6 +77,8 @@ def lib():
         i64, vp, ci = ctypes.c_int64, ctypes.c_void_p, ctypes.c_int
         P64 = ctypes.POINTER(i64)
         L.fftjh_plan_c2c.argtypes = [ctypes.POINTER(vp), ci, ci, P64, i64, ci, ctypes.c_double, ctypes.c_uint]
+        L.fftjh_plan_c2c_tuned.argtypes = [ctypes.POINTER(vp), ci, ci, P64, i64, ci, ctypes.c_double,
+                                           ctypes.POINTER(Tuning)]
         L.fftjh_plan_r2c.argtypes = [ctypes.POINTER(vp), ci, i64, i64, ctypes.c_uint]
         L.fftjh_plan_c2r.argtypes = [ctypes.POINTER(vp), ci, i64, i64, ci, ctypes.c_uint]
         L.fftjh_execute.argtypes = [vp, vp, vp, vp]

@@ -103,8 +103,9 @@ class FFTPlan(_PlanBase):
     """Complex plan: 1-D (the reference's FFTPlan/FourStepPlan/ResamplePlan,
     chosen by the GPU planner) or N-d (NdPlan).  Fields n, sign, scale."""
 
-    def __init__(self, dtype, dims, sign, scale, batch=1, shape=None):
+    def __init__(self, dtype, dims, sign, scale, batch=1, shape=None, tuning=None):
         self.dtype_code = dtype
+        self.tuning = tuning
         self.dims = tuple(int(d) for d in dims)  # column-major, dims[0] fastest
         self.sign = int(sign)
         self.scale = float(scale)
@@ -113,8 +114,12 @@ class FFTPlan(_PlanBase):
         self.shape = tuple(shape) if shape is not None else (
             (self.n,) if len(self.dims) == 1 else tuple(reversed(self.dims)))
         h = ctypes.c_void_p()
-        check(lib().fftjh_plan_c2c(ctypes.byref(h), dtype, len(self.dims), dims_array(self.dims), self.batch,
-                                   self.sign, self.scale, 0))
+        if tuning is None:
+            check(lib().fftjh_plan_c2c(ctypes.byref(h), dtype, len(self.dims), dims_array(self.dims), self.batch,
+                                       self.sign, self.scale, 0))
+        else:
+            check(lib().fftjh_plan_c2c_tuned(ctypes.byref(h), dtype, len(self.dims), dims_array(self.dims),
+                                             self.batch, self.sign, self.scale, ctypes.byref(tuning)))
         self._h = _Handle(h.value)
         self._scaled = {}
 
@@ -148,11 +153,11 @@ class FFTPlan(_PlanBase):
         key = float(s)
         if key not in self._scaled:
             self._scaled[key] = FFTPlan(self.dtype_code, self.dims, self.sign, self.scale * key, self.batch,
-                                        self.shape)
+                                        self.shape, self.tuning)
         return self._scaled[key]
 
     def _sibling(self, sign, scale):
-        return FFTPlan(self.dtype_code, self.dims, sign, scale, self.batch, self.shape)
+        return FFTPlan(self.dtype_code, self.dims, sign, scale, self.batch, self.shape, self.tuning)
 
     def __repr__(self):
         return f"FFTPlan(dims={self.dims}, sign={self.sign}, scale={self.scale}, batch={self.batch})"

@@ -183,6 +183,8 @@ struct Step {
 };
 
 struct Layout {
+  fftjh_tuning tune{};  // search-space overrides (zero = planner's choice)
+  int npass = 0;        // pass launches laid out so far (index into tune.pass)
   int dtype = 0, kind = FFTJH_KIND_C2C, sign = -1, rank = 1;
   int64_t dims[8] = {0};
   int64_t batch = 1;
@@ -216,21 +218,44 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   d.in = in;
   d.out = out;
   d.scale = scale;
+  const fftjh_pass_tuning* pt = L.npass < 4 ? &L.tune.pass[L.npass] : nullptr;
+  ++L.npass;
   s.radices = (n == 1) ? std::vector<int>{} : default_radices(n);
+  if (pt && pt->nradix > 0) {
+    if (pt->nradix > kMaxStages) fail(FFTJH_ERR_ARGUMENT, "too many radices in tuning");
+    std::vector<int> r(pt->radices, pt->radices + pt->nradix);
+    int64_t prod = 1;
+    for (int R : r) {
+      if (R != 2 && R != 3 && R != 4 && R != 5 && R != 7 && R != 8 && R != 11 && R != 13 && R != 16)
+        fail(FFTJH_ERR_ARGUMENT, "tuning radix " + std::to_string(R) + " has no kernel");
+      prod *= R;
+    }
+    if (prod != n) fail(FFTJH_ERR_ARGUMENT, "tuning radices do not multiply to the pass length");
+    s.radices = r;
+  }
   if (n > 1 && s.radices.empty()) fail(FFTJH_ERR_UNSUPPORTED, "pass length has a prime factor above 13");
   if (static_cast<int>(s.radices.size()) > kMaxStages) fail(FFTJH_ERR_UNSUPPORTED, "too many stages");
   // transforms per workgroup: strided passes take adjacent columns for
   // contiguous HBM runs; row passes group short transforms to fill a workgroup
   const bool colmode = in.stride > 1 || out.stride > 1;
   int bestC = 0, bestT = 0;
-  for (int C = colmode ? 16 : 64; C >= 1; C >>= 1) {
+  if (pt && pt->C > 0) {
     int t;
-    if (!pass_fits(L.dtype, n, C, s.radices, &t)) continue;
-    if (C > 1 && C > batch) continue;
-    if (!colmode && t > 256 && C > 1) continue;
-    bestC = C;
+    if ((pt->C & (pt->C - 1)) || pt->C > 64) fail(FFTJH_ERR_ARGUMENT, "tuning C must be a power of two <= 64");
+    if (!pass_fits(L.dtype, n, pt->C, s.radices, &t))
+      fail(FFTJH_ERR_ARGUMENT, "tuning C=" + std::to_string(pt->C) + " does not fit one workgroup");
+    bestC = pt->C;
     bestT = t;
-    break;
+  } else {
+    for (int C = colmode ? 16 : 64; C >= 1; C >>= 1) {
+      int t;
+      if (!pass_fits(L.dtype, n, C, s.radices, &t)) continue;
+      if (C > 1 && C > batch) continue;
+      if (!colmode && t > 256 && C > 1) continue;
+      bestC = C;
+      bestT = t;
+      break;
+    }
   }
   if (!bestC) fail(FFTJH_ERR_UNSUPPORTED, "pass does not fit one workgroup");
   d.C = bestC;
@@ -241,6 +266,7 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   if (groups > 0x7fffffff) fail(FFTJH_ERR_UNSUPPORTED, "batch too large for one launch");
   d.ngroups = static_cast<int>(groups);
   d.xcd_remap = colmode ? 1 : 0;
+  if (pt && pt->xcd_remap > 0) d.xcd_remap = pt->xcd_remap == 2 ? 1 : 0;
   int ns = 1, off = 0;
   d.nstages = static_cast<int>(s.radices.size());
   for (size_t i = 0; i < s.radices.size(); ++i) {
@@ -375,20 +401,29 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
       g.batch = batch;
       g.Bi = Bi;
       g.ga = gin;
-      g.gb = rows;
+      // row b*n under the two-level index b = (b % Bi) + Bi*(b / Bi)
+      const Geometry rows_bi = geo(1, n, Bi * n);
+      g.gb = rows_bi;
       L.steps.push_back(g);
       build_c2c(L, n, batch, 1, rows, rows, dir, scale, r, r, std::vector<int>(free.begin() + 1, free.end()));
       Step s2 = g;
       s2.src = r;
       s2.dst = dst;
-      s2.ga = rows;
+      s2.ga = rows_bi;
       s2.gb = gout;
       L.steps.push_back(s2);
       return;
     }
     int64_t N1, N2;
-    if (!fourstep_split(L.dtype, n, N1, N2))
+    if (L.tune.split > 0 && L.npass == 0) {
+      N2 = L.tune.split;
+      if (n % N2 || !single_pass_ok(L.dtype, N2) || !single_pass_ok(L.dtype, n / N2))
+        fail(FFTJH_ERR_ARGUMENT, "tuning split " + std::to_string(N2) + " is not a two-pass split of " +
+                                     std::to_string(n));
+      N1 = n / N2;
+    } else if (!fourstep_split(L.dtype, n, N1, N2)) {
       fail(FFTJH_ERR_UNSUPPORTED, "no two-pass split of length " + std::to_string(n));
+    }
     if (free.empty()) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for four-step");
     const int z = free[0];
     need_scratch(L, z, n * batch);
@@ -444,7 +479,8 @@ static void check_dtype(int dtype) {
     fail(FFTJH_ERR_ARGUMENT, "dtype must be FFTJH_F32 or FFTJH_F64");
 }
 
-static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch, int sign, double scale) {
+static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch, int sign, double scale,
+                         const fftjh_tuning* tune = nullptr) {
   check_dtype(dtype);
   if (rank < 1 || rank > 8) fail(FFTJH_ERR_ARGUMENT, "rank must be in 1..8");
   if (!dims) fail(FFTJH_ERR_ARGUMENT, "dims is NULL");
@@ -458,6 +494,7 @@ static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch
     len *= dims[i];
   }
   Layout L;
+  if (tune) L.tune = *tune;
   L.dtype = dtype;
   L.kind = FFTJH_KIND_C2C;
   L.sign = sign;
@@ -787,6 +824,14 @@ int fftjh_plan_c2c(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, i
     if (plan) *plan = nullptr;
     if (flags) fail(FFTJH_ERR_ARGUMENT, "flags must be 0");
     make_plan(plan, layout_c2c(dtype, rank, dims, batch, sign, scale));
+  });
+}
+
+int fftjh_plan_c2c_tuned(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
+                         double scale, const fftjh_tuning* tune) {
+  return guarded([&] {
+    if (plan) *plan = nullptr;
+    make_plan(plan, layout_c2c(dtype, rank, dims, batch, sign, scale, tune));
   });
 }
 

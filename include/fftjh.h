@@ -61,6 +61,26 @@ typedef void* fftjh_stream;
 int fftjh_plan_c2c(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
                    double scale, unsigned flags);
 
+/* Search-space overrides for one plan (zero fields keep the planner's
+ * choice, so a zero-initialized struct is the default plan).  The exhaustive planner and the autotuner enumerate
+ * these; the reference has no counterpart (its routing is a fixed tree of
+ * host thresholds, plan.jl:183-213, and FFTW flags are ignored). */
+typedef struct fftjh_pass_tuning {
+  int32_t C;            /* transforms per workgroup, a power of two <= 64 */
+  int32_t nradix;       /* > 0: use radices[0..nradix) as the pass's stage chain */
+  int32_t radices[16];  /* each in {2,3,4,5,7,8,11,13,16}, product = pass length */
+  int32_t xcd_remap;    /* 0 planner, 1 off, 2 on */
+} fftjh_pass_tuning;
+
+typedef struct fftjh_tuning {
+  int64_t split;                /* four-step column length N2 of a 1-D plan */
+  fftjh_pass_tuning pass[4];    /* by launch order of the pass kernels */
+} fftjh_tuning;
+
+/* fftjh_plan_c2c with explicit search-space choices (NULL = planner's). */
+int fftjh_plan_c2c_tuned(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
+                         double scale, const fftjh_tuning* tune);
+
 /* Real-to-complex forward plan: n reals -> n/2+1 complex per transform.
  * Replaces plan_rfft(::Type{T}, n)                        real.jl:41-52 */
 int fftjh_plan_r2c(fftjh_plan* plan, int dtype, int64_t n, int64_t batch, unsigned flags);
