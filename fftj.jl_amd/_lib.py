@@ -8,7 +8,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libfftjh.so")
+# FFTJH_LIB names another build of the same library (kernel variants)
+LIB_PATH = os.environ.get("FFTJH_LIB") or os.path.join(HERE, "lib", "libfftjh.so")
 
 OK, ERR_ARGUMENT, ERR_DIMENSION, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
@@ -28,13 +29,14 @@ class PassTuning(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("split", ctypes.c_int64), ("pass_", PassTuning * 4)]
+    _fields_ = [("splits", ctypes.c_int64 * 4), ("pass_", PassTuning * 4)]
 
 
-def make_tuning(split=0, passes=()):
+def make_tuning(splits=(), passes=()):
     """passes: up to 4 dicts with optional keys C, radices, xcd_remap (0 planner, 1 off, 2 on)."""
     t = Tuning()
-    t.split = int(split)
+    for i, f in enumerate(splits or ()):
+        t.splits[i] = int(f)
     for i, p in enumerate(passes):
         t.pass_[i].C = int(p.get("C", 0))
         rad = p.get("radices") or []

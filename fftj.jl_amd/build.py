@@ -24,7 +24,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["stockham.hip", "twiddle.hip", "real.hip", "resample.hip", "plan.cpp"]
+SOURCES = ["stockham.hip", "stockham_pow2.hip", "twiddle.hip", "real.hip", "resample.hip", "plan.cpp"]
 
 
 def _compile(src):

@@ -28,11 +28,15 @@ struct StageDesc {
   uint32_t ns_magic;  // floor(2^32/ns)+1: j/ns == umulhi(j, magic) for j*ns < 2^32
 };
 
-// element i of transform b lives at  (b % Bi)*s_in + (b / Bi)*s_out + i*stride
+// element i of transform b lives at
+//   (b % Bi)*s_in + ((b / Bi) % B2)*s_mid + (b / (Bi*B2))*s_out + i*stride
+// (Bi and B2 belong to the pass; kernels outside the pass kernels use the
+// two-level form (b % Bi)*s_in + (b / Bi)*s_out)
 struct Geometry {
   int64_t stride;
   int64_t s_in;
   int64_t s_out;
+  int64_t s_mid;
 };
 
 struct PassDesc {
@@ -40,20 +44,26 @@ struct PassDesc {
   int32_t nstages;
   StageDesc st[kMaxStages];
   int64_t batch;     // transforms in the launch
-  int64_t Bi;        // inner batch count of the two-level batch index
+  int64_t Bi;        // inner batch count of the three-level batch index
+  int64_t B2;        // middle batch count
   int32_t C;         // transforms per workgroup (power of two)
   int32_t logC;
   int32_t nthreads;  // threads per workgroup (multiple of 64 and of C)
   int32_t ngroups;   // workgroups in the launch
   Geometry in;
   Geometry out;
-  // four-step twiddle at the store: output k of transform b is multiplied by
-  // w_N^{(b % Bi) * k} = hi[e >> lo_bits] * lo[e & (2^lo_bits - 1)]
-  int32_t post_tw;
-  int32_t lo_bits;
+  // multi-pass twiddle at the store: output k of transform b is multiplied
+  // by w_P^e, e = q*(tw_acoef*((b / Bi) % B2) + tw_bcoef*k), q = (b % Bi) / tw_qdiv,
+  // P = tw_period (0 = none); evaluated in FP64 from exact integers
+  int64_t tw_period;
+  int64_t tw_qdiv;
+  int64_t tw_acoef;
+  int64_t tw_bcoef;
   double scale;      // applied at the store
   int32_t xcd_remap;  // map consecutive logical groups onto one XCD
-  int32_t pad_;
+  int32_t grid;       // workgroups launched (persistent: each walks several groups)
+  int32_t rows_in;    // power-of-two kernel: stage HBM rows through LDS on the way in
+  int32_t rows_out;   // ... and on the way out (row-major lanes, contiguous runs)
 };
 
 // LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)

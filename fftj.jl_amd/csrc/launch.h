@@ -9,9 +9,18 @@ namespace fftjh {
 
 // d: host copy (launch geometry); dev_desc: the same descriptor in device memory
 hipError_t launch_pass(int dtype, int dir, const PassDesc& d, const PassDesc* dev_desc, const void* x,
-                       void* y, const void* tw, const void* thi, const void* tlo, hipStream_t stream);
+                       void* y, const void* tw, hipStream_t stream);
 // largest workgroup the pass kernel is compiled for
 int max_threads(int dtype);
+// workgroups of this pass that fit the device at once (0 if unknown)
+int pass_resident_groups(int dtype, int dir, const PassDesc& d);
+
+// compile-time-chain kernels for power-of-two lengths 256..16384
+bool pow2_chain(int n, int* radices, int* count);
+hipError_t launch_pass_pow2(int dtype, int dir, const PassDesc& d, const PassDesc* dev_desc, const void* x,
+                            void* y, const void* tw, hipStream_t stream);
+int pass_pow2_resident_groups(int dtype, int dir, const PassDesc& d);
+int pow2_max_threads(int dtype);
 
 hipError_t launch_tw_stage(int dtype, void* out, int ns, int R, int dir, hipStream_t stream);
 hipError_t launch_tw_pow(int dtype, void* out, int64_t count, int64_t mult, int64_t L, int dir,

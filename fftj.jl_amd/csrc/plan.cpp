@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -135,6 +136,22 @@ static bool pass_fits(int dtype, int n, int C, const std::vector<int>& radices, 
   return true;
 }
 
+// a row pass may drop to C = cap if a workgroup of that size still has a wave
+static bool pass_fits_rows_ok(int dtype, int n, int C, int cap) {
+  (void)dtype;
+  (void)C;
+  return static_cast<int64_t>(cap) * (n / 16) >= 64;
+}
+
+// the power-of-two kernel: 16 elements per thread, C*n/16 threads
+static bool pow2_fits(int dtype, int n, int C, int* threads) {
+  if (static_cast<int64_t>(lds_elems(C * n)) * esize(dtype) > kLdsBytes) return false;
+  const int t = C * (n / 16);
+  if (t > pow2_max_threads(dtype) || t % 64) return false;
+  if (threads) *threads = t;
+  return true;
+}
+
 static bool single_pass_ok(int dtype, int64_t n) {
   if (n == 1) return true;
   if (n > (1 << 16)) return false;
@@ -169,8 +186,8 @@ struct Step {
   int dir = -1;
   // ST_PASS
   PassDesc d{};
+  bool pow2 = false;  // compile-time-chain kernel
   std::vector<int> radices;
-  int64_t post_n = 0;  // four-step length (post twiddle)
   // other kernels
   int64_t n = 0, m = 0, h = 0, batch = 0, s1 = 0, s2 = 0, keep = 0;
   int w = 0;
@@ -179,7 +196,7 @@ struct Step {
   int64_t Bi = 1;
   Geometry ga{1, 0, 0}, gb{1, 0, 0};
   // device resources
-  DevBufPtr tw, thi, tlo, desc, aux;
+  DevBufPtr tw, desc, aux;
 };
 
 struct Layout {
@@ -203,18 +220,27 @@ static uint32_t magic_for(int ns) {
 }
 
 // one launch of the pass kernel
+// multi-pass twiddle of a pass (period 0 = none), see PassDesc
+struct PassTw {
+  int64_t period = 0, qdiv = 1, acoef = 0, bcoef = 0;
+};
+
 static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geometry in, Geometry out,
-                     double scale, int src, int dst, int64_t post_n) {
+                     double scale, int src, int dst, PassTw tw = PassTw(), int64_t B2 = 1) {
   Step s;
   s.kind = ST_PASS;
   s.src = src;
   s.dst = dst;
   s.dir = dir;
-  s.post_n = post_n;
   PassDesc& d = s.d;
   d.n = n;
   d.batch = batch;
   d.Bi = Bi;
+  d.B2 = B2;
+  d.tw_period = tw.period;
+  d.tw_qdiv = tw.qdiv;
+  d.tw_acoef = tw.acoef;
+  d.tw_bcoef = tw.bcoef;
   d.in = in;
   d.out = out;
   d.scale = scale;
@@ -235,6 +261,15 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   }
   if (n > 1 && s.radices.empty()) fail(FFTJH_ERR_UNSUPPORTED, "pass length has a prime factor above 13");
   if (static_cast<int>(s.radices.size()) > kMaxStages) fail(FFTJH_ERR_UNSUPPORTED, "too many stages");
+  // power-of-two lengths with a compile-time chain take the fused kernel
+  {
+    int pr[8], pc = 0;
+    static const bool no_pow2 = std::getenv("FFTJH_NO_POW2") != nullptr;
+    if (!no_pow2 && !(pt && pt->nradix > 0) && pow2_chain(n, pr, &pc)) {
+      s.pow2 = true;
+      s.radices.assign(pr, pr + pc);
+    }
+  }
   // transforms per workgroup: strided passes take adjacent columns for
   // contiguous HBM runs; row passes group short transforms to fill a workgroup
   const bool colmode = in.stride > 1 || out.stride > 1;
@@ -242,16 +277,20 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   if (pt && pt->C > 0) {
     int t;
     if ((pt->C & (pt->C - 1)) || pt->C > 64) fail(FFTJH_ERR_ARGUMENT, "tuning C must be a power of two <= 64");
-    if (!pass_fits(L.dtype, n, pt->C, s.radices, &t))
+    if (!(s.pow2 ? pow2_fits(L.dtype, n, pt->C, &t) : pass_fits(L.dtype, n, pt->C, s.radices, &t)))
       fail(FFTJH_ERR_ARGUMENT, "tuning C=" + std::to_string(pt->C) + " does not fit one workgroup");
     bestC = pt->C;
     bestT = t;
   } else {
+    // row passes: a wave covers C rows x 64/C consecutive elements, so keep
+    // C <= 128 bytes / 8*esize ... i.e. at least 128-byte runs
+    const int row_cap = L.dtype == FFTJH_F32 ? 4 : 8;
     for (int C = colmode ? 16 : 64; C >= 1; C >>= 1) {
       int t;
-      if (!pass_fits(L.dtype, n, C, s.radices, &t)) continue;
+      if (!(s.pow2 ? pow2_fits(L.dtype, n, C, &t) : pass_fits(L.dtype, n, C, s.radices, &t))) continue;
       if (C > 1 && C > batch) continue;
       if (!colmode && t > 256 && C > 1) continue;
+      if (!colmode && C > row_cap && t >= 64 * (C / row_cap) && pass_fits_rows_ok(L.dtype, n, C, row_cap)) continue;
       bestC = C;
       bestT = t;
       break;
@@ -266,6 +305,12 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   if (groups > 0x7fffffff) fail(FFTJH_ERR_UNSUPPORTED, "batch too large for one launch");
   d.ngroups = static_cast<int>(groups);
   d.xcd_remap = colmode ? 1 : 0;
+  // mixed passes (contiguous rows on one side, strided columns on the other)
+  // stage the row side through LDS with row-major lanes
+  if (s.pow2 && d.C > (L.dtype == FFTJH_F32 ? 4 : 8)) {
+    d.rows_in = (in.stride == 1 && out.stride > 1) ? 1 : 0;
+    d.rows_out = (out.stride == 1 && in.stride > 1) ? 1 : 0;
+  }
   if (pt && pt->xcd_remap > 0) d.xcd_remap = pt->xcd_remap == 2 ? 1 : 0;
   int ns = 1, off = 0;
   d.nstages = static_cast<int>(s.radices.size());
@@ -278,39 +323,64 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
     off += (R - 1) * ns;
     ns *= R;
   }
-  if (post_n) {
-    int lb = 0;
-    while ((int64_t(1) << (2 * lb)) < post_n) ++lb;  // 2^lb >= sqrt(N)
-    d.post_tw = 1;
-    d.lo_bits = lb;
-  }
   L.steps.push_back(s);
 }
 
-static Geometry geo(int64_t stride, int64_t s_in, int64_t s_out) { return Geometry{stride, s_in, s_out}; }
+static Geometry geo(int64_t stride, int64_t s_in, int64_t s_out, int64_t s_mid = 0) {
+  return Geometry{stride, s_in, s_out, s_mid};
+}
 
-// four-step split n = N1*N2 (N2 = pass-A column length) with both factors a
-// single pass, nearest to sqrt(n) (the role of _fourstep_split/_fs_pick,
-// fourstep.jl:166-199: nearest divisor to sqrt(n) with buildable factors)
-static bool fourstep_split(int dtype, int64_t n, int64_t& N1, int64_t& N2) {
-  double best = 1e300;
-  N1 = N2 = 0;
-  const double rt = std::sqrt(static_cast<double>(n));
-  for (int64_t q = 2; q * q <= n; ++q) {
-    if (n % q) continue;
-    const int64_t cands[2] = {q, n / q};
-    for (int64_t a : cands) {
-      const int64_t b = n / a;
-      if (!single_pass_ok(dtype, a) || !single_pass_ok(dtype, b)) continue;
-      const double dist = std::fabs(std::log(static_cast<double>(a) / rt));
-      if (dist < best - 1e-12 || (std::fabs(dist - best) <= 1e-12 && a < N2)) {
-        best = dist;
-        N2 = a;
-        N1 = b;
-      }
+// Factors n = F[0]*F[1]*...*F[P-1] of a multi-pass plan, each a single pass:
+// F[0] is the contiguous row pass (run last), F[P-1] the outermost column
+// pass (run first).  The fewest passes whose factors fit max_factor, then
+// the most balanced split (the role of _fourstep_split/_fs_pick,
+// fourstep.jl:166-199: nearest divisor to sqrt(n) with buildable factors).
+static void factor_search(int dtype, int64_t n, int parts, int64_t max_factor, std::vector<int64_t>& cur,
+                          std::vector<int64_t>& best, double& best_score) {
+  if (parts == 1) {
+    if (n > max_factor || !single_pass_ok(dtype, n)) return;
+    cur.push_back(n);
+    double lo = 1e300, hi = 0;
+    int non_pow2 = 0;
+    for (int64_t f : cur) {
+      lo = std::min(lo, static_cast<double>(f));
+      hi = std::max(hi, static_cast<double>(f));
+      non_pow2 += (f & (f - 1)) != 0;
     }
+    const double score = std::log(hi / lo) + 0.01 * non_pow2;
+    if (score < best_score - 1e-12) {
+      best_score = score;
+      best = cur;
+    }
+    cur.pop_back();
+    return;
   }
-  return N2 != 0;
+  for (int64_t f = 2; f <= std::min<int64_t>(n, max_factor); ++f) {
+    if (n % f || !single_pass_ok(dtype, f)) continue;
+    cur.push_back(f);
+    factor_search(dtype, n / f, parts - 1, max_factor, cur, best, best_score);
+    cur.pop_back();
+  }
+}
+
+static std::vector<int64_t> choose_factors(int dtype, int64_t n, int64_t max_factor) {
+  for (int parts = 2; parts <= 4; ++parts) {
+    std::vector<int64_t> cur, best;
+    double score = 1e300;
+    factor_search(dtype, n, parts, max_factor, cur, best, score);
+    if (!best.empty()) return best;
+  }
+  return {};
+}
+
+// largest factor the default multi-pass planner uses (the tile of one pass)
+static int64_t default_max_factor(int dtype) {
+  static const int64_t env = [] {
+    const char* e = std::getenv("FFTJH_MAX_FACTOR");
+    return e ? std::atoll(e) : 0LL;
+  }();
+  if (env > 0) return env;
+  return 256;
 }
 
 // ---- resampling parameters (resample.jl:12-21, 33-53, 202-256) ----
@@ -382,7 +452,7 @@ static bool is_rows(const Geometry& g, int64_t n, int64_t Bi) { return Bi == 1 &
 static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry gin, Geometry gout, int dir,
                       double scale, int src, int dst, std::vector<int> free) {
   if (single_pass_ok(L.dtype, n)) {
-    add_pass(L, dir, static_cast<int>(n), batch, Bi, gin, gout, scale, src, dst, 0);
+    add_pass(L, dir, static_cast<int>(n), batch, Bi, gin, gout, scale, src, dst);
     return;
   }
   int a, cnt[5];
@@ -414,23 +484,52 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
       L.steps.push_back(s2);
       return;
     }
-    int64_t N1, N2;
-    if (L.tune.split > 0 && L.npass == 0) {
-      N2 = L.tune.split;
-      if (n % N2 || !single_pass_ok(L.dtype, N2) || !single_pass_ok(L.dtype, n / N2))
-        fail(FFTJH_ERR_ARGUMENT, "tuning split " + std::to_string(N2) + " is not a two-pass split of " +
-                                     std::to_string(n));
-      N1 = n / N2;
-    } else if (!fourstep_split(L.dtype, n, N1, N2)) {
-      fail(FFTJH_ERR_UNSUPPORTED, "no two-pass split of length " + std::to_string(n));
+    std::vector<int64_t> F;
+    if (L.tune.splits[0] > 0 && L.npass == 0) {
+      int64_t prod = 1;
+      for (int i = 0; i < 4 && L.tune.splits[i] > 0; ++i) {
+        F.push_back(L.tune.splits[i]);
+        prod *= L.tune.splits[i];
+        if (!single_pass_ok(L.dtype, L.tune.splits[i]))
+          fail(FFTJH_ERR_ARGUMENT, "tuning factor " + std::to_string(L.tune.splits[i]) + " is not a single pass");
+      }
+      if (prod != n) fail(FFTJH_ERR_ARGUMENT, "tuning factors do not multiply to " + std::to_string(n));
+    } else {
+      F = choose_factors(L.dtype, n, default_max_factor(L.dtype));
     }
-    if (free.empty()) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for four-step");
+    if (F.size() < 2) fail(FFTJH_ERR_UNSUPPORTED, "no multi-pass split of length " + std::to_string(n));
+    if (free.empty()) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for a multi-pass plan");
     const int z = free[0];
     need_scratch(L, z, n * batch);
-    // pass A: columns j1 over j2 (stride N1), twiddle w_n^{j1*k2}, z[j1 + N1*k2]
-    add_pass(L, dir, static_cast<int>(N2), batch * N1, N1, geo(N1, 1, n), geo(N1, 1, n), 1.0, src, z, n);
-    // pass B: rows k2 of z over j1, y[k2 + N2*k1]
-    add_pass(L, dir, static_cast<int>(N1), batch * N2, N2, geo(1, N1, n), geo(N2, 1, n), scale, z, dst, 0);
+    // Pass d (P down to 1) transforms digit j_d of j = j1 + N1*(j2 + N2*(...)):
+    // for each low index jl < S_d and processed outputs K_d < M_d it reads
+    // jl + S_d*(j_d + N_d*K_d), writes jl + S_d*(K_d + M_d*k_d), and (d >= 2)
+    // multiplies by w_{N_{d-1} M_{d-1}}^{j_{d-1}(K_d + M_d k_d)} -- the nested
+    // four-step of fourstep.jl:1-17 with every level's twiddle fused into
+    // the pass that produces it.  Output digits end as k = kP + NP*(...+ N2*k1).
+    const int P = static_cast<int>(F.size());
+    std::vector<int64_t> S(P + 1, 1), M(P + 2, 1);
+    for (int d = 2; d <= P; ++d) S[d] = S[d - 1] * F[d - 2];
+    for (int d = P - 1; d >= 0; --d) M[d] = M[d + 1] * F[d];  // M[d] = prod F[d..P-1] (0-based)
+    int cur = src;
+    for (int d = P; d >= 1; --d) {
+      const int64_t Nd = F[d - 1];
+      const int64_t Sd = S[d];
+      const int64_t Md = M[d];  // prod of F[d..P-1] = N_{d+1}...N_P
+      const int target = (d == 1) ? dst : ((d % 2 == 1) ? dst : z);
+      // the first pass reads x with the same geometry it writes, so it may run in place
+      const int out = (d == P && P % 2 == 1) ? dst : target;
+      PassTw tw;
+      if (d >= 2) {
+        tw.period = F[d - 2] * M[d - 1];
+        tw.qdiv = S[d - 1];
+        tw.acoef = 1;
+        tw.bcoef = Md;
+      }
+      add_pass(L, dir, static_cast<int>(Nd), batch * (n / Nd), Sd, geo(Sd, 1, n, Sd * Nd), geo(Sd * Md, 1, n, Sd),
+               d == 1 ? scale : 1.0, cur, out, tw, Md);
+      cur = out;
+    }
     return;
   }
   // not 13-smooth: resample onto a smooth m (resample.jl:258-276)
@@ -513,7 +612,7 @@ static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch
   for (int i = 0; i < rank; ++i)
     if (dims[i] > 1) axes.push_back(i);
   if (axes.empty()) {
-    add_pass(L, sign, 1, batch, 1, geo(1, 0, 1), geo(1, 0, 1), scale, RX, RY, 0);
+    add_pass(L, sign, 1, batch, 1, geo(1, 0, 1), geo(1, 0, 1), scale, RX, RY);
     return L;
   }
   int src = RX;
@@ -648,9 +747,12 @@ static std::string describe(const Layout& L) {
     if (s.kind == ST_PASS) {
       o << " n=" << s.d.n << " radices=";
       for (size_t i = 0; i < s.radices.size(); ++i) o << (i ? "," : "") << s.radices[i];
+      if (s.pow2) o << " kernel=pow2";
+      if (s.d.rows_in) o << " rows_in";
+      if (s.d.rows_out) o << " rows_out";
       o << " batch=" << s.d.batch << " C=" << s.d.C << " threads=" << s.d.nthreads << " groups=" << s.d.ngroups
         << " in_stride=" << s.d.in.stride << " out_stride=" << s.d.out.stride;
-      if (s.d.post_tw) o << " twiddle=w_" << s.post_n;
+      if (s.d.tw_period) o << " twiddle=w_" << s.d.tw_period;
       if (s.d.scale != 1.0) o << " scale=" << s.d.scale;
     } else if (s.kind == ST_RS_DECONV || s.kind == ST_RS_INTERP) {
       o << " n=" << s.n << " m=" << s.m << " w=" << s.w << " beta=" << s.beta;
@@ -709,14 +811,17 @@ static void materialize(fftjh_plan_s& P) {
           nsl *= R;
         }
       });
-      if (s.d.post_tw) {
-        const int64_t N = s.post_n, K = int64_t(1) << s.d.lo_bits, nh = (N + K - 1) / K;
-        s.tlo = cached(key_of("fs_lo", {dev, double(dt), double(dir), double(N)}), K * es, [&](void* p) {
-          hip_check(launch_tw_pow(dt, p, K, 1, N, dir, 0), "twiddle kernel");
-        });
-        s.thi = cached(key_of("fs_hi", {dev, double(dt), double(dir), double(N), double(K)}), nh * es,
-                       [&](void* p) { hip_check(launch_tw_pow(dt, p, nh, K, N, dir, 0), "twiddle kernel"); });
-      }
+      // persistent grid: as many workgroups as are resident at once
+      // persistent grid (as many workgroups as are resident at once) for the
+      // power-of-two kernel; one workgroup per group for the generic one
+      static const int persist_env = [] {
+        const char* e = std::getenv("FFTJH_PERSIST");
+        return e ? std::atoi(e) : -1;
+      }();
+      const bool persist = persist_env >= 0 ? persist_env != 0 : s.pow2;
+      const int resident =
+          !persist ? 0 : s.pow2 ? pass_pow2_resident_groups(dt, s.dir, s.d) : pass_resident_groups(dt, s.dir, s.d);
+      s.d.grid = resident > 0 ? std::min(resident, s.d.ngroups) : s.d.ngroups;
       s.desc = std::make_shared<DevBuf>(sizeof(PassDesc));
       hip_check(hipMemcpy(s.desc->p, &s.d, sizeof(PassDesc), hipMemcpyHostToDevice), "descriptor upload");
     } else if (s.kind == ST_R2C_TWIST || s.kind == ST_C2R_PRE) {
@@ -754,8 +859,11 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
   const int dt = P.L.dtype;
   switch (s.kind) {
     case ST_PASS:
+      if (s.pow2)
+        return launch_pass_pow2(dt, s.dir, s.d, static_cast<const PassDesc*>(s.desc->p), ptr[s.src], ptr[s.dst],
+                                s.tw->p, st);
       return launch_pass(dt, s.dir, s.d, static_cast<const PassDesc*>(s.desc->p), ptr[s.src], ptr[s.dst], s.tw->p,
-                         s.thi ? s.thi->p : nullptr, s.tlo ? s.tlo->p : nullptr, st);
+                         st);
     case ST_R2C_TWIST:
       return launch_r2c_twist(dt, ptr[s.src], ptr[s.dst], s.aux->p, s.h, s.batch, s.s1, s.s2, st);
     case ST_C2R_PRE:

@@ -73,7 +73,9 @@ typedef struct fftjh_pass_tuning {
 } fftjh_pass_tuning;
 
 typedef struct fftjh_tuning {
-  int64_t split;                /* four-step column length N2 of a 1-D plan */
+  int64_t splits[4];            /* multi-pass factors of a 1-D plan: splits[0] the
+                                   contiguous row pass (run last), then outward;
+                                   zero-terminated, product = n */
   fftjh_pass_tuning pass[4];    /* by launch order of the pass kernels */
 } fftjh_tuning;
 

@@ -58,21 +58,23 @@ def _passes(desc):
 def test_schedule_config1_single_launch():
     d = _lib.describe_request(_lib.KIND_C2C, _lib.F64, [1024])
     steps = _passes(d)
-    assert len(steps) == 1 and "n=1024" in steps[0] and "radices=4,16,16" in steps[0]
+    assert len(steps) == 1 and "n=1024" in steps[0] and "radices=16,16,4" in steps[0] and "kernel=pow2" in steps[0]
 
 
 def test_schedule_config2():
     # (a) 4096 x 16384 contiguous: one launch, one transform per workgroup row
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [4096], batch=16384))
     assert len(s) == 1 and "batch=16384" in s[0]
-    # (b) 2^24: four-step 4096 x 4096, twiddle fused into pass A
+    # (b) 2^24: three 256-point passes (nested four-step), twiddles fused
+    # into the first two, natural-order output from the third
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1 << 24]))
-    assert len(s) == 2
-    assert "n=4096" in s[0] and "twiddle=w_16777216" in s[0] and "in_stride=4096" in s[0]
-    assert "n=4096" in s[1] and "in_stride=1" in s[1] and "out_stride=4096" in s[1]
-    # (c) 10^6 = 1000 x 1000
+    assert len(s) == 3 and all("n=256" in p and "kernel=pow2" in p for p in s)
+    assert "in_stride=65536" in s[0] and "twiddle=w_65536" in s[0]
+    assert "in_stride=256" in s[1] and "twiddle=w_16777216" in s[1]
+    assert "in_stride=1 " in s[2] and "out_stride=65536" in s[2] and "twiddle" not in s[2]
+    # (c) 10^6 = 100 x 100 x 100
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [10**6], batch=8))
-    assert "n=1000" in s[0] and "n=1000" in s[1]
+    assert len(s) == 3 and all("n=100 " in p for p in s)
 
 
 def test_schedule_config3():
