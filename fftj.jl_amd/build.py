@@ -24,7 +24,9 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["stockham.hip", "stockham_pow2.hip", "twiddle.hip", "real.hip", "resample.hip", "plan.cpp"]
+SOURCES = ["stockham.hip", "stockham_pow2.hip", "stockham_pow2_f32_fwd.hip", "stockham_pow2_f32_inv.hip",
+           "stockham_pow2_f64_fwd.hip", "stockham_pow2_f64_inv.hip", "twiddle.hip", "real.hip", "resample.hip",
+           "plan.cpp"]
 
 
 def _compile(src):

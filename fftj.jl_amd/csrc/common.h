@@ -64,6 +64,10 @@ struct PassDesc {
   int32_t grid;       // workgroups launched (persistent: each walks several groups)
   int32_t rows_in;    // power-of-two kernel: stage HBM rows through LDS on the way in
   int32_t rows_out;   // ... and on the way out (row-major lanes, contiguous runs)
+  // with rows_in / rows_out: the C transforms of a group start at
+  // base(g*C) + c*cdelta (the planner checks that this holds)
+  int64_t in_cdelta;
+  int64_t out_cdelta;
 };
 
 // LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)

@@ -285,15 +285,23 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
     // row passes: a wave covers C rows x 64/C consecutive elements, so keep
     // C <= 128 bytes / 8*esize ... i.e. at least 128-byte runs
     const int row_cap = L.dtype == FFTJH_F32 ? 4 : 8;
+    for (int attempt = 0; attempt < 2 && !bestC; ++attempt) {
+    if (attempt == 1) {
+      if (!s.pow2) break;
+      s.pow2 = false;  // no group size fits the power-of-two kernel: generic chain
+      s.radices = default_radices(n);
+    }
     for (int C = colmode ? 16 : 64; C >= 1; C >>= 1) {
       int t;
       if (!(s.pow2 ? pow2_fits(L.dtype, n, C, &t) : pass_fits(L.dtype, n, C, s.radices, &t))) continue;
-      if (C > 1 && C > batch) continue;
+      // more columns than transforms only to fill one wave (idle columns)
+      if (C > 1 && C > batch && !(s.pow2 && (C / 2) * (n / 16) < 64)) continue;
       if (!colmode && t > 256 && C > 1) continue;
       if (!colmode && C > row_cap && t >= 64 * (C / row_cap) && pass_fits_rows_ok(L.dtype, n, C, row_cap)) continue;
       bestC = C;
       bestT = t;
       break;
+    }
     }
   }
   if (!bestC) fail(FFTJH_ERR_UNSUPPORTED, "pass does not fit one workgroup");
@@ -308,8 +316,24 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   // mixed passes (contiguous rows on one side, strided columns on the other)
   // stage the row side through LDS with row-major lanes
   if (s.pow2 && d.C > (L.dtype == FFTJH_F32 ? 4 : 8)) {
-    d.rows_in = (in.stride == 1 && out.stride > 1) ? 1 : 0;
-    d.rows_out = (out.stride == 1 && in.stride > 1) ? 1 : 0;
+    // the C transforms of a group must sit at a constant offset step
+    int64_t cin = -1, cout = -1;
+    if (Bi > 1 && Bi % d.C == 0) {
+      cin = in.s_in;
+      cout = out.s_in;
+    } else if (Bi == 1 && B2 > 1 && B2 % d.C == 0) {
+      cin = in.s_mid;
+      cout = out.s_mid;
+    } else if (Bi == 1 && B2 == 1) {
+      cin = in.s_out;
+      cout = out.s_out;
+    }
+    if (cin >= 0) {
+      d.in_cdelta = cin;
+      d.out_cdelta = cout;
+      d.rows_in = (in.stride == 1 && out.stride > 1) ? 1 : 0;
+      d.rows_out = (out.stride == 1 && in.stride > 1 && d.tw_period == 0) ? 1 : 0;
+    }
   }
   if (pt && pt->xcd_remap > 0) d.xcd_remap = pt->xcd_remap == 2 ? 1 : 0;
   int ns = 1, off = 0;

@@ -1,334 +1,22 @@
-// Pass kernel for power-of-two lengths 256..16384 with a compile-time radix
-// chain (radices from {2, 4, 8, 16}).
-//
-// Each thread owns E = 16 elements of a transform, rows jt + u*J (u < 16,
-// J = n/16).  With the autosort indexing a radix-R stage's butterflies
-// j = jt + b*J (b < 16/R) read rows j + r*n/R = jt + J*(b + r*16/R) -- the
-// same 16 rows -- and the last stage (k == j) writes those rows too.  So:
-//   stage 1   runs on the registers the HBM loads filled, writes LDS;
-//   stages    LDS -> registers -> LDS (two barriers each);
-//   last      LDS -> registers -> HBM, with scale and four-step twiddle.
-// Two LDS round trips for a 4096-point transform instead of five, no calls,
-// everything unrolled at compile time.  The kernel is persistent: while the
-// later stages of group g run, the loads of the next group are in flight.
-//
-// Four-step twiddle w_N^{j1*k} for k = jt + u*J: w = A * S^u with
-// A = w_N^{j1*jt} and S = w_N^{j1*J} from FP64 sincospi of exact integer
-// angles, the powers formed in FP64 and rounded once, so no table loads
-// sit in the store phase.
-#include "butterflies.h"
-#include "common.h"
-#include "launch.h"
-#include "passcommon.h"
+// Public entry points of the power-of-two pass kernels; the kernels are
+// instantiated per (precision, direction) in stockham_pow2_<t><dir>.hip so
+// they compile in parallel.
+#include "stockham_pow2.h"
 
 namespace fftjh {
 
-namespace {
-
-constexpr int kE = 16;
-
-#ifndef FFTJH_P2_MAXT
-#define FFTJH_P2_MAXT 512
-#endif
-#ifndef FFTJH_P2_PREFETCH
-#define FFTJH_P2_PREFETCH 0
-#endif
-template <typename T> struct MaxThreadsP2 { static constexpr int value = FFTJH_P2_MAXT; };
-template <> struct MaxThreadsP2<double> { static constexpr int value = FFTJH_P2_MAXT / 2; };
-
-__device__ __forceinline__ int lpad2(int f) { return f + (f >> 4); }
-
-// the chain's radices, and per stage its ns and twiddle-table offset
-// (entries (r-1)*ns + k per stage)
-template <int R1, int R2, int R3, int R4>
-struct Radices {
-  static constexpr int count = (R1 > 1) + (R2 > 1) + (R3 > 1) + (R4 > 1);
-  static constexpr int get(int s) { return s == 0 ? R1 : s == 1 ? R2 : s == 2 ? R3 : R4; }
-  static constexpr int ns(int s) { return s == 0 ? 1 : ns(s - 1) * get(s - 1); }
-  static constexpr int off(int s) { return s == 0 ? 0 : off(s - 1) + (get(s - 1) - 1) * ns(s - 1); }
-};
-
-// butterflies of one stage on the thread's registers: b < 16/R, rows
-// j = jt + b*J, inputs v[b + r*16/R]; twiddles w[b][r-1] already applied
-template <typename T, int DIR, int R>
-__device__ __forceinline__ void stage_regs(V2<T> (&v)[kE]) {
-  constexpr int BPT = kE / R;
-#pragma unroll
-  for (int b = 0; b < BPT; ++b) {
-    V2<T> a[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) a[r] = v[b + r * BPT];
-    dft<T, DIR, R>(a);
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[b + r * BPT] = a[r];
-  }
-}
-
-// twiddles of stage (R, NS) for this thread's butterflies, from the table
-template <typename T, int R, int NS>
-__device__ __forceinline__ void stage_twiddle(V2<T> (&v)[kE], const V2<T>* tw, int jt, int J) {
-  if constexpr (NS > 1) {
-    constexpr int BPT = kE / R;
-#pragma unroll
-    for (int b = 0; b < BPT; ++b) {
-      const int j = jt + b * J;
-      const int k = j & (NS - 1);
-      const V2<T>* twp = tw + k;
-#pragma unroll
-      for (int r = 1; r < R; ++r) v[b + r * BPT] = cmul<T>(v[b + r * BPT], twp[(r - 1) * NS]);
-    }
-  }
-}
-
-// scatter a stage's outputs into LDS: output r of butterfly j at
-// (j - k)*R + k + r*NS, k = j mod NS
-template <typename T, int R, int NS>
-__device__ __forceinline__ void stage_scatter(const V2<T> (&v)[kE], V2<T>* lds, int C, int c, int jt, int J) {
-  constexpr int BPT = kE / R;
-#pragma unroll
-  for (int b = 0; b < BPT; ++b) {
-    const int j = jt + b * J;
-    const int k = j & (NS - 1);
-    const int base = (j - k) * R + k;
-#pragma unroll
-    for (int r = 0; r < R; ++r) lds[lpad2((base + r * NS) * C + c)] = v[b + r * BPT];
-  }
-}
-
-// gather a stage's inputs from LDS: input r of butterfly j at j + r*N/R
-template <typename T, int N, int R>
-__device__ __forceinline__ void stage_gather(V2<T> (&v)[kE], const V2<T>* lds, int C, int c, int jt, int J) {
-  constexpr int BPT = kE / R;
-#pragma unroll
-  for (int b = 0; b < BPT; ++b) {
-    const int j = jt + b * J;
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[b + r * BPT] = lds[lpad2((j + r * (N / R)) * C + c)];
-  }
-}
-
-template <typename T, int DIR, int N, typename Ch, int S>
-__device__ __forceinline__ void middle_stages(V2<T> (&v)[kE], V2<T>* lds, const V2<T>* tw, int C,
-                                              int c, int jt) {
-  if constexpr (S < Ch::count - 1) {
-    constexpr int R = Ch::get(S), NS = Ch::ns(S), OFF = Ch::off(S);
-    constexpr int J = N / kE;
-    stage_gather<T, N, R>(v, lds, C, c, jt, J);
-    __syncthreads();
-    stage_twiddle<T, R, NS>(v, tw + OFF, jt, J);
-    stage_regs<T, DIR, R>(v);
-    stage_scatter<T, R, NS>(v, lds, C, c, jt, J);
-    __syncthreads();
-    middle_stages<T, DIR, N, Ch, S + 1>(v, lds, tw, C, c, jt);
-  }
-}
-
-}  // namespace
-
-template <typename T, int DIR, int N, int R1, int R2, int R3, int R4>
-__global__ void __launch_bounds__(MaxThreadsP2<T>::value)
-k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<T>* tw) {
-  using Ch = Radices<R1, R2, R3, R4>;
-  constexpr int NST = Ch::count;
-  constexpr int J = N / kE;
-  static_assert(NST >= 2, "fused first and last stages need two stages");
-  extern __shared__ __align__(16) unsigned char smem[];
-  V2<T>* lds = reinterpret_cast<V2<T>*>(smem);
-  const PassDesc& d = *dp;
-  const int C = d.C;
-  const int tid = static_cast<int>(threadIdx.x);
-  const int c = tid & (C - 1);
-  const int jt = tid >> d.logC;  // < J
-  const int64_t ist = d.in.stride, ost = d.out.stride;
-  const GroupWalk walk(d);
-
-  V2<T> v[kE], pf[kE];
-  auto load = [&](int g, V2<T> (&dst)[kE]) {
-    const int64_t b = static_cast<int64_t>(g) * C + c;
-    const bool valid = b < d.batch;
-    const V2<T>* xb = x + (BatchIndex(d, b).base(d.in) + jt * ist);
-#pragma unroll
-    for (int u = 0; u < kE; ++u) dst[u] = valid ? xb[u * J * ist] : mk<T>(T(0), T(0));
-  };
-
-  // Row-major staging for contiguous rows when C columns share a workgroup:
-  // lanes run along a row (f = tid + u*nthreads -> column f / N, row f % N),
-  // so every HBM access is a contiguous run, and LDS does the transposition.
-  const int nthr = d.nthreads;
-  auto load_rows = [&](int g) {
-#pragma unroll
-    for (int u = 0; u < kE; ++u) {
-      const int f = tid + u * nthr;
-      const int cc = f / N, i = f % N;
-      const int64_t b = static_cast<int64_t>(g) * C + cc;
-      V2<T> val = mk<T>(T(0), T(0));
-      if (b < d.batch) val = x[BatchIndex(d, b).base(d.in) + i * ist];
-      lds[lpad2(i * C + cc)] = val;
-    }
-  };
-  auto store_rows = [&](int g) {
-#pragma unroll
-    for (int u = 0; u < kE; ++u) {
-      const int f = tid + u * nthr;
-      const int cc = f / N, k = f % N;
-      const int64_t b = static_cast<int64_t>(g) * C + cc;
-      if (b < d.batch) {
-        const BatchIndex ix(d, b);
-        V2<T> val = lds[lpad2(k * C + cc)];
-        if (d.tw_period) {
-          PostTwiddle ptw;
-          ptw.init<DIR>(d, ix, k, 1);
-          val = cmul<T>(val, mk<T>(static_cast<T>(ptw.w.x), static_cast<T>(ptw.w.y)));
-        }
-        if (d.scale != 1.0) val = cscale<T>(val, static_cast<T>(d.scale));
-        y[ix.base(d.out) + k * ost] = val;
-      }
-    }
-  };
-
-  int g = walk.start;
-  if (g >= walk.end) return;
-  if (d.rows_in) {
-    load_rows(g);
-    __syncthreads();
-    stage_gather<T, N, R1>(v, lds, C, c, jt, J);
-    __syncthreads();
-  } else {
-    load(g, v);
-  }
-  const T scale = static_cast<T>(d.scale);
-  const bool scaled = d.scale != 1.0;
-  while (true) {
-    // stage 1 on the loaded registers (ns = 1: no twiddles)
-    stage_regs<T, DIR, R1>(v);
-    stage_scatter<T, R1, 1>(v, lds, C, c, jt, J);
-    __syncthreads();
-    const int gn = g + walk.step;
-    const bool more = gn < walk.end;
-    if (FFTJH_P2_PREFETCH && more) load(gn, pf);
-    middle_stages<T, DIR, N, Ch, 1>(v, lds, tw, C, c, jt);
-    // last stage: LDS -> registers -> HBM
-    {
-      constexpr int S = NST - 1;
-      constexpr int R = Ch::get(S), NS = Ch::ns(S), OFF = Ch::off(S);
-      const int64_t b = static_cast<int64_t>(g) * C + c;
-      const BatchIndex ix(d, b);
-      // the multi-pass twiddle's base and step, formed while v is dead
-      PostTwiddle pt;
-      if (d.tw_period && !d.rows_out) pt.init<DIR>(d, ix, jt, J);
-      stage_gather<T, N, R>(v, lds, C, c, jt, J);
-      __syncthreads();
-      stage_twiddle<T, R, NS>(v, tw + OFF, jt, J);
-      stage_regs<T, DIR, R>(v);
-      if (d.rows_out) {
-        // last stage: k == j, so output r of butterfly j goes to row j + r*N/R
-        stage_scatter<T, R, N / R>(v, lds, C, c, jt, J);
-        __syncthreads();
-        store_rows(g);
-        __syncthreads();
-      } else if (b < d.batch) {
-        V2<T>* yb = y + (ix.base(d.out) + jt * ost);
-        if (d.tw_period) {
-#pragma unroll
-          for (int u = 0; u < kE; ++u) {
-            V2<T> val = cmul<T>(v[u], mk<T>(static_cast<T>(pt.w.x), static_cast<T>(pt.w.y)));
-            if (scaled) val = cscale<T>(val, scale);
-            yb[u * J * ost] = val;
-            pt.advance();
-          }
-        } else {
-#pragma unroll
-          for (int u = 0; u < kE; ++u) yb[u * J * ost] = scaled ? cscale<T>(v[u], scale) : v[u];
-        }
-      }
-    }
-    if (!more) break;
-    if (FFTJH_P2_PREFETCH) {
-#pragma unroll
-      for (int u = 0; u < kE; ++u) v[u] = pf[u];
-    } else if (d.rows_in) {
-      load_rows(gn);
-      __syncthreads();
-      stage_gather<T, N, R1>(v, lds, C, c, jt, J);
-      __syncthreads();
-    } else {
-      load(gn, v);
-    }
-    g = gn;
-  }
-}
-
-namespace {
-
-template <typename T, int DIR, int N, int R1, int R2, int R3, int R4>
-hipError_t launch_p2(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
-                     hipStream_t st) {
-  static bool attr = false;
-  auto* fn = &k_pass_pow2<T, DIR, N, R1, R2, R3, R4>;
-  if (!attr) {
-    hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
-    attr = true;
-  }
-  const size_t lds = static_cast<size_t>(lds_elems(d.C * N)) * sizeof(V2<T>);
-  const int grid = d.grid > 0 ? d.grid : d.ngroups;
-  hipLaunchKernelGGL(fn, dim3(grid), dim3(d.nthreads), lds, st, dd, static_cast<const V2<T>*>(x),
-                     static_cast<V2<T>*>(y), static_cast<const V2<T>*>(tw));
-  return hipGetLastError();
-}
-
-template <typename T, int DIR, int N, int R1, int R2, int R3, int R4>
-int resident_p2(const PassDesc& d) {
-  auto* fn = &k_pass_pow2<T, DIR, N, R1, R2, R3, R4>;
-  hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
-  const size_t lds = static_cast<size_t>(lds_elems(d.C * N)) * sizeof(V2<T>);
-  int per_cu = 0, dev = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), d.nthreads, lds) !=
-      hipSuccess)
-    return 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  return per_cu * cus;
-}
-
-// the compile-time chains: n -> radices (16s first, the remainder last)
-#define FFTJH_POW2_CHAINS(X)  \
-  X(32, 16, 2, 1, 1)          \
-  X(64, 16, 4, 1, 1)          \
-  X(128, 16, 8, 1, 1)         \
-  X(256, 16, 16, 1, 1)        \
-  X(512, 16, 16, 2, 1)        \
-  X(1024, 16, 16, 4, 1)       \
-  X(2048, 16, 16, 8, 1)       \
-  X(4096, 16, 16, 16, 1)      \
-  X(8192, 16, 16, 16, 2)      \
-  X(16384, 16, 16, 16, 4)
-
-template <typename T, int DIR>
-hipError_t dispatch_launch(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
-                           hipStream_t st) {
-  switch (d.n) {
-#define X(N, A, B, C_, D_) \
-  case N:                  \
-    return launch_p2<T, DIR, N, A, B, C_, D_>(d, dd, x, y, tw, st);
-    FFTJH_POW2_CHAINS(X)
-#undef X
-  }
-  return hipErrorInvalidValue;
-}
-
-template <typename T, int DIR>
-int dispatch_resident(const PassDesc& d) {
-  switch (d.n) {
-#define X(N, A, B, C_, D_) \
-  case N:                  \
-    return resident_p2<T, DIR, N, A, B, C_, D_>(d);
-    FFTJH_POW2_CHAINS(X)
-#undef X
-  }
-  return 0;
-}
-
-}  // namespace
+hipError_t launch_pow2_f32_fwd(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
+                               hipStream_t st);
+hipError_t launch_pow2_f32_inv(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
+                               hipStream_t st);
+hipError_t launch_pow2_f64_fwd(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
+                               hipStream_t st);
+hipError_t launch_pow2_f64_inv(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
+                               hipStream_t st);
+int resident_pow2_f32_fwd(const PassDesc& d);
+int resident_pow2_f32_inv(const PassDesc& d);
+int resident_pow2_f64_fwd(const PassDesc& d);
+int resident_pow2_f64_inv(const PassDesc& d);
 
 bool pow2_chain(int n, int* radices, int* count) {
   switch (n) {
@@ -350,13 +38,13 @@ bool pow2_chain(int n, int* radices, int* count) {
 hipError_t launch_pass_pow2(int dtype, int dir, const PassDesc& d, const PassDesc* dd, const void* x, void* y,
                             const void* tw, hipStream_t st) {
   if (dtype == 0)
-    return dir < 0 ? dispatch_launch<float, -1>(d, dd, x, y, tw, st) : dispatch_launch<float, 1>(d, dd, x, y, tw, st);
-  return dir < 0 ? dispatch_launch<double, -1>(d, dd, x, y, tw, st) : dispatch_launch<double, 1>(d, dd, x, y, tw, st);
+    return dir < 0 ? launch_pow2_f32_fwd(d, dd, x, y, tw, st) : launch_pow2_f32_inv(d, dd, x, y, tw, st);
+  return dir < 0 ? launch_pow2_f64_fwd(d, dd, x, y, tw, st) : launch_pow2_f64_inv(d, dd, x, y, tw, st);
 }
 
 int pass_pow2_resident_groups(int dtype, int dir, const PassDesc& d) {
-  if (dtype == 0) return dir < 0 ? dispatch_resident<float, -1>(d) : dispatch_resident<float, 1>(d);
-  return dir < 0 ? dispatch_resident<double, -1>(d) : dispatch_resident<double, 1>(d);
+  if (dtype == 0) return dir < 0 ? resident_pow2_f32_fwd(d) : resident_pow2_f32_inv(d);
+  return dir < 0 ? resident_pow2_f64_fwd(d) : resident_pow2_f64_inv(d);
 }
 
 int pow2_max_threads(int dtype) { return dtype == 0 ? MaxThreadsP2<float>::value : MaxThreadsP2<double>::value; }
