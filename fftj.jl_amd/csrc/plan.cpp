@@ -315,7 +315,10 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   d.xcd_remap = colmode ? 1 : 0;
   // mixed passes (contiguous rows on one side, strided columns on the other)
   // stage the row side through LDS with row-major lanes
-  if (s.pow2 && d.C > (L.dtype == FFTJH_F32 ? 4 : 8)) {
+  // (measured slower than register I/O at the variant's register cost: off
+  // unless FFTJH_ROWS_IO=1)
+  static const bool rows_io = std::getenv("FFTJH_ROWS_IO") != nullptr;
+  if (rows_io && s.pow2 && d.C > (L.dtype == FFTJH_F32 ? 4 : 8)) {
     // the C transforms of a group must sit at a constant offset step
     int64_t cin = -1, cout = -1;
     if (Bi > 1 && Bi % d.C == 0) {
@@ -842,7 +845,10 @@ static void materialize(fftjh_plan_s& P) {
         const char* e = std::getenv("FFTJH_PERSIST");
         return e ? std::atoi(e) : -1;
       }();
-      const bool persist = persist_env >= 0 ? persist_env != 0 : s.pow2;
+      // (small generic tiles leave room for several workgroups per CU and
+      // gain from it; a 1-workgroup-per-CU generic tile loses)
+      const bool small_tile = static_cast<int64_t>(lds_elems(s.d.C * s.d.n)) * es <= 48 * 1024;
+      const bool persist = persist_env >= 0 ? persist_env != 0 : (s.pow2 || small_tile);
       const int resident =
           !persist ? 0 : s.pow2 ? pass_pow2_resident_groups(dt, s.dir, s.d) : pass_resident_groups(dt, s.dir, s.d);
       s.d.grid = resident > 0 ? std::min(resident, s.d.ngroups) : s.d.ngroups;
