@@ -24,7 +24,7 @@ HBM_PEAK_GBS = 8000.0
 
 # name -> (n, batch, description); the other configurations are parity cases
 WORKLOADS = {
-    "c2c_f32_2pow24": (1 << 24, 1, "C2C FP32 N=2^24 batch 1, four-step 4096x4096, forward+inverse"),
+    "c2c_f32_2pow24": (1 << 24, 1, "C2C FP32 N=2^24 batch 1, three 256-point passes, forward+inverse"),
     "c2c_f32_4096x16384": (4096, 16384, "C2C FP32 N=4096 batch 16384, forward+inverse"),
     "c2c_f32_1e6x8": (10**6, 8, "C2C FP32 N=10^6 batch 8, forward+inverse"),
 }
@@ -134,11 +134,14 @@ def main():
     t_launch = launches[k] * 1e-3
     bytes_per_launch = 16.0 * n * batch  # one read + one write of the complex FP32 array
     achieved = bytes_per_launch / t_launch / 1e9
+    # HBM bytes per pass launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
     traffic = None
-    tp = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    tp = os.path.join(ROOT, "profiles", f"pmc_traffic_{args.workload}.md")
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    traffic = int(line.split(":")[1])
 
     result = {
         "metric": "FP32 C2C GFLOP/s & %HBM peak at N=2^24; ragged-batch tuned vs plain speedup",
