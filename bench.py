@@ -38,7 +38,30 @@ def parse():
     ap.add_argument("--workload", default="c2c_f32_2pow24", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="counter runs: 3 forward plans of the workload, then 3 copies of the same bytes "
+                         "(the calibration kernel of known traffic); no timing, no JSON")
     return ap.parse_args()
+
+
+def pmc_probe(args):
+    import numpy as np
+    import torch
+    import __graft_entry__
+
+    F = __graft_entry__.load_package()
+    n, batch, _ = WORKLOADS[args.workload]
+    x = torch.from_numpy(F.rng.uniform_complex(0, n * batch).astype(np.complex64)).to("cuda:0")
+    y = torch.empty_like(x)
+    p = F.plan_fft(torch.float32, n, batch=batch)
+    print(p.describe(), flush=True)
+    for _ in range(3):
+        p(y, x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("pmc probe done", flush=True)
 
 
 def cpu_baseline(n, batch, target_s):
@@ -84,6 +107,9 @@ def cpu_baseline(n, batch, target_s):
 
 def main():
     args = parse()
+    if args.pmc_probe:
+        pmc_probe(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described in DESIGN.md, not built)"}))
