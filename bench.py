@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--workload", default="c2c_f32_2pow24", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch leg")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 3 forward plans of the workload, then 3 copies of the same bytes "
                          "(the calibration kernel of known traffic); no timing, no JSON")
@@ -103,6 +104,34 @@ def cpu_baseline(n, batch, target_s):
         "sample": f"{reps} forward+inverse steps of the same workload on {threads} threads "
                   f"(oracle FP32 port of FFTJ's threaded {'FourStepPlan' if batch == 1 else 'per-transform FFTPlan loop'})",
     }
+
+
+def ragged_leg(F, quick=False):
+    """The ragged configuration: 10,000 FP32 transforms, one persistent launch
+    (autotuned) vs the reference's loop of per-length 1-D plans."""
+    import numpy as np
+    import torch
+
+    sizes = F.ragged.benchmark_sizes()
+    x = torch.from_numpy(F.rng.uniform_complex(1, int(sizes.sum())).astype(np.complex64)).to("cuda:0")
+    y = torch.empty_like(x)
+    t0 = time.perf_counter()
+    plan, cfg, results = F.tune.autotune_ragged(torch.float32, sizes, x=x, y=y, iters=5,
+                                               budget=2 if quick else None)
+    tune_s = time.perf_counter() - t0
+    tuned_ms = F.tune.time_plan(plan, y, x, iters=20)
+    default_ms = F.tune.time_plan(F.plan_ragged(torch.float32, sizes), y, x, iters=20)
+    loop = F.ragged.PlainLoop(torch.float32, sizes)
+    plain_ms = F.tune.time_plan(loop, y, x, iters=3, warmup=1)
+    bytes_ = 16.0 * float(sizes.sum())
+    flops = float((5.0 * sizes * np.log2(np.maximum(sizes, 2))).sum())
+    return {"transforms": int(len(sizes)), "distinct_sizes": int(len(np.unique(sizes))),
+            "sum_n": int(sizes.sum()), "tuned_ms": round(tuned_ms, 4), "default_ms": round(default_ms, 4),
+            "plain_loop_ms": round(plain_ms, 3), "speedup_tuned_vs_plain": round(plain_ms / tuned_ms, 2),
+            "tuned_config": cfg, "tuning_s": round(tune_s, 2),
+            "tuned_gbs": round(bytes_ / (tuned_ms * 1e-3) / 1e9, 1),
+            "tuned_gflops": round(flops / (tuned_ms * 1e-3) / 1e9, 1),
+            "candidates": [[c, round(ms, 4)] for c, ms in results]}
 
 
 def main():
@@ -194,6 +223,8 @@ def main():
                                f"{k % len(launches_f)})",
                      "launch_ms": [round(v, 5) for v in launches]},
     }
+    if not args.no_ragged:
+        result["ragged"] = ragged_leg(F)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(n, batch, args.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -13,14 +13,20 @@ LIB_PATH = os.environ.get("FFTJH_LIB") or os.path.join(HERE, "lib", "libfftjh.so
 
 OK, ERR_ARGUMENT, ERR_DIMENSION, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
-KIND_C2C, KIND_R2C, KIND_C2R = 0, 1, 2
+KIND_C2C, KIND_R2C, KIND_C2R, KIND_RAGGED = 0, 1, 2, 3
 
 # every entry point include/fftjh.h declares
 EXPORTS = (
     "fftjh_plan_c2c", "fftjh_plan_r2c", "fftjh_plan_c2r", "fftjh_execute", "fftjh_execute_checked",
     "fftjh_plan_info", "fftjh_plan_describe", "fftjh_describe_request", "fftjh_plan_destroy",
-    "fftjh_last_error", "fftjh_version", "fftjh_profile_launches", "fftjh_plan_c2c_tuned",
+    "fftjh_last_error", "fftjh_version", "fftjh_profile_launches", "fftjh_plan_c2c_tuned", "fftjh_plan_ragged",
+    "fftjh_plan_ragged_tuned",
 )
+
+
+class RaggedTuning(ctypes.Structure):
+    _fields_ = [("groups", ctypes.c_int32), ("radix_pref", ctypes.c_int32), ("tile_max", ctypes.c_int32),
+                ("pad_", ctypes.c_int32), ("costs", ctypes.c_void_p)]
 
 
 class PassTuning(ctypes.Structure):
@@ -81,6 +87,9 @@ def lib():
         L.fftjh_plan_c2c.argtypes = [ctypes.POINTER(vp), ci, ci, P64, i64, ci, ctypes.c_double, ctypes.c_uint]
         L.fftjh_plan_c2c_tuned.argtypes = [ctypes.POINTER(vp), ci, ci, P64, i64, ci, ctypes.c_double,
                                            ctypes.POINTER(Tuning)]
+        L.fftjh_plan_ragged.argtypes = [ctypes.POINTER(vp), ci, i64, P64, P64, ci, ci, ctypes.c_uint]
+        L.fftjh_plan_ragged_tuned.argtypes = [ctypes.POINTER(vp), ci, i64, P64, P64, ci, ci,
+                                              ctypes.POINTER(RaggedTuning)]
         L.fftjh_plan_r2c.argtypes = [ctypes.POINTER(vp), ci, i64, i64, ctypes.c_uint]
         L.fftjh_plan_c2r.argtypes = [ctypes.POINTER(vp), ci, i64, i64, ci, ctypes.c_uint]
         L.fftjh_execute.argtypes = [vp, vp, vp, vp]

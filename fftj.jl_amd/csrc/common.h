@@ -73,13 +73,4 @@ struct PassDesc {
 // LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)
 inline int lds_elems(int cn) { return cn + (cn >> 4) + 1; }
 
-// Work item of the ragged persistent kernel (one whole transform).
-struct RaggedItem {
-  int64_t offset;    // complex offset of the transform in the packed buffers
-  int32_t n;
-  int32_t kind;      // 0 = direct LDS chain, 1 = two-level (four-step in one workgroup), 2 = resample
-  int32_t sched;     // index into the schedule table
-  int32_t pad_;
-};
-
 }  // namespace fftjh

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <queue>
 #include <memory>
 #include <mutex>
 #include <sstream>
@@ -25,6 +26,7 @@
 #include "../../include/fftjh.h"
 #include "common.h"
 #include "launch.h"
+#include "ragged.h"
 
 namespace fftjh {
 
@@ -175,10 +177,12 @@ enum StepKind {
   ST_RS_DECONV,
   ST_RS_INTERP,
   ST_STRIDED_COPY,
+  ST_RAGGED,
 };
 
 static const char* kStepNames[] = {"pass",           "r2c_twist", "c2r_pre",   "real_to_complex", "copy_prefix",
-                                   "hermitian_fill", "take_real", "rs_deconv", "rs_interp",       "strided_copy"};
+                                   "hermitian_fill", "take_real", "rs_deconv", "rs_interp",       "strided_copy",
+                                   "ragged"};
 
 struct Step {
   int kind = ST_PASS;
@@ -197,6 +201,7 @@ struct Step {
   Geometry ga{1, 0, 0}, gb{1, 0, 0};
   // device resources
   DevBufPtr tw, desc, aux;
+  std::shared_ptr<struct RaggedData> rg;  // ST_RAGGED
 };
 
 struct Layout {
@@ -762,10 +767,261 @@ static Layout layout_c2r(int dtype, int64_t n, int64_t batch, int normalized) {
   return L;
 }
 
+// ----------------------------------------------------------- ragged --
+
+// Host side of one ragged launch.  Tables are per plan (the reference keeps
+// no ragged state at all; its baseline is a loop of 1-D plans).
+struct RaggedData {
+  int dir = -1;
+  int groups = 0;
+  int radix_pref = 16;     // 16: radix-16 towers; 8: radix-8 towers
+  int tile_max = kRaggedTile;
+  std::vector<RaggedItem> items;
+  std::vector<RaggedSched> scheds;
+  std::map<int, int> sched_of;  // tile length -> schedule
+  std::vector<int32_t> wg_ptr, wg_items;
+  std::vector<int64_t> wg_ws;
+  int64_t ws_total = 0;
+  int64_t tw_total = 0, corr_total = 0;
+  struct StageJob { int64_t off; int ns, R; };
+  struct PowJob { int64_t off, count, mult, L; };
+  struct CorrJob { int64_t off, n, m; int w; double beta, scale; };
+  std::vector<StageJob> stage_jobs;
+  std::vector<PowJob> pow_jobs;
+  std::vector<CorrJob> corr_jobs;
+  std::vector<double> cost;     // per item
+  int64_t counts[4] = {0, 0, 0, 0};
+  DevBufPtr d_items, d_wgptr, d_wgitems, d_scheds, d_tw, d_corr, d_ws, d_wgws;
+};
+
+static std::vector<int> radices_pref(int64_t n, int pref) {
+  int a, cnt[5];
+  std::vector<int> r;
+  if (!factor13(n, a, cnt)) return r;
+  const int big = pref == 8 ? 3 : 4;
+  if (a % big) r.push_back(1 << (a % big));
+  for (int i = 0; i < a / big; ++i) r.push_back(1 << big);
+  for (int i = 0; i < 5; ++i)
+    for (int k = 0; k < cnt[i]; ++k) r.push_back(kOddRadices[i]);
+  return r;
+}
+
+// largest power-of-two number of length-L columns one tile holds
+static int ragged_tile_C(const RaggedData& R, int64_t L, const std::vector<int>& rad) {
+  for (int C = 64; C >= 1; C >>= 1) {
+    if (C * L > R.tile_max) continue;
+    if (pass_threads(static_cast<int>(L), C, rad) > kRaggedThreads) continue;
+    return C;
+  }
+  return 0;
+}
+
+static bool ragged_direct_ok(const RaggedData& R, int64_t L) {
+  if (L > R.tile_max) return false;
+  auto rad = radices_pref(L, R.radix_pref);
+  if (rad.empty() || static_cast<int>(rad.size()) > kMaxStages) return false;
+  return ragged_tile_C(R, L, rad) >= 1;
+}
+
+static int ragged_sched(RaggedData& R, int64_t L) {
+  auto it = R.sched_of.find(static_cast<int>(L));
+  if (it != R.sched_of.end()) return it->second;
+  RaggedSched sc{};
+  sc.n = static_cast<int>(L);
+  auto rad = radices_pref(L, R.radix_pref);
+  sc.nstages = static_cast<int>(rad.size());
+  int64_t ns = 1;
+  for (size_t i = 0; i < rad.size(); ++i) {
+    sc.st[i].radix = rad[i];
+    sc.st[i].ns = static_cast<int>(ns);
+    sc.st[i].tw_off = static_cast<int>(R.tw_total);
+    sc.st[i].ns_magic = magic_for(static_cast<int>(ns));
+    R.stage_jobs.push_back({R.tw_total, static_cast<int>(ns), rad[i]});
+    R.tw_total += (rad[i] - 1) * ns;
+    ns *= rad[i];
+  }
+  if (R.tw_total >= (int64_t(1) << 31)) fail(FFTJH_ERR_UNSUPPORTED, "ragged twiddle pool too large");
+  R.scheds.push_back(sc);
+  const int id = static_cast<int>(R.scheds.size()) - 1;
+  R.sched_of[static_cast<int>(L)] = id;
+  return id;
+}
+
+// two-level split of L with both factors direct, nearest sqrt(L)
+static void ragged_two_level(RaggedData& R, RaggedItem& it, int64_t L) {
+  int64_t best_n2 = 0;
+  double best = 1e300;
+  for (int64_t q = 2; q * q <= L * 64 && q < L; ++q) {
+    if (L % q || !ragged_direct_ok(R, q) || !ragged_direct_ok(R, L / q)) continue;
+    const double d = std::fabs(std::log(static_cast<double>(q) / std::sqrt(static_cast<double>(L))));
+    if (d < best) {
+      best = d;
+      best_n2 = q;
+    }
+  }
+  if (!best_n2) fail(FFTJH_ERR_UNSUPPORTED, "ragged length " + std::to_string(L) + " has no two-level split");
+  it.n2 = static_cast<int>(best_n2);
+  it.n1 = static_cast<int>(L / best_n2);
+  it.sA = ragged_sched(R, it.n2);
+  it.sB = ragged_sched(R, it.n1);
+  auto radA = radices_pref(it.n2, R.radix_pref), radB = radices_pref(it.n1, R.radix_pref);
+  it.cA = ragged_tile_C(R, it.n2, radA);
+  it.cB = ragged_tile_C(R, it.n1, radB);
+  it.logcA = 0;
+  while ((1 << it.logcA) < it.cA) ++it.logcA;
+  it.logcB = 0;
+  while ((1 << it.logcB) < it.cB) ++it.logcB;
+  int lb = 0;
+  while ((int64_t(1) << (2 * lb)) < L) ++lb;
+  it.lo_bits = lb;
+  const int64_t K = int64_t(1) << lb, nh = (L + K - 1) / K;
+  it.tw_lo = R.tw_total;
+  R.pow_jobs.push_back({R.tw_total, K, 1, L});
+  R.tw_total += K;
+  it.tw_hi = R.tw_total;
+  R.pow_jobs.push_back({R.tw_total, nh, K, L});
+  R.tw_total += nh;
+}
+
+static double ragged_cost(const RaggedItem& it) {
+  const double n = it.n;
+  switch (it.kind) {
+    case kRaggedDirect: return n * std::log2(std::max(n, 2.0)) + 2000.0;
+    case kRaggedTwoLevel: return 2.6 * n * std::log2(n) + 6000.0;
+    case kRaggedRsDirect: return it.m * std::log2(static_cast<double>(it.m)) + 3.0 * n * it.w + 3000.0;
+    default: return 2.6 * it.m * std::log2(static_cast<double>(it.m)) + 3.0 * n * it.w + 8000.0;
+  }
+}
+
+static Layout layout_ragged(int dtype, int64_t count, const int64_t* sizes, const int64_t* offsets, int sign,
+                            int normalized, const fftjh_ragged_tuning* tune) {
+  check_dtype(dtype);
+  if (count < 1) fail(FFTJH_ERR_ARGUMENT, "ragged batch needs at least one transform");
+  if (count > (int64_t(1) << 30)) fail(FFTJH_ERR_UNSUPPORTED, "too many transforms");
+  if (!sizes) fail(FFTJH_ERR_ARGUMENT, "sizes is NULL");
+  if (sign != -1 && sign != 1) fail(FFTJH_ERR_ARGUMENT, "sign must be -1 or +1");
+  auto R = std::make_shared<RaggedData>();
+  R->dir = sign;
+  if (tune && tune->radix_pref) {
+    if (tune->radix_pref != 8 && tune->radix_pref != 16) fail(FFTJH_ERR_ARGUMENT, "radix_pref must be 8 or 16");
+    R->radix_pref = tune->radix_pref;
+  }
+  if (tune && tune->tile_max) {
+    if (tune->tile_max < 256 || tune->tile_max > kRaggedTile)
+      fail(FFTJH_ERR_ARGUMENT, "tile_max must be in [256, " + std::to_string(kRaggedTile) + "]");
+    R->tile_max = tune->tile_max;
+  }
+  const int64_t nmax = dtype == FFTJH_F32 ? 65536 : 32768;
+  int64_t extent = 0, off = 0;
+  std::map<int64_t, int64_t> corr_of;
+  R->items.reserve(count);
+  for (int64_t i = 0; i < count; ++i) {
+    const int64_t n = sizes[i];
+    if (n < 1) fail(FFTJH_ERR_ARGUMENT, "transform length must be positive (got " + std::to_string(n) + ")");
+    if (n > nmax) fail(FFTJH_ERR_UNSUPPORTED, "ragged lengths above " + std::to_string(nmax) + " need a plan of their own");
+    const int64_t o = offsets ? offsets[i] : off;
+    if (o < 0) fail(FFTJH_ERR_ARGUMENT, "offsets must be non-negative");
+    off = o + n;
+    extent = std::max(extent, o + n);
+    RaggedItem it{};
+    it.off = o;
+    it.n = static_cast<int>(n);
+    it.scale = (sign > 0 && normalized) ? 1.0 / static_cast<double>(n) : 1.0;
+    int a, cnt[5];
+    if (factor13(n, a, cnt)) {
+      if (ragged_direct_ok(*R, n)) {
+        it.kind = kRaggedDirect;
+        it.sA = ragged_sched(*R, n);
+      } else {
+        it.kind = kRaggedTwoLevel;
+        ragged_two_level(*R, it, n);
+      }
+    } else {
+      const RsParams rp = rs_params(dtype, n);
+      it.m = static_cast<int>(rp.m);
+      it.w = rp.w;
+      it.beta = rp.beta;
+      auto c = corr_of.find(n);
+      if (c == corr_of.end()) {
+        R->corr_jobs.push_back({R->corr_total, n, rp.m, rp.w, rp.beta, it.scale});
+        c = corr_of.emplace(n, R->corr_total).first;
+        R->corr_total += n;
+      }
+      it.corr_off = c->second;
+      if (ragged_direct_ok(*R, rp.m)) {
+        it.kind = kRaggedRsDirect;
+        it.sA = ragged_sched(*R, rp.m);
+      } else {
+        it.kind = kRaggedRsTwo;
+        ragged_two_level(*R, it, rp.m);
+      }
+    }
+    R->counts[it.kind]++;
+    R->items.push_back(it);
+  }
+  // predicted per-item cost: the caller's (a learned model) or the analytic one
+  R->cost.resize(count);
+  for (int64_t i = 0; i < count; ++i)
+    R->cost[i] = (tune && tune->costs) ? tune->costs[i] : ragged_cost(R->items[i]);
+  // workgroups: resident ones (set at materialize time), or the tuning's
+  R->groups = (tune && tune->groups > 0) ? tune->groups : 0;
+  Layout L;
+  L.dtype = dtype;
+  L.kind = FFTJH_KIND_RAGGED;
+  L.sign = sign;
+  L.rank = 1;
+  L.dims[0] = count;
+  L.batch = 1;
+  L.scale = 1.0;  // per-transform scales live in the items
+  L.in_elems = L.out_elems = extent;
+  Step st;
+  st.kind = ST_RAGGED;
+  st.src = RX;
+  st.dst = RY;
+  st.dir = sign;
+  st.rg = R;
+  L.steps.push_back(st);
+  return L;
+}
+
+// longest-predicted-first onto the least-loaded workgroup (deterministic)
+static void ragged_assign(RaggedData& R) {
+  const int G = R.groups;
+  const int64_t count = static_cast<int64_t>(R.items.size());
+  std::vector<int64_t> order(count);
+  for (int64_t i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return R.cost[a] > R.cost[b]; });
+  std::vector<std::vector<int32_t>> lists(G);
+  std::vector<int64_t> need(G, 0);
+  using Slot = std::pair<double, int>;
+  std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+  for (int g = 0; g < G; ++g) heap.push({0.0, g});
+  for (int64_t i : order) {
+    auto [load, g] = heap.top();
+    heap.pop();
+    lists[g].push_back(static_cast<int32_t>(i));
+    const RaggedItem& it = R.items[i];
+    const int64_t ws = it.kind == kRaggedTwoLevel ? it.n : it.kind == kRaggedRsTwo ? 2 * int64_t(it.m) : 0;
+    need[g] = std::max(need[g], ws);
+    heap.push({load + R.cost[i], g});
+  }
+  R.wg_ptr.assign(G + 1, 0);
+  R.wg_items.clear();
+  R.wg_ws.assign(G, 0);
+  R.ws_total = 0;
+  for (int g = 0; g < G; ++g) {
+    R.wg_ptr[g] = static_cast<int32_t>(R.wg_items.size());
+    R.wg_items.insert(R.wg_items.end(), lists[g].begin(), lists[g].end());
+    R.wg_ws[g] = R.ws_total;
+    R.ws_total += need[g];
+  }
+  R.wg_ptr[G] = static_cast<int32_t>(R.wg_items.size());
+}
+
 static std::string describe(const Layout& L) {
   static const char* roles[] = {"x", "y", "s0", "s1", "s2", "s3"};
   std::ostringstream o;
-  o << (L.kind == FFTJH_KIND_C2C ? "c2c" : L.kind == FFTJH_KIND_R2C ? "r2c" : "c2r")
+  o << (L.kind == FFTJH_KIND_C2C ? "c2c" : L.kind == FFTJH_KIND_R2C ? "r2c" : L.kind == FFTJH_KIND_C2R ? "c2r" : "ragged")
     << (L.dtype == FFTJH_F32 ? " f32" : " f64") << " dims=";
   for (int i = 0; i < L.rank; ++i) o << (i ? "x" : "") << L.dims[i];
   o << " batch=" << L.batch << " sign=" << L.sign << " launches=" << L.steps.size() << "\n";
@@ -781,6 +1037,12 @@ static std::string describe(const Layout& L) {
         << " in_stride=" << s.d.in.stride << " out_stride=" << s.d.out.stride;
       if (s.d.tw_period) o << " twiddle=w_" << s.d.tw_period;
       if (s.d.scale != 1.0) o << " scale=" << s.d.scale;
+    } else if (s.kind == ST_RAGGED) {
+      const RaggedData& R = *s.rg;
+      o << " transforms=" << R.items.size() << " direct=" << R.counts[0] << " two_level=" << R.counts[1]
+        << " resample=" << R.counts[2] << " resample_two_level=" << R.counts[3] << " tile_lengths=" << R.scheds.size()
+        << " radix_pref=" << R.radix_pref << " tile_max=" << R.tile_max;
+      if (R.groups) o << " workgroups=" << R.groups;
     } else if (s.kind == ST_RS_DECONV || s.kind == ST_RS_INTERP) {
       o << " n=" << s.n << " m=" << s.m << " w=" << s.w << " beta=" << s.beta;
     } else {
@@ -854,6 +1116,56 @@ static void materialize(fftjh_plan_s& P) {
       s.d.grid = resident > 0 ? std::min(resident, s.d.ngroups) : s.d.ngroups;
       s.desc = std::make_shared<DevBuf>(sizeof(PassDesc));
       hip_check(hipMemcpy(s.desc->p, &s.d, sizeof(PassDesc), hipMemcpyHostToDevice), "descriptor upload");
+    } else if (s.kind == ST_RAGGED) {
+      RaggedData& R = *s.rg;
+      if (R.groups <= 0) {
+        const int res = ragged_resident_groups(dt, R.dir);
+        R.groups = res > 0 ? res : 1024;
+      }
+      R.groups = static_cast<int>(std::min<int64_t>(R.groups, static_cast<int64_t>(R.items.size())));
+      ragged_assign(R);
+      auto upload = [&](const void* src, size_t bytes) {
+        auto b = std::make_shared<DevBuf>(std::max<size_t>(bytes, 16));
+        if (bytes) hip_check(hipMemcpy(b->p, src, bytes, hipMemcpyHostToDevice), "ragged table upload");
+        return b;
+      };
+      R.d_items = upload(R.items.data(), R.items.size() * sizeof(RaggedItem));
+      R.d_wgptr = upload(R.wg_ptr.data(), R.wg_ptr.size() * sizeof(int32_t));
+      R.d_wgitems = upload(R.wg_items.data(), R.wg_items.size() * sizeof(int32_t));
+      R.d_wgws = upload(R.wg_ws.data(), R.wg_ws.size() * sizeof(int64_t));
+      R.d_scheds = upload(R.scheds.data(), R.scheds.size() * sizeof(RaggedSched));
+      R.d_tw = std::make_shared<DevBuf>(std::max<int64_t>(R.tw_total, 1) * es);
+      for (const auto& j : R.stage_jobs)
+        hip_check(launch_tw_stage(dt, static_cast<char*>(R.d_tw->p) + j.off * es, j.ns, j.R, R.dir, 0), "twiddles");
+      for (const auto& j : R.pow_jobs)
+        hip_check(launch_tw_pow(dt, static_cast<char*>(R.d_tw->p) + j.off * es, j.count, j.mult, j.L, R.dir, 0),
+                  "twiddles");
+      R.d_corr = std::make_shared<DevBuf>(std::max<int64_t>(R.corr_total, 1) * (es / 2));
+      if (!R.corr_jobs.empty()) {
+        int wmax = 0;
+        for (const auto& j : R.corr_jobs) wmax = std::max(wmax, j.w);
+        std::map<int, std::shared_ptr<DevBuf>> nodes;
+        for (const auto& j : R.corr_jobs) {
+          auto& nb = nodes[j.w];
+          const int nq = 2 * j.w + 20;
+          if (!nb) {
+            std::vector<double> zq, wq;
+            gauss_legendre(nq, zq, wq);
+            nb = std::make_shared<DevBuf>(2 * nq * sizeof(double));
+            hip_check(hipMemcpy(nb->p, zq.data(), nq * sizeof(double), hipMemcpyHostToDevice), "nodes");
+            hip_check(hipMemcpy(static_cast<double*>(nb->p) + nq, wq.data(), nq * sizeof(double),
+                                hipMemcpyHostToDevice),
+                      "nodes");
+          }
+          hip_check(launch_rs_corr(dt, static_cast<char*>(R.d_corr->p) + j.off * (es / 2), j.n, j.m, j.w, j.beta,
+                                   static_cast<const double*>(nb->p), static_cast<const double*>(nb->p) + nq, nq,
+                                   j.scale, 0),
+                    "corr kernel");
+        }
+        hip_check(hipDeviceSynchronize(), "corr kernel");
+      }
+      hip_check(hipDeviceSynchronize(), "ragged tables");
+      R.d_ws = std::make_shared<DevBuf>(std::max<int64_t>(R.ws_total, 1) * es);
     } else if (s.kind == ST_R2C_TWIST || s.kind == ST_C2R_PRE) {
       // w_k = exp(dir*2*pi*i*k/n), k = 0..h (real.jl:26-39)
       const int64_t cnt = s.h + 1, n = s.n;
@@ -912,6 +1224,22 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
       return launch_rs_interp(dt, ptr[s.src], ptr[s.dst], s.n, s.m, s.w, s.beta, s.batch, s.Bi, s.gb, st);
     case ST_STRIDED_COPY:
       return launch_strided_copy(dt, ptr[s.src], ptr[s.dst], s.n, s.batch, s.Bi, s.ga, s.gb, st);
+    case ST_RAGGED: {
+      const RaggedData& R = *s.rg;
+      RaggedLaunch rl;
+      rl.groups = R.groups;
+      rl.items = R.d_items->p;
+      rl.wg_ptr = R.d_wgptr->p;
+      rl.wg_items = R.d_wgitems->p;
+      rl.scheds = R.d_scheds->p;
+      rl.tw = R.d_tw->p;
+      rl.corr = R.d_corr->p;
+      rl.x = ptr[s.src];
+      rl.y = ptr[s.dst];
+      rl.ws = R.d_ws->p;
+      rl.wg_ws = R.d_wgws->p;
+      return launch_ragged(dt, s.dir, rl, st);
+    }
   }
   return hipErrorInvalidValue;
 }
@@ -970,6 +1298,23 @@ int fftjh_plan_c2c_tuned(fftjh_plan* plan, int dtype, int rank, const int64_t* d
   return guarded([&] {
     if (plan) *plan = nullptr;
     make_plan(plan, layout_c2c(dtype, rank, dims, batch, sign, scale, tune));
+  });
+}
+
+int fftjh_plan_ragged(fftjh_plan* plan, int dtype, int64_t count, const int64_t* sizes, const int64_t* offsets,
+                      int sign, int normalized, unsigned flags) {
+  return guarded([&] {
+    if (plan) *plan = nullptr;
+    if (flags) fail(FFTJH_ERR_ARGUMENT, "flags must be 0");
+    make_plan(plan, layout_ragged(dtype, count, sizes, offsets, sign, normalized, nullptr));
+  });
+}
+
+int fftjh_plan_ragged_tuned(fftjh_plan* plan, int dtype, int64_t count, const int64_t* sizes,
+                            const int64_t* offsets, int sign, int normalized, const fftjh_ragged_tuning* tune) {
+  return guarded([&] {
+    if (plan) *plan = nullptr;
+    make_plan(plan, layout_ragged(dtype, count, sizes, offsets, sign, normalized, tune));
   });
 }
 

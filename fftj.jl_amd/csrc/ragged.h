@@ -1,0 +1,55 @@
+// Descriptors of the ragged persistent kernel (host planner <-> device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "common.h"
+
+namespace fftjh {
+
+constexpr int kRaggedThreads = 256;  // workgroup size
+constexpr int kRaggedTile = 4096;    // complex elements one LDS tile holds
+
+enum RaggedKind : int32_t { kRaggedDirect = 0, kRaggedTwoLevel = 1, kRaggedRsDirect = 2, kRaggedRsTwo = 3 };
+
+// stage chain of one tile length (tw_off into the shared twiddle pool)
+struct RaggedSched {
+  int32_t n;
+  int32_t nstages;
+  StageDesc st[kMaxStages];
+};
+
+struct RaggedItem {
+  int64_t off;       // complex offset in x and y
+  int32_t n;         // transform length
+  int32_t kind;      // RaggedKind
+  int32_t sA, sB;    // schedules: direct / column tiles (sA), row tiles (sB)
+  int32_t n1, n2;    // two-level split of n (or of m): rows n1, columns n2
+  int32_t cA, logcA; // columns per phase-A tile
+  int32_t cB, logcB; // rows per phase-B tile
+  int32_t lo_bits;   // two-level twiddle w^e = pool[tw_hi + (e >> lo_bits)] * pool[tw_lo + (e & mask)]
+  int32_t m, w;      // resampling grid and taps
+  int32_t pad_;
+  int64_t tw_hi, tw_lo;
+  int64_t corr_off;  // into the correction pool (reals)
+  double beta;
+  double scale;
+};
+
+struct RaggedLaunch {
+  int groups;
+  const void* items;
+  const void* wg_ptr;
+  const void* wg_items;
+  const void* scheds;
+  const void* tw;
+  const void* corr;
+  const void* x;
+  void* y;
+  void* ws;
+  const void* wg_ws;  // int64 per workgroup: its workspace offset (complex elements)
+};
+
+hipError_t launch_ragged(int dtype, int dir, const RaggedLaunch& L, hipStream_t st);
+int ragged_resident_groups(int dtype, int dir);
+
+}  // namespace fftjh

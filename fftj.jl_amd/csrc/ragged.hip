@@ -1,0 +1,234 @@
+// Ragged batch: thousands of independent complex transforms of uneven
+// lengths, packed back to back with an offset table, in ONE persistent
+// launch.  Each workgroup walks its own list of whole transforms (assigned
+// on the host, longest predicted cost first, to the least-loaded workgroup),
+// so every transform's arithmetic is the same whichever workgroup runs it.
+//
+// Per transform, by kind:
+//   DIRECT    n fits one LDS tile: load, all stages in LDS, store.
+//   TWO_LEVEL n = n1*n2 (four-step inside one workgroup): column tiles
+//             (length n2, twiddle w_n^{j1*k2}) into the workgroup's
+//             workspace, then row tiles (length n1) to natural order.
+//   RS_DIRECT non-smooth n, resampled onto m that fits one tile: the
+//             deconvolution lands in LDS, the m-point transform runs there,
+//             the w-tap interpolation reads it (resample.jl:258-276).
+//   RS_TWO    the same with m two-level through the workspace.
+// The reference has no ragged API: its baseline is a loop of 1-D plans.
+#include "butterflies.h"
+#include "common.h"
+#include "launch.h"
+#include "passcommon.h"
+#include "ragged.h"
+#include "stages.h"
+
+namespace fftjh {
+
+namespace {
+
+constexpr int kRT = kRaggedThreads;
+
+template <typename T, int DIR>
+__device__ void tile_stages(V2<T>* lds, const RaggedSched& sc, int C, int logC, const V2<T>* tw) {
+  const int tid = static_cast<int>(threadIdx.x);
+  const int c = tid & (C - 1), jt = tid >> logC, jstep = kRT >> logC;
+  for (int s = 0; s < sc.nstages; ++s) {
+    const StageDesc& st = sc.st[s];
+    switch (st.radix) {
+      case 2: run_stage<T, DIR, 2>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 3: run_stage<T, DIR, 3>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 4: run_stage<T, DIR, 4>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 5: run_stage<T, DIR, 5>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 7: run_stage<T, DIR, 7>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 8: run_stage<T, DIR, 8>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 11: run_stage<T, DIR, 11>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 13: run_stage<T, DIR, 13>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      default: run_stage<T, DIR, 16>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+    }
+  }
+}
+
+// length-n transform src -> dst (both contiguous, global) as two tile phases
+// through `mid` (global, n elements); dst may equal src
+template <typename T, int DIR>
+__device__ void two_level(V2<T>* lds, const RaggedItem& it, const RaggedSched* scheds, const V2<T>* tw,
+                          const V2<T>* src, V2<T>* mid, V2<T>* dst, T scale) {
+  const int tid = static_cast<int>(threadIdx.x);
+  const int n1 = it.n1, n2 = it.n2;
+  const RaggedSched& sa = scheds[it.sA];
+  const RaggedSched& sb = scheds[it.sB];
+  const V2<T>* thi = tw + it.tw_hi;
+  const V2<T>* tlo = tw + it.tw_lo;
+  const uint32_t lo_mask = (1u << it.lo_bits) - 1u;
+  // phase A: columns j1 (stride n1) of length n2, twiddle w_n^{j1*k2}, mid[j1 + n1*k2]
+  {
+    const int C = it.cA, logC = it.logcA;
+    for (int c0 = 0; c0 < n1; c0 += C) {
+      for (int f = tid; f < C * n2; f += kRT) {
+        const int c = f & (C - 1), i = f >> logC;
+        const int col = c0 + c;
+        lds[lpad(i * C + c)] = col < n1 ? src[col + static_cast<int64_t>(n1) * i] : mk<T>(T(0), T(0));
+      }
+      __syncthreads();
+      tile_stages<T, DIR>(lds, sa, C, logC, tw);
+      for (int f = tid; f < C * n2; f += kRT) {
+        const int c = f & (C - 1), k2 = f >> logC;
+        const int col = c0 + c;
+        if (col < n1) {
+          const uint32_t e = static_cast<uint32_t>(col) * static_cast<uint32_t>(k2);
+          const V2<T> w = cmul<T>(thi[e >> it.lo_bits], tlo[e & lo_mask]);
+          mid[col + static_cast<int64_t>(n1) * k2] = cmul<T>(lds[lpad(k2 * C + c)], w);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  // phase B: rows k2 of mid (length n1), output dst[k2 + n2*k1]
+  {
+    const int C = it.cB, logC = it.logcB;
+    for (int r0 = 0; r0 < n2; r0 += C) {
+      for (int f = tid; f < C * n1; f += kRT) {
+        const int c = f & (C - 1), i = f >> logC;
+        const int row = r0 + c;
+        lds[lpad(i * C + c)] = row < n2 ? mid[static_cast<int64_t>(row) * n1 + i] : mk<T>(T(0), T(0));
+      }
+      __syncthreads();
+      tile_stages<T, DIR>(lds, sb, C, logC, tw);
+      for (int f = tid; f < C * n1; f += kRT) {
+        const int c = f & (C - 1), k1 = f >> logC;
+        const int row = r0 + c;
+        if (row < n2) dst[row + static_cast<int64_t>(n2) * k1] = cscale<T>(lds[lpad(k1 * C + c)], scale);
+      }
+      __syncthreads();
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+}
+
+__device__ __forceinline__ double es_weight(double z, double beta) {
+  return fabs(z) >= 1.0 ? 0.0 : exp(beta * (sqrt(1.0 - z * z) - 1.0));
+}
+
+// y[k] = sum_{i<w} psi((delta + h - 1 - i)/h) * G[(q - h + 1 + i) mod m],  k*m = q*n + r
+template <typename T, bool FROM_LDS>
+__device__ void interp(const RaggedItem& it, const V2<T>* G, V2<T>* y) {
+  const int n = it.n, m = it.m, w = it.w, h = it.w / 2;
+  for (int k = static_cast<int>(threadIdx.x); k < n; k += kRT) {
+    const int64_t km = static_cast<int64_t>(k) * m;
+    const int64_t q = km / n, r = km - q * n;
+    const double delta = static_cast<double>(r) / n;
+    int l = static_cast<int>(q) - h + 1;
+    if (l < 0) l += m;
+    V2<T> acc = mk<T>(T(0), T(0));
+    for (int i = 0; i < w; ++i) {
+      const T wt = static_cast<T>(es_weight((delta + (h - 1 - i)) / h, it.beta));
+      int li = l + i;
+      if (li >= m) li -= m;
+      const V2<T> g = FROM_LDS ? G[lpad(li)] : G[li];
+      acc.x = fma(wt, g.x, acc.x);
+      acc.y = fma(wt, g.y, acc.y);
+    }
+    y[k] = acc;
+  }
+}
+
+}  // namespace
+
+template <typename T, int DIR>
+__global__ void __launch_bounds__(kRaggedThreads)
+k_ragged(const RaggedItem* __restrict__ items, const int32_t* __restrict__ wg_ptr,
+         const int32_t* __restrict__ wg_items, const RaggedSched* __restrict__ scheds, const V2<T>* tw,
+         const T* __restrict__ corr, const V2<T>* x, V2<T>* y, V2<T>* ws, const int64_t* __restrict__ wg_ws) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  V2<T>* lds = reinterpret_cast<V2<T>*>(smem);
+  const int tid = static_cast<int>(threadIdx.x);
+  const int wg = static_cast<int>(blockIdx.x);
+  // this workgroup's workspace: two-level items use n elements, two-level
+  // resampling items 2m (grid, then the phase-A intermediate)
+  V2<T>* wsA = ws + wg_ws[wg];
+  for (int p = wg_ptr[wg]; p < wg_ptr[wg + 1]; ++p) {
+    const RaggedItem& it = items[wg_items[p]];
+    const V2<T>* xi = x + it.off;
+    V2<T>* yi = y + it.off;
+    const T scale = static_cast<T>(it.scale);
+    const int n = it.n;
+    if (it.kind == kRaggedDirect) {
+      const RaggedSched& sc = scheds[it.sA];
+      for (int i = tid; i < n; i += kRT) lds[lpad(i)] = xi[i];
+      __syncthreads();
+      tile_stages<T, DIR>(lds, sc, 1, 0, tw);
+      for (int k = tid; k < n; k += kRT) yi[k] = cscale<T>(lds[lpad(k)], scale);
+      __syncthreads();
+    } else if (it.kind == kRaggedTwoLevel) {
+      two_level<T, DIR>(lds, it, scheds, tw, xi, wsA, yi, scale);
+    } else {
+      // resampling: corr carries the scale; the m-point transform is unscaled
+      const int m = it.m, hp = (n + 1) / 2, hi0 = m - n + hp;
+      const T* cr = corr + it.corr_off;
+      if (it.kind == kRaggedRsDirect) {
+        for (int j = tid; j < m; j += kRT) {
+          const int src = j < hp ? j : (j >= hi0 ? j - (m - n) : -1);
+          lds[lpad(j)] = src >= 0 ? cscale<T>(xi[src], cr[src]) : mk<T>(T(0), T(0));
+        }
+        __syncthreads();
+        tile_stages<T, DIR>(lds, scheds[it.sA], 1, 0, tw);
+        interp<T, true>(it, lds, yi);
+        __syncthreads();
+      } else {
+        V2<T>* grid = wsA;
+        V2<T>* mid = wsA + m;
+        for (int j = tid; j < m; j += kRT) {
+          const int src = j < hp ? j : (j >= hi0 ? j - (m - n) : -1);
+          grid[j] = src >= 0 ? cscale<T>(xi[src], cr[src]) : mk<T>(T(0), T(0));
+        }
+        __threadfence_block();
+        __syncthreads();
+        two_level<T, DIR>(lds, it, scheds, tw, grid, mid, grid, T(1));
+        interp<T, false>(it, grid, yi);
+        __threadfence_block();
+        __syncthreads();
+      }
+    }
+  }
+}
+
+template <typename T, int DIR>
+static hipError_t launch_ragged_t(const RaggedLaunch& L, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ragged<T, DIR>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+    attr = true;
+  }
+  const size_t lds = static_cast<size_t>(lds_elems(kRaggedTile)) * sizeof(V2<T>);
+  hipLaunchKernelGGL((k_ragged<T, DIR>), dim3(L.groups), dim3(kRaggedThreads), lds, st,
+                     static_cast<const RaggedItem*>(L.items), static_cast<const int32_t*>(L.wg_ptr),
+                     static_cast<const int32_t*>(L.wg_items), static_cast<const RaggedSched*>(L.scheds),
+                     static_cast<const V2<T>*>(L.tw), static_cast<const T*>(L.corr),
+                     static_cast<const V2<T>*>(L.x), static_cast<V2<T>*>(L.y), static_cast<V2<T>*>(L.ws),
+                     static_cast<const int64_t*>(L.wg_ws));
+  return hipGetLastError();
+}
+
+hipError_t launch_ragged(int dtype, int dir, const RaggedLaunch& L, hipStream_t st) {
+  if (dtype == 0) return dir < 0 ? launch_ragged_t<float, -1>(L, st) : launch_ragged_t<float, 1>(L, st);
+  return dir < 0 ? launch_ragged_t<double, -1>(L, st) : launch_ragged_t<double, 1>(L, st);
+}
+
+int ragged_resident_groups(int dtype, int dir) {
+  const void* fn = dtype == 0 ? (dir < 0 ? reinterpret_cast<const void*>(&k_ragged<float, -1>)
+                                         : reinterpret_cast<const void*>(&k_ragged<float, 1>))
+                              : (dir < 0 ? reinterpret_cast<const void*>(&k_ragged<double, -1>)
+                                         : reinterpret_cast<const void*>(&k_ragged<double, 1>));
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+  const size_t lds = static_cast<size_t>(lds_elems(kRaggedTile)) * (dtype == 0 ? 8 : 16);
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kRaggedThreads, lds) != hipSuccess) return 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  return per_cu * cus;
+}
+
+}  // namespace fftjh

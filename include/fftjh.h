@@ -45,6 +45,7 @@ extern "C" {
 #define FFTJH_KIND_C2C 0
 #define FFTJH_KIND_R2C 1
 #define FFTJH_KIND_C2R 2
+#define FFTJH_KIND_RAGGED 3
 
 typedef struct fftjh_plan_s* fftjh_plan;
 /* hipStream_t, passed as an opaque pointer (NULL = the null stream) */
@@ -82,6 +83,28 @@ typedef struct fftjh_tuning {
 /* fftjh_plan_c2c with explicit search-space choices (NULL = planner's). */
 int fftjh_plan_c2c_tuned(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
                          double scale, const fftjh_tuning* tune);
+
+/* Ragged batch: `count` independent complex transforms of lengths sizes[i]
+ * at complex offsets offsets[i] of x and y (NULL = packed back to back), run
+ * by one persistent kernel launch.  sign -1 forward; +1 inverse, scaled by
+ * 1/sizes[i] when normalized != 0.  Lengths up to 65536 (FP32) / 32768
+ * (FP64).  No reference counterpart: FFTJ has no batch or ragged API, and a
+ * caller loops over 1-D plans (plan.jl:207-213); this stands in for that loop. */
+int fftjh_plan_ragged(fftjh_plan* plan, int dtype, int64_t count, const int64_t* sizes, const int64_t* offsets,
+                      int sign, int normalized, unsigned flags);
+
+/* Ragged search-space overrides (zero fields keep the planner's choice). */
+typedef struct fftjh_ragged_tuning {
+  int32_t groups;        /* persistent workgroups (0 = all resident) */
+  int32_t radix_pref;    /* 16 or 8: radix of the power-of-two tower */
+  int32_t tile_max;      /* largest length done in one LDS tile, 256..4096 */
+  int32_t pad_;
+  const double* costs;   /* per-transform predicted cost for the workgroup
+                            assignment (NULL = the analytic model) */
+} fftjh_ragged_tuning;
+
+int fftjh_plan_ragged_tuned(fftjh_plan* plan, int dtype, int64_t count, const int64_t* sizes,
+                            const int64_t* offsets, int sign, int normalized, const fftjh_ragged_tuning* tune);
 
 /* Real-to-complex forward plan: n reals -> n/2+1 complex per transform.
  * Replaces plan_rfft(::Type{T}, n)                        real.jl:41-52 */

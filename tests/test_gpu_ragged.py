@@ -1,0 +1,87 @@
+"""GPU parity of the ragged persistent kernel: every kind of transform it
+runs (one LDS tile, two-level, resampled in one tile, resampled two-level)
+against the FP64 oracle, in-place use, and the 10,000-transform benchmark
+batch (sampled transforms plus a whole-batch round trip)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from conftest import F, inputs_c, rel_l2, to_dev, to_host, tol_for, torch_dtype
+
+pytestmark = pytest.mark.gpu
+
+# one tile, two-level (4096 < n), resampled (primes / non-smooth) in one tile and two-level
+SIZES = [1, 2, 3, 13, 64, 100, 143, 1000, 2310, 4096, 4913, 6000, 8192, 3 * 5 * 7 * 11 * 13, 65536,
+         17, 257, 1009, 2053, 4099, 8191, 4097, 9973]
+
+
+def _check(prec, sizes, seed=5, sign=-1):
+    sizes = np.asarray(sizes, dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    x = inputs_c(seed, int(sizes.sum()), prec)
+    p = F.plan_ragged(torch_dtype(prec), sizes, sign=sign)
+    y = to_host(p(to_dev(x)))
+    worst = 0.0
+    for n, o in zip(sizes.tolist(), offs.tolist()):
+        ref = O.fft(x[o:o + n].astype(np.complex128), sign=sign, scale=1.0 if sign < 0 else 1.0 / n)
+        err = rel_l2(y[o:o + n], ref)
+        assert err <= tol_for(prec, n), (prec, n, err)
+        worst = max(worst, err)
+    return p, x, y
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_ragged_all_kinds(prec):
+    sizes = [n for n in SIZES if prec == "f32" or n <= 32768]
+    p, _, _ = _check(prec, sizes)
+    d = p.describe()
+    for kind in ("direct=", "two_level=", "resample=", "resample_two_level="):
+        assert kind in d
+    assert " two_level=0" not in d and " resample=0" not in d and "resample_two_level=0" not in d
+
+
+def test_ragged_inverse_and_inplace():
+    sizes = [64, 4096, 8192, 257, 8191, 6000]
+    p, x, y = _check("f32", sizes, sign=1)
+    xd = to_dev(x)
+    assert p(xd, xd) is xd
+    assert np.array_equal(to_host(xd), y)
+
+
+def test_ragged_offsets_with_gaps():
+    sizes = np.array([100, 4096, 257], dtype=np.int64)
+    offs = np.array([10, 200, 5000], dtype=np.int64)
+    x = inputs_c(3, 6000, "f32")
+    y0 = np.zeros_like(x)
+    p = F.plan_ragged(torch.float32, sizes, offs)
+    yd = to_dev(y0)
+    p(yd, to_dev(x))
+    y = to_host(yd)
+    for n, o in zip(sizes, offs):
+        assert rel_l2(y[o:o + n], O.fft(x[o:o + n].astype(np.complex128))) <= tol_for("f32", n)
+    assert not y[:10].any() and not y[100 + 10:200].any()   # gaps untouched
+
+
+def test_ragged_benchmark_batch():
+    sizes = F.ragged.benchmark_sizes()
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    x = F.rng.uniform_complex(1, int(sizes.sum())).astype(np.complex64)
+    xd = to_dev(x)
+    pf = F.plan_ragged(torch.float32, sizes)
+    pi = F.plan_ragged(torch.float32, sizes, sign=1)
+    yd = pf(xd)
+    y = to_host(yd)
+    rng = np.random.default_rng(0)
+    picks = list(rng.choice(len(sizes), 40, replace=False)) + [int(np.argmax(sizes)), int(np.argmin(sizes))]
+    for i in picks:
+        n, o = int(sizes[i]), int(offs[i])
+        assert rel_l2(y[o:o + n], O.fft(x[o:o + n].astype(np.complex128))) <= tol_for("f32", n), n
+    back = to_host(pi(yd))
+    assert rel_l2(back, x) <= 2 * tol_for("f32", 65536)
+    # the plain per-transform loop computes the same transforms
+    loop = F.ragged.PlainLoop(torch.float32, sizes[:300], offs[:300])
+    y2 = torch.zeros_like(xd)
+    loop(y2, xd)
+    e = int(offs[299] + sizes[299])
+    assert rel_l2(to_host(y2)[:e], y[:e]) <= tol_for("f32", 65536)
