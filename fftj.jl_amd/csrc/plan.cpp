@@ -817,6 +817,7 @@ static int ragged_tile_C(const RaggedData& R, int64_t L, const std::vector<int>&
 }
 
 static bool ragged_direct_ok(const RaggedData& R, int64_t L) {
+  if (L == 1) return true;  // no stages: a scaled copy
   if (L > R.tile_max) return false;
   auto rad = radices_pref(L, R.radix_pref);
   if (rad.empty() || static_cast<int>(rad.size()) > kMaxStages) return false;
