@@ -123,11 +123,27 @@ def ragged_leg(F, quick=False):
     default_ms = F.tune.time_plan(F.plan_ragged(torch.float32, sizes), y, x, iters=20)
     loop = F.ragged.PlainLoop(torch.float32, sizes)
     plain_ms = F.tune.time_plan(loop, y, x, iters=3, warmup=1)
+    # both sides captured in a HIP graph (no host launch cost): the ragged
+    # plan is one node, the plain loop one node per transform
+    graph_ms = {}
+    for name, fn in (("ragged", lambda: plan(y, x)), ("plain_loop", lambda: loop(y, x))):
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            fn()
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            fn()
+        graph_ms[name] = F.tune.time_plan(lambda: g.replay(), y, x, iters=5 if name == "plain_loop" else 20)
+        del g
     bytes_ = 16.0 * float(sizes.sum())
     flops = float((5.0 * sizes * np.log2(np.maximum(sizes, 2))).sum())
     return {"transforms": int(len(sizes)), "distinct_sizes": int(len(np.unique(sizes))),
             "sum_n": int(sizes.sum()), "tuned_ms": round(tuned_ms, 4), "default_ms": round(default_ms, 4),
             "plain_loop_ms": round(plain_ms, 3), "speedup_tuned_vs_plain": round(plain_ms / tuned_ms, 2),
+            "graph_ragged_ms": round(graph_ms["ragged"], 4), "graph_plain_loop_ms": round(graph_ms["plain_loop"], 3),
+            "speedup_in_graphs": round(graph_ms["plain_loop"] / graph_ms["ragged"], 2),
             "tuned_config": cfg, "tuning_s": round(tune_s, 2),
             "tuned_gbs": round(bytes_ / (tuned_ms * 1e-3) / 1e9, 1),
             "tuned_gflops": round(flops / (tuned_ms * 1e-3) / 1e9, 1),
