@@ -135,7 +135,7 @@ def ragged_leg(F, quick=False):
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             fn()
-        graph_ms[name] = F.tune.time_plan(lambda: g.replay(), y, x, iters=5 if name == "plain_loop" else 20)
+        graph_ms[name] = F.tune.time_plan(lambda _y, _x: g.replay(), y, x, iters=5 if name == "plain_loop" else 20)
         del g
     bytes_ = 16.0 * float(sizes.sum())
     flops = float((5.0 * sizes * np.log2(np.maximum(sizes, 2))).sum())
