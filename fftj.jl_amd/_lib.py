@@ -18,9 +18,11 @@ KIND_C2C, KIND_R2C, KIND_C2R, KIND_RAGGED = 0, 1, 2, 3
 # every entry point include/fftjh.h declares
 EXPORTS = (
     "fftjh_plan_c2c", "fftjh_plan_r2c", "fftjh_plan_c2r", "fftjh_execute", "fftjh_execute_checked",
-    "fftjh_plan_info", "fftjh_plan_describe", "fftjh_describe_request", "fftjh_plan_destroy",
+    "fftjh_plan_info", "fftjh_plan_describe", "fftjh_describe_request", "fftjh_describe_tuned", "fftjh_plan_destroy",
     "fftjh_last_error", "fftjh_version", "fftjh_profile_launches", "fftjh_plan_c2c_tuned", "fftjh_plan_ragged",
-    "fftjh_plan_ragged_tuned",
+    "fftjh_plan_ragged_tuned", "fftjh_mlp_create", "fftjh_mlp_destroy", "fftjh_mlp_param_count",
+    "fftjh_mlp_set_params", "fftjh_mlp_get_params", "fftjh_mlp_predict", "fftjh_mlp_gradients",
+    "fftjh_mlp_train_step", "fftjh_mlp_last_error",
 )
 
 
@@ -90,6 +92,16 @@ def lib():
         L.fftjh_plan_ragged.argtypes = [ctypes.POINTER(vp), ci, i64, P64, P64, ci, ci, ctypes.c_uint]
         L.fftjh_plan_ragged_tuned.argtypes = [ctypes.POINTER(vp), ci, i64, P64, P64, ci, ci,
                                               ctypes.POINTER(RaggedTuning)]
+        L.fftjh_mlp_create.argtypes = [ctypes.POINTER(vp), ci, ci]
+        L.fftjh_mlp_destroy.argtypes = [vp]
+        L.fftjh_mlp_param_count.argtypes = [vp]
+        L.fftjh_mlp_param_count.restype = i64
+        L.fftjh_mlp_set_params.argtypes = [vp, vp]
+        L.fftjh_mlp_get_params.argtypes = [vp, vp]
+        L.fftjh_mlp_predict.argtypes = [vp, vp, vp, ci, vp]
+        L.fftjh_mlp_gradients.argtypes = [vp, vp, vp, ci, vp, ctypes.POINTER(ctypes.c_float), vp]
+        L.fftjh_mlp_train_step.argtypes = [vp, vp, vp, ci, ctypes.c_float, ctypes.POINTER(ctypes.c_float), vp]
+        L.fftjh_mlp_last_error.restype = ctypes.c_char_p
         L.fftjh_plan_r2c.argtypes = [ctypes.POINTER(vp), ci, i64, i64, ctypes.c_uint]
         L.fftjh_plan_c2r.argtypes = [ctypes.POINTER(vp), ci, i64, i64, ci, ctypes.c_uint]
         L.fftjh_execute.argtypes = [vp, vp, vp, vp]
@@ -98,6 +110,7 @@ def lib():
                                       ctypes.POINTER(ci), P64, P64, ctypes.POINTER(ctypes.c_double), P64, P64]
         L.fftjh_plan_describe.argtypes = [vp, ctypes.c_char_p, i64]
         L.fftjh_describe_request.argtypes = [ci, ci, ci, P64, i64, ci, ctypes.c_char_p, i64]
+        L.fftjh_describe_tuned.argtypes = [ci, ci, P64, i64, ci, ctypes.POINTER(Tuning), ctypes.c_char_p, i64]
         L.fftjh_plan_destroy.argtypes = [vp]
         L.fftjh_last_error.restype = ctypes.c_char_p
         L.fftjh_profile_launches.argtypes = [vp, vp, vp, vp, ci, ctypes.POINTER(ctypes.c_double),
@@ -119,4 +132,12 @@ def dims_array(dims):
 def describe_request(kind, dtype, dims, batch=1, sign=-1):
     buf = ctypes.create_string_buffer(1 << 16)
     check(lib().fftjh_describe_request(kind, dtype, len(dims), dims_array(dims), batch, sign, buf, len(buf)))
+    return buf.value.decode()
+
+
+def describe_tuned(dtype, dims, batch=1, sign=-1, tuning=None):
+    """Host-only schedule of a complex plan with search-space choices."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    t = tuning if tuning is not None else Tuning()
+    check(lib().fftjh_describe_tuned(dtype, len(dims), dims_array(dims), batch, sign, ctypes.byref(t), buf, len(buf)))
     return buf.value.decode()

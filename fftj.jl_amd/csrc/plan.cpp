@@ -483,7 +483,8 @@ static bool is_rows(const Geometry& g, int64_t n, int64_t Bi) { return Bi == 1 &
 // the scratch roles in `free` (front first).
 static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry gin, Geometry gout, int dir,
                       double scale, int src, int dst, std::vector<int> free) {
-  if (single_pass_ok(L.dtype, n)) {
+  const bool split_asked = L.tune.splits[0] > 0 && L.npass == 0;
+  if (single_pass_ok(L.dtype, n) && !split_asked) {
     add_pass(L, dir, static_cast<int>(n), batch, Bi, gin, gout, scale, src, dst);
     return;
   }
@@ -1395,13 +1396,18 @@ int fftjh_describe_request(int kind, int dtype, int rank, const int64_t* dims, i
                            int64_t len) {
   return guarded([&] {
     Layout L;
-    if (kind == FFTJH_KIND_C2C) L = layout_c2c(dtype, rank, dims, batch, sign, sign < 0 ? 1.0 : 1.0);
+    if (kind == FFTJH_KIND_C2C) L = layout_c2c(dtype, rank, dims, batch, sign, 1.0);
     else if (kind == FFTJH_KIND_R2C || kind == FFTJH_KIND_C2R) {
       if (rank != 1 || !dims) fail(FFTJH_ERR_ARGUMENT, "real plans are 1-D");
       L = kind == FFTJH_KIND_R2C ? layout_r2c(dtype, dims[0], batch) : layout_c2r(dtype, dims[0], batch, 1);
     } else fail(FFTJH_ERR_ARGUMENT, "unknown kind");
     copy_out(describe(L), buf, len);
   });
+}
+
+int fftjh_describe_tuned(int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
+                         const fftjh_tuning* tune, char* buf, int64_t len) {
+  return guarded([&] { copy_out(describe(layout_c2c(dtype, rank, dims, batch, sign, 1.0, tune)), buf, len); });
 }
 
 int fftjh_plan_destroy(fftjh_plan plan) {

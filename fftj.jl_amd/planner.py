@@ -1,0 +1,531 @@
+"""Planners for 1-D complex plans: the heuristic default, an exhaustive
+planner that measures every candidate, and a learned planner whose cost
+model (a 3-layer MLP, hidden 64, FP32) is trained on timings the GPU plans
+themselves produce, with libfftjh's own HIP forward/backward/Adam kernels.
+
+The reference has no measuring planner: it routes by fixed host thresholds
+(plan.jl:183-213) and ignores FFTW planner flags (api.jl:107-108).
+
+Candidates for a length n: the planner's default; for a one-pass length,
+other workgroup group sizes and generic radix chains (radix-8 or radix-4
+towers instead of the fused power-of-two kernel); for longer lengths,
+multi-pass factorizations n = F0*F1*...*F(P-1) (F0 the contiguous row pass,
+run last), each also with a narrower first pass.  Features describe the
+launch schedule a candidate compiles to (parsed from the plan's own
+description), so the model sees exactly what runs.
+
+Run as a script on the GPU to collect timings and train:
+    python fftj.jl_amd/planner.py --out DIR [--sizes 360]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+if __package__ in (None, ""):
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import __graft_entry__
+    _pkg = __graft_entry__.load_package()
+    _lib = sys.modules[_pkg.__name__ + "._lib"]
+    FFTPlan = sys.modules[_pkg.__name__ + ".api"].FFTPlan
+    rng = sys.modules[_pkg.__name__ + ".rng"]
+else:
+    from . import _lib, rng
+    from .api import FFTPlan
+
+check, lib = _lib.check, _lib.lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DATA_DIR = os.path.join(HERE, "planner_data")
+DATASET = "c2c_f32_mi355x"
+MAX_PASSES = 4
+PASS_FEATURES = 11
+N_FEATURES = 6 + MAX_PASSES * PASS_FEATURES
+HIDDEN = 64
+_ONE_PASS_MAX = 16384
+
+
+# --------------------------------------------------------------- candidates
+
+def smooth13(n):
+    for p in (2, 3, 5, 7, 11, 13):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+def _divisors(n, lo=16, hi=_ONE_PASS_MAX):
+    ds = set()
+    i = 1
+    while i * i <= n:
+        if n % i == 0:
+            ds.add(i)
+            ds.add(n // i)
+        i += 1
+    return sorted(d for d in ds if lo <= d <= hi and smooth13(d))
+
+
+def _factor_lists(n, parts):
+    """Ordered factorizations of n into `parts` one-pass factors (>= 16)."""
+    divs = _divisors(n)
+    out = []
+
+    def rec(rem, k, cur):
+        if k == 1:
+            if 16 <= rem <= _ONE_PASS_MAX and smooth13(rem):
+                out.append(cur + [rem])
+            return
+        for f in divs:
+            if f > rem:
+                break
+            if rem % f == 0:
+                rec(rem // f, k - 1, cur + [f])
+
+    rec(n, parts, [])
+    return out
+
+
+def radix_chain(n, top):
+    """Radices of n with power-of-two radices at most `top` (largest first),
+    then odd primes; None when a factor above 13 remains."""
+    rad = []
+    for p in (13, 11, 7, 5, 3):
+        while n % p == 0:
+            rad.append(p)
+            n //= p
+    pw = []
+    r = top
+    while n > 1 and r >= 2:
+        while n % r == 0:
+            pw.append(r)
+            n //= r
+        r //= 2
+    return pw + rad if n == 1 else None
+
+
+def candidates(n, max_per_p=4):
+    """Tuning dicts (splits, passes) for length n; {} is the default plan."""
+    cands = [{}]
+    if n <= _ONE_PASS_MAX:
+        for C in (1, 2, 4, 8):
+            cands.append({"passes": [{"C": C}]})
+        for top in (16, 8, 4):
+            rc = radix_chain(n, top)
+            if rc and len(rc) <= 16:
+                cands.append({"passes": [{"radices": rc}]})
+    for parts in (2, 3, 4):
+        if n < 16 ** parts:
+            continue
+        fl = _factor_lists(n, parts)
+        fl.sort(key=lambda f: (max(f) / min(f), f))
+        for f in fl[:max_per_p]:
+            cands.append({"splits": f})
+            cands.append({"splits": f, "passes": [{"C": 4}]})
+    seen, out = set(), []
+    for c in cands:
+        k = repr(c)
+        if k not in seen:
+            seen.add(k)
+            out.append(c)
+    return out
+
+
+def _tuning(cand):
+    return _lib.make_tuning(cand.get("splits", ()), cand.get("passes", ())) if cand else None
+
+
+def make_plan(n, batch, cand, sign=-1, dtype=0):
+    return FFTPlan(dtype, (n,), sign, 1.0, batch, tuning=_tuning(cand))
+
+
+def schedule(n, batch, cand, sign=-1, dtype=0):
+    """The launch schedule a candidate compiles to (host only, no GPU)."""
+    return _lib.describe_tuned(dtype, (n,), batch, sign, _tuning(cand))
+
+
+def valid_candidates(n, batch, sign=-1, dtype=0):
+    """(candidate, schedule) for each candidate the planner accepts."""
+    out = []
+    for cand in candidates(n):
+        try:
+            out.append((cand, schedule(n, batch, cand, sign, dtype)))
+        except (ValueError, NotImplementedError):
+            continue
+    return out
+
+
+# ----------------------------------------------------------------- features
+
+def _lg(v):
+    return math.log2(max(float(v), 1.0)) / 24.0
+
+
+def features(desc, n, batch, dtype=0):
+    """Fixed-length feature vector of a plan from its launch schedule."""
+    lines = [ln.strip() for ln in desc.splitlines()[1:] if ln.strip()]
+    f = np.zeros(N_FEATURES, dtype=np.float32)
+    f[0] = _lg(n)
+    f[1] = _lg(batch)
+    f[2] = _lg(n * batch)
+    f[3] = float(dtype)
+    f[4] = len(lines) / 4.0
+    f[5] = sum(1 for ln in lines if not ln.startswith("pass")) / 4.0
+    passes = [ln for ln in lines if ln.startswith("pass")][:MAX_PASSES]
+    for k, ln in enumerate(passes):
+        kv = dict(tok.split("=", 1) for tok in ln.split() if "=" in tok)
+        base = 6 + k * PASS_FEATURES
+        rad = [int(r) for r in kv.get("radices", "1").split(",") if r]
+        f[base + 0] = 1.0
+        f[base + 1] = 1.0 if kv.get("kernel") == "pow2" else 0.0
+        f[base + 2] = _lg(int(kv.get("n", 1)))
+        f[base + 3] = _lg(int(kv.get("C", 1))) * 4.0
+        f[base + 4] = _lg(int(kv.get("threads", 64))) * 2.0
+        f[base + 5] = _lg(int(kv.get("groups", 1)))
+        f[base + 6] = _lg(int(kv.get("in_stride", 1)))
+        f[base + 7] = _lg(int(kv.get("out_stride", 1)))
+        f[base + 8] = len(rad) / 8.0
+        f[base + 9] = sum(1 for r in rad if r < 8 or r % 2) / 8.0
+        f[base + 10] = 1.0 if "twiddle" in kv else 0.0
+    return f
+
+
+# ------------------------------------------------------------- measurement
+
+def time_plan(p, y, x, iters=5, warmup=1):
+    """Mean milliseconds of p(y, x) over `iters` calls (HIP events)."""
+    for _ in range(warmup):
+        p(y, x)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(iters):
+        p(y, x)
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / iters
+
+
+def batch_for(n, target=1 << 22):
+    return max(1, target // n)
+
+
+def training_sizes(count, lo=1 << 10, hi=1 << 24, seed=7):
+    """Distinct 13-smooth lengths drawn log-uniformly from [lo, hi] (splitmix64)."""
+    u = (rng.uniform(seed, 64 * count) + 1.0) / 2.0
+    out, seen = [], set()
+    for v in u:
+        m = int(round(math.exp(math.log(lo) + float(v) * (math.log(hi) - math.log(lo)))))
+        while not smooth13(m):
+            m += 1
+        if m <= hi and m not in seen:
+            seen.add(m)
+            out.append(m)
+        if len(out) == count:
+            break
+    return out
+
+
+def collect(sizes, log=None):
+    """Measure every candidate plan of every size (the exhaustive planner's
+    work); returns a dict of arrays, one row per (size, candidate)."""
+    feats, logt, size_id, cand_id, is_def, ns, bs = [], [], [], [], [], [], []
+    for si, n in enumerate(sizes):
+        batch = batch_for(n)
+        x = torch.randn(n * batch, dtype=torch.complex64, device="cuda")
+        y = torch.empty_like(x)
+        got = 0
+        for ci, cand in enumerate(candidates(n)):
+            try:
+                p = make_plan(n, batch, cand)
+            except (ValueError, NotImplementedError):
+                continue
+            ms = time_plan(p, y, x)
+            feats.append(features(p.describe(), n, batch))
+            logt.append(math.log(ms * 1e3))  # log microseconds
+            size_id.append(si)
+            cand_id.append(ci)
+            is_def.append(1 if not cand else 0)
+            ns.append(n)
+            bs.append(batch)
+            got += 1
+            del p
+        del x, y
+        if log and (si % 20 == 0 or si == len(sizes) - 1):
+            log(f"size {si + 1}/{len(sizes)} n={n} batch={batch}: {got} candidates, {len(logt)} timings so far")
+    return {"features": np.array(feats, dtype=np.float32), "log_us": np.array(logt, dtype=np.float32),
+            "size_id": np.array(size_id, dtype=np.int32), "cand_id": np.array(cand_id, dtype=np.int32),
+            "is_default": np.array(is_def, dtype=np.int8), "n": np.array(ns, dtype=np.int64),
+            "batch": np.array(bs, dtype=np.int64)}
+
+
+# --------------------------------------------------------------- the model
+
+def init_params(in_features=N_FEATURES, hidden=HIDDEN, seed=0):
+    """He-uniform weights, zero biases (seeded numpy generator)."""
+    r = np.random.default_rng(seed)
+
+    def w(o, i):
+        lim = math.sqrt(6.0 / i)
+        return r.uniform(-lim, lim, size=(o, i)).astype(np.float32)
+
+    parts = [w(hidden, in_features), np.zeros(hidden, np.float32), w(hidden, hidden), np.zeros(hidden, np.float32),
+             w(1, hidden), np.zeros(1, np.float32)]
+    return np.concatenate([p.ravel() for p in parts])
+
+
+class CostModel:
+    """The MLP on the GPU (libfftjh's HIP kernels): in -> 64 -> 64 -> 1,
+    predicting log microseconds.  Input features are standardized with the
+    training set's mean/std, kept with the weights."""
+
+    def __init__(self, in_features=N_FEATURES, hidden=HIDDEN, seed=0):
+        self.in_features, self.hidden = in_features, hidden
+        h = ctypes.c_void_p()
+        check(lib().fftjh_mlp_create(ctypes.byref(h), in_features, hidden))
+        self._h = h
+        self.nparams = int(lib().fftjh_mlp_param_count(h))
+        self.mu = np.zeros(in_features, np.float32)
+        self.sd = np.ones(in_features, np.float32)
+        self.set_params(init_params(in_features, hidden, seed))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._LIB is not None:
+            _lib._LIB.fftjh_mlp_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def _ok(rc):
+        if rc:
+            raise RuntimeError("fftjh_mlp: " + lib().fftjh_mlp_last_error().decode())
+
+    @staticmethod
+    def _stream():
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparams:
+            raise ValueError(f"expected {self.nparams} parameters, got {p.size}")
+        self._ok(lib().fftjh_mlp_set_params(self._h, p.ctypes.data))
+
+    def get_params(self):
+        p = np.empty(self.nparams, dtype=np.float32)
+        self._ok(lib().fftjh_mlp_get_params(self._h, p.ctypes.data))
+        return p
+
+    def normalize(self, X):
+        return ((np.asarray(X, np.float32) - self.mu) / self.sd).astype(np.float32)
+
+    def _dev(self, X, raw):
+        if isinstance(X, torch.Tensor) and X.is_cuda:
+            return X.contiguous()
+        X = np.asarray(X, np.float32)
+        if not raw:
+            X = self.normalize(X)
+        return torch.from_numpy(np.ascontiguousarray(X)).to("cuda")
+
+    def predict(self, X, raw=False):
+        """Predicted log microseconds for feature rows X (HIP forward kernel)."""
+        Xd = self._dev(X, raw)
+        out = torch.empty(Xd.shape[0], dtype=torch.float32, device="cuda")
+        self._ok(lib().fftjh_mlp_predict(self._h, ctypes.c_void_p(Xd.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                         Xd.shape[0], self._stream()))
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    def gradients(self, X, t, raw=True):
+        """Flat loss gradient and loss (HIP forward + backward kernels)."""
+        Xd = self._dev(X, raw)
+        td = torch.as_tensor(np.asarray(t, np.float32)).to("cuda")
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._ok(lib().fftjh_mlp_gradients(self._h, ctypes.c_void_p(Xd.data_ptr()), ctypes.c_void_p(td.data_ptr()),
+                                           Xd.shape[0], g.ctypes.data, ctypes.byref(loss), self._stream()))
+        return g, loss.value
+
+    def train_step(self, Xd, td, lr, want_loss=True):
+        """One Adam step on device rows Xd (standardized) and targets td."""
+        loss = ctypes.c_float()
+        self._ok(lib().fftjh_mlp_train_step(self._h, ctypes.c_void_p(Xd.data_ptr()), ctypes.c_void_p(td.data_ptr()),
+                                            Xd.shape[0], ctypes.c_float(lr),
+                                            ctypes.byref(loss) if want_loss else None, self._stream()))
+        return loss.value if want_loss else None
+
+    def fit(self, X, t, steps=6000, batch=256, lr=3e-3, seed=0, log=None):
+        """Adam on seeded mini-batches (HIP kernels); returns the loss curve
+        sampled every 100 steps.  Sets the feature standardization first."""
+        X = np.asarray(X, np.float32)
+        self.mu = X.mean(0).astype(np.float32)
+        self.sd = (X.std(0) + 1e-3).astype(np.float32)
+        Xd = torch.from_numpy(self.normalize(X)).to("cuda")
+        td = torch.as_tensor(np.asarray(t, np.float32)).to("cuda")
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        curve = []
+        bs = min(batch, Xd.shape[0])
+        for s in range(steps):
+            idx = torch.randint(0, Xd.shape[0], (bs,), generator=g).to("cuda")
+            lr_s = lr * (0.05 ** (s / steps))
+            want = s % 100 == 0 or s == steps - 1
+            loss = self.train_step(Xd[idx].contiguous(), td[idx].contiguous(), lr_s, want_loss=want)
+            if want:
+                curve.append(loss)
+                if log and s % 1000 == 0:
+                    log(f"step {s} loss {loss:.4f}")
+        torch.cuda.synchronize()
+        return curve
+
+    def save(self, path):
+        np.savez(path, params=self.get_params(), mu=self.mu, sd=self.sd,
+                 shape=np.array([self.in_features, self.hidden], np.int64))
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as z:
+            m = cls(int(z["shape"][0]), int(z["shape"][1]))
+            m.set_params(z["params"])
+            m.mu, m.sd = z["mu"].astype(np.float32), z["sd"].astype(np.float32)
+        return m
+
+
+# ----------------------------------------------------------------- planners
+
+def held_out_mask(data, every=5):
+    """Every `every`-th size (by index) is held out of training."""
+    sids = np.unique(data["size_id"])
+    return np.isin(data["size_id"], sids[::every])
+
+
+def regret_report(data, pred, mask):
+    """Over the sizes in `mask`: measured time of the learned pick and of the
+    default plan, relative to the exhaustive planner's best."""
+    rows = []
+    for sid in np.unique(data["size_id"][mask]):
+        m = data["size_id"] == sid
+        t = np.exp(data["log_us"][m].astype(np.float64))
+        best = t.min()
+        learned = t[int(np.argmin(pred[m]))]
+        d = data["is_default"][m] == 1
+        default = t[d][0] if d.any() else t.max()
+        rows.append((int(data["n"][m][0]), best, learned, default, int(m.sum())))
+    r = np.array(rows, dtype=np.float64)
+    reg_l = r[:, 2] / r[:, 1] - 1.0
+    reg_d = r[:, 3] / r[:, 1] - 1.0
+    return {
+        "sizes": int(len(r)),
+        "candidates_per_size_mean": float(r[:, 4].mean()),
+        "mean_regret_learned": float(reg_l.mean()),
+        "mean_regret_default": float(reg_d.mean()),
+        "median_regret_learned": float(np.median(reg_l)),
+        "median_regret_default": float(np.median(reg_d)),
+        "p90_regret_learned": float(np.quantile(reg_l, 0.9)),
+        "p90_regret_default": float(np.quantile(reg_d, 0.9)),
+        "geomean_speedup_learned_over_default": float(np.exp(np.mean(np.log(r[:, 3] / r[:, 2])))),
+        "learned_picks_exhaustive_best": float(np.mean(r[:, 2] <= r[:, 1])),
+        "default_is_exhaustive_best": float(np.mean(r[:, 3] <= r[:, 1])),
+    }
+
+
+def load_dataset(name=DATASET, directory=DATA_DIR):
+    with np.load(os.path.join(directory, name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def load_model(name=DATASET, directory=DATA_DIR):
+    return CostModel.load(os.path.join(directory, name + "_model.npz"))
+
+
+class LearnedPlanner:
+    """Picks, among a length's candidates, the one the cost model predicts
+    fastest; lengths it has no candidates for fall back to the default."""
+
+    def __init__(self, model=None):
+        self.model = model if model is not None else load_model()
+
+    def rank(self, n, batch, sign=-1):
+        """Candidates of length n ordered by predicted time: [(cand, us)]."""
+        vc = valid_candidates(n, batch, sign)
+        pred = self.model.predict(np.array([features(d, n, batch) for _, d in vc]))
+        order = np.argsort(pred, kind="stable")
+        return [(vc[k][0], float(np.exp(pred[k]))) for k in order]
+
+    def choose(self, n, batch, sign=-1):
+        """Builds only the predicted-fastest candidate: (plan, cand, us)."""
+        cand, us = self.rank(n, batch, sign)[0]
+        return make_plan(n, batch, cand, sign=sign), cand, us
+
+    def per_transform_us(self, sizes):
+        """Predicted microseconds per transform for each length (default
+        plans at the training batch) -- per-item costs for the ragged
+        planner's workgroup balance."""
+        sizes = [int(s) for s in sizes]
+        feats = []
+        for n in sizes:
+            b = batch_for(n)
+            feats.append(features(schedule(n, b, {}), n, b))
+        pred = self.model.predict(np.array(feats))
+        return np.exp(pred.astype(np.float64)) / np.array([batch_for(n) for n in sizes], np.float64)
+
+
+def exhaustive_choose(n, batch, sign=-1, x=None, y=None):
+    """Measure every candidate; return (plan, candidate, microseconds)."""
+    if x is None:
+        x = torch.randn(n * batch, dtype=torch.complex64, device="cuda")
+        y = torch.empty_like(x)
+    best = (None, None, float("inf"))
+    for cand in candidates(n):
+        try:
+            p = make_plan(n, batch, cand, sign=sign)
+        except (ValueError, NotImplementedError):
+            continue
+        us = time_plan(p, y, x) * 1e3
+        if us < best[2]:
+            best = (p, cand, us)
+    return best
+
+
+# ------------------------------------------------------------------ script
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="collect plan timings on the GPU and train the cost model")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--sizes", type=int, default=360)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--data", default=None, help="reuse a collected dataset (.npz) instead of measuring")
+    a = ap.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+
+    def log(s):
+        print(s, flush=True)
+
+    t0 = time.time()
+    if a.data:
+        with np.load(a.data, allow_pickle=False) as z:
+            data = {k: z[k] for k in z.files}
+    else:
+        sizes = training_sizes(a.sizes)
+        data = collect(sizes, log=log)
+        np.savez_compressed(os.path.join(a.out, DATASET + ".npz"), **data)
+    log(f"{len(data['log_us'])} timings over {len(np.unique(data['size_id']))} sizes ({time.time() - t0:.0f} s)")
+    test = held_out_mask(data)
+    model = CostModel()
+    curve = model.fit(data["features"][~test], data["log_us"][~test], steps=a.steps, log=log)
+    model.save(os.path.join(a.out, DATASET + "_model.npz"))
+    pred = model.predict(data["features"])
+    rmse = lambda m: float(np.sqrt(np.mean((pred[m] - data["log_us"][m]) ** 2)))  # noqa: E731
+    report = {"timings": int(len(pred)), "train_rmse_log": rmse(~test), "test_rmse_log": rmse(test),
+              "loss_first": curve[0], "loss_last": curve[-1],
+              "held_out": regret_report(data, pred, test), "train": regret_report(data, pred, ~test)}
+    with open(os.path.join(a.out, DATASET + "_report.json"), "w") as fh:
+        json.dump(report, fh, indent=1)
+    log(json.dumps(report["held_out"]))
+    log(f"done in {time.time() - t0:.0f} s")
+
+
+if __name__ == "__main__":
+    main()

@@ -141,6 +141,12 @@ int fftjh_plan_describe(fftjh_plan plan, char* buf, int64_t len);
 int fftjh_describe_request(int kind, int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
                            char* buf, int64_t len);
 
+/* The same for a complex request with search-space choices (host only); the
+ * planners score candidate schedules from it without building them.  No
+ * reference counterpart (the reference has no measuring planner). */
+int fftjh_describe_tuned(int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
+                         const fftjh_tuning* tune, char* buf, int64_t len);
+
 /* Release the plan's scratch and its hold on the shared tables. */
 int fftjh_plan_destroy(fftjh_plan plan);
 
@@ -156,6 +162,30 @@ int fftjh_version(void);
  * entries; *nlaunch is set to the plan's launch count).  Host-synchronous. */
 int fftjh_profile_launches(fftjh_plan plan, void* y, const void* x, fftjh_stream stream, int iters,
                            double* ms, int* nlaunch);
+
+/* ---- learned planner: the cost model's network ---------------------------
+ * A 3-layer MLP (in -> hidden -> hidden -> 1, ReLU, FP32) that predicts the
+ * log run time of a plan from its features, trained on timings the GPU plans
+ * produce.  Forward, backward and Adam run in this library's HIP kernels.
+ * No reference counterpart: FFTJ routes by fixed host thresholds
+ * (plan.jl:183-213) and has no measuring planner.  X is a device array of
+ * batch x in_features floats (row-major), targets/outputs device arrays of
+ * batch floats.  Parameter layout: W1[h][in], b1[h], W2[h][h], b2[h],
+ * W3[1][h], b3[1]. */
+typedef struct fftjh_mlp_s* fftjh_mlp;
+int fftjh_mlp_create(fftjh_mlp* model, int in_features, int hidden);
+int fftjh_mlp_destroy(fftjh_mlp model);
+int64_t fftjh_mlp_param_count(fftjh_mlp model);
+int fftjh_mlp_set_params(fftjh_mlp model, const float* host_params);
+int fftjh_mlp_get_params(fftjh_mlp model, float* host_params);
+int fftjh_mlp_predict(fftjh_mlp model, const float* X, float* out, int batch, fftjh_stream stream);
+/* gradients of L = sum_r (out_r - t_r)^2 / (2 batch) into host_grads (may be NULL) and the loss */
+int fftjh_mlp_gradients(fftjh_mlp model, const float* X, const float* target, int batch, float* host_grads,
+                        float* loss, fftjh_stream stream);
+/* one Adam step (beta 0.9 / 0.999) on that loss; writes the loss before the step */
+int fftjh_mlp_train_step(fftjh_mlp model, const float* X, const float* target, int batch, float lr, float* loss,
+                         fftjh_stream stream);
+const char* fftjh_mlp_last_error(void);
 
 #ifdef __cplusplus
 }

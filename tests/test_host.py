@@ -15,7 +15,7 @@ _lib = F._lib
 
 def _declared():
     src = open(os.path.join(ROOT, "include", "fftjh.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fftjh_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(fftjh_\w+)\s*\(", src, re.M)))
 
 
 def test_exports_every_declared_symbol():
