@@ -329,7 +329,7 @@ def _positive_n(n):
     return n
 
 
-def _plan_c2c(T_or_x, n, sign, scale_kind, region, batch):
+def _plan_c2c(T_or_x, n, sign, scale_kind, region, batch, planner=None):
     if isinstance(T_or_x, torch.Tensor):
         x = T_or_x
         if not x.is_complex():
@@ -347,22 +347,40 @@ def _plan_c2c(T_or_x, n, sign, scale_kind, region, batch):
     code = _code(T_or_x)
     n = _positive_n(n)
     scale = 1.0 / n if scale_kind == "inv" else 1.0
-    return FFTPlan(code, (n,), sign, scale, batch)
+    return FFTPlan(code, (n,), sign, scale, batch, tuning=_planner_tuning(planner, code, n, batch, sign))
 
 
-def plan_fft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, **flags):
-    """plan_fft(T, n) (plan.jl:207-208) or plan_fft(x[, region]) (api.jl:135-140)."""
-    return _plan_c2c(T_or_x, n, -1, "one", region, batch)
+def _planner_tuning(planner, code, n, batch, sign):
+    """Search-space choice for a 1-D complex plan: None/"heuristic" is the
+    library's own planner; "learned" ranks the candidate schedules with the
+    trained cost model (FP32 plans; FP64 keeps the heuristic); "exhaustive"
+    times every candidate on the current stream and keeps the fastest."""
+    if planner in (None, "heuristic"):
+        return None
+    from . import planner as _pl
+    if planner == "learned":
+        if code != _lib.F32:
+            return None
+        return _pl._tuning(_pl.default_learned().rank(n, batch, sign)[0][0])
+    if planner == "exhaustive":
+        return _pl._tuning(_pl.exhaustive_choose(n, batch, sign, dtype=code)[1])
+    raise ArgumentError(f"planner must be None, 'heuristic', 'learned' or 'exhaustive' (got {planner!r})")
 
 
-def plan_ifft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, **flags):
+def plan_fft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, **flags):
+    """plan_fft(T, n) (plan.jl:207-208) or plan_fft(x[, region]) (api.jl:135-140).
+    planner= picks the schedule search for 1-D plans of a given length."""
+    return _plan_c2c(T_or_x, n, -1, "one", region, batch, planner)
+
+
+def plan_ifft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, **flags):
     """plan_ifft (plan.jl:209-210, api.jl:147-152): inverse scaled by 1/n."""
-    return _plan_c2c(T_or_x, n, +1, "inv", region, batch)
+    return _plan_c2c(T_or_x, n, +1, "inv", region, batch, planner)
 
 
-def plan_bfft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, **flags):
+def plan_bfft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, **flags):
     """plan_bfft (plan.jl:212-213, api.jl:141-146): unnormalized inverse."""
-    return _plan_c2c(T_or_x, n, +1, "one", region, batch)
+    return _plan_c2c(T_or_x, n, +1, "one", region, batch, planner)
 
 
 def plan_fft_(T_or_x, n=None, region=None, **kw):

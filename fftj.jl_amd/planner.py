@@ -15,7 +15,7 @@ launch schedule a candidate compiles to (parsed from the plan's own
 description), so the model sees exactly what runs.
 
 Run as a script on the GPU to collect timings and train:
-    python fftj.jl_amd/planner.py --out DIR [--sizes 360]
+    python fftj.jl_amd/planner.py --out DIR [--sizes 400]
 """
 import argparse
 import ctypes
@@ -150,13 +150,18 @@ def schedule(n, batch, cand, sign=-1, dtype=0):
 
 
 def valid_candidates(n, batch, sign=-1, dtype=0):
-    """(candidate, schedule) for each candidate the planner accepts."""
-    out = []
+    """(candidate, schedule) for each candidate the planner accepts; a
+    candidate whose schedule repeats an earlier one is dropped (the default
+    comes first, so it is always kept)."""
+    out, seen = [], set()
     for cand in candidates(n):
         try:
-            out.append((cand, schedule(n, batch, cand, sign, dtype)))
+            d = schedule(n, batch, cand, sign, dtype)
         except (ValueError, NotImplementedError):
             continue
+        if d not in seen:
+            seen.add(d)
+            out.append((cand, d))
     return out
 
 
@@ -197,18 +202,20 @@ def features(desc, n, batch, dtype=0):
 
 # ------------------------------------------------------------- measurement
 
-def time_plan(p, y, x, iters=5, warmup=1):
-    """Mean milliseconds of p(y, x) over `iters` calls (HIP events)."""
+def time_plan(p, y, x, iters=10, warmup=2, repeats=3):
+    """Milliseconds of p(y, x): the median over `repeats` of the mean of
+    `iters` back-to-back calls (HIP events on the current stream)."""
     for _ in range(warmup):
         p(y, x)
     torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(iters):
-        p(y, x)
-    b.record()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(repeats)]
+    for a, b in ev:
+        a.record()
+        for _ in range(iters):
+            p(y, x)
+        b.record()
     torch.cuda.synchronize()
-    return a.elapsed_time(b) / iters
+    return float(np.median([a.elapsed_time(b) / iters for a, b in ev]))
 
 
 def batch_for(n, target=1 << 22):
@@ -240,11 +247,8 @@ def collect(sizes, log=None):
         x = torch.randn(n * batch, dtype=torch.complex64, device="cuda")
         y = torch.empty_like(x)
         got = 0
-        for ci, cand in enumerate(candidates(n)):
-            try:
-                p = make_plan(n, batch, cand)
-            except (ValueError, NotImplementedError):
-                continue
+        for ci, (cand, _) in enumerate(valid_candidates(n, batch)):
+            p = make_plan(n, batch, cand)
             ms = time_plan(p, y, x)
             feats.append(features(p.describe(), n, batch))
             logt.append(math.log(ms * 1e3))  # log microseconds
@@ -472,17 +476,25 @@ class LearnedPlanner:
         return np.exp(pred.astype(np.float64)) / np.array([batch_for(n) for n in sizes], np.float64)
 
 
-def exhaustive_choose(n, batch, sign=-1, x=None, y=None):
+_LEARNED = None
+
+
+def default_learned():
+    """The planner over the cost model trained on this GPU's timings."""
+    global _LEARNED
+    if _LEARNED is None:
+        _LEARNED = LearnedPlanner()
+    return _LEARNED
+
+
+def exhaustive_choose(n, batch, sign=-1, x=None, y=None, dtype=0):
     """Measure every candidate; return (plan, candidate, microseconds)."""
     if x is None:
-        x = torch.randn(n * batch, dtype=torch.complex64, device="cuda")
+        x = torch.randn(n * batch, dtype=torch.complex64 if dtype == 0 else torch.complex128, device="cuda")
         y = torch.empty_like(x)
     best = (None, None, float("inf"))
-    for cand in candidates(n):
-        try:
-            p = make_plan(n, batch, cand, sign=sign)
-        except (ValueError, NotImplementedError):
-            continue
+    for cand, _ in valid_candidates(n, batch, sign, dtype):
+        p = make_plan(n, batch, cand, sign=sign, dtype=dtype)
         us = time_plan(p, y, x) * 1e3
         if us < best[2]:
             best = (p, cand, us)
@@ -494,7 +506,7 @@ def exhaustive_choose(n, batch, sign=-1, x=None, y=None):
 def main(argv=None):
     ap = argparse.ArgumentParser(description="collect plan timings on the GPU and train the cost model")
     ap.add_argument("--out", required=True)
-    ap.add_argument("--sizes", type=int, default=360)
+    ap.add_argument("--sizes", type=int, default=400)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--data", default=None, help="reuse a collected dataset (.npz) instead of measuring")
     a = ap.parse_args(argv)

@@ -190,3 +190,9 @@ def test_committed_dataset_shape():
     assert int(data["is_default"].sum()) == len(sids)
     with np.load(_artifacts()[1], allow_pickle=False) as z:
         assert z["params"].size == M.param_count(P.N_FEATURES, P.HIDDEN)
+
+
+def test_planner_option_validated():
+    import torch
+    with pytest.raises(F.ArgumentError):
+        F.plan_fft(torch.float32, 4096, planner="fastest")
