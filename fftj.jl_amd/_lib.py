@@ -37,12 +37,19 @@ class PassTuning(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("splits", ctypes.c_int64 * 4), ("pass_", PassTuning * 4)]
+    _fields_ = [("splits", ctypes.c_int64 * 4), ("pass_", PassTuning * 4), ("flags", ctypes.c_uint32),
+                ("pad_", ctypes.c_uint32)]
 
 
-def make_tuning(splits=(), passes=()):
-    """passes: up to 4 dicts with optional keys C, radices, xcd_remap (0 planner, 1 off, 2 on)."""
+FLAG_MFMA_F16, FLAG_MFMA_BF16, FLAG_MFMA_BF16X3 = 0x1, 0x2, 0x4
+PRECISION_FLAGS = {"f16": FLAG_MFMA_F16, "bf16": FLAG_MFMA_BF16, "bf16x3": FLAG_MFMA_BF16X3}
+
+
+def make_tuning(splits=(), passes=(), flags=0):
+    """passes: up to 4 dicts with optional keys C, radices, xcd_remap (0 planner, 1 off, 2 on);
+    flags: FLAG_MFMA_* (split-precision MFMA stages)."""
     t = Tuning()
+    t.flags = int(flags)
     for i, f in enumerate(splits or ()):
         t.splits[i] = int(f)
     for i, p in enumerate(passes):

@@ -103,9 +103,10 @@ class FFTPlan(_PlanBase):
     """Complex plan: 1-D (the reference's FFTPlan/FourStepPlan/ResamplePlan,
     chosen by the GPU planner) or N-d (NdPlan).  Fields n, sign, scale."""
 
-    def __init__(self, dtype, dims, sign, scale, batch=1, shape=None, tuning=None):
+    def __init__(self, dtype, dims, sign, scale, batch=1, shape=None, tuning=None, flags=0):
         self.dtype_code = dtype
         self.tuning = tuning
+        self.flags = int(flags)
         self.dims = tuple(int(d) for d in dims)  # column-major, dims[0] fastest
         self.sign = int(sign)
         self.scale = float(scale)
@@ -116,8 +117,11 @@ class FFTPlan(_PlanBase):
         h = ctypes.c_void_p()
         if tuning is None:
             check(lib().fftjh_plan_c2c(ctypes.byref(h), dtype, len(self.dims), dims_array(self.dims), self.batch,
-                                       self.sign, self.scale, 0))
+                                       self.sign, self.scale, self.flags))
         else:
+            if self.flags:
+                tuning = _lib.Tuning.from_buffer_copy(tuning)
+                tuning.flags |= self.flags
             check(lib().fftjh_plan_c2c_tuned(ctypes.byref(h), dtype, len(self.dims), dims_array(self.dims),
                                              self.batch, self.sign, self.scale, ctypes.byref(tuning)))
         self._h = _Handle(h.value)
@@ -153,11 +157,11 @@ class FFTPlan(_PlanBase):
         key = float(s)
         if key not in self._scaled:
             self._scaled[key] = FFTPlan(self.dtype_code, self.dims, self.sign, self.scale * key, self.batch,
-                                        self.shape, self.tuning)
+                                        self.shape, self.tuning, self.flags)
         return self._scaled[key]
 
     def _sibling(self, sign, scale):
-        return FFTPlan(self.dtype_code, self.dims, sign, scale, self.batch, self.shape, self.tuning)
+        return FFTPlan(self.dtype_code, self.dims, sign, scale, self.batch, self.shape, self.tuning, self.flags)
 
     def __repr__(self):
         return f"FFTPlan(dims={self.dims}, sign={self.sign}, scale={self.scale}, batch={self.batch})"
@@ -329,7 +333,18 @@ def _positive_n(n):
     return n
 
 
-def _plan_c2c(T_or_x, n, sign, scale_kind, region, batch, planner=None):
+def _precision_flags(precision):
+    """precision=None (plain FP32/FP64 stages) or "f16" / "bf16" / "bf16x3":
+    radix-9/16/32 stages of an FP32 plan as split-operand MFMA products."""
+    if precision is None:
+        return 0
+    if precision not in _lib.PRECISION_FLAGS:
+        raise ArgumentError(f"precision must be None, 'f16', 'bf16' or 'bf16x3' (got {precision!r})")
+    return _lib.PRECISION_FLAGS[precision]
+
+
+def _plan_c2c(T_or_x, n, sign, scale_kind, region, batch, planner=None, precision=None):
+    flags = _precision_flags(precision)
     if isinstance(T_or_x, torch.Tensor):
         x = T_or_x
         if not x.is_complex():
@@ -343,11 +358,12 @@ def _plan_c2c(T_or_x, n, sign, scale_kind, region, batch, planner=None):
             raise ArgumentError(f"dimensions must be positive (got {shape})")
         length = math.prod(dims)
         scale = 1.0 / length if scale_kind == "inv" else 1.0
-        return FFTPlan(code, dims, sign, scale, 1, shape if N > 1 else (length,))
+        return FFTPlan(code, dims, sign, scale, 1, shape if N > 1 else (length,), flags=flags)
     code = _code(T_or_x)
     n = _positive_n(n)
     scale = 1.0 / n if scale_kind == "inv" else 1.0
-    return FFTPlan(code, (n,), sign, scale, batch, tuning=_planner_tuning(planner, code, n, batch, sign))
+    tuning = None if flags else _planner_tuning(planner, code, n, batch, sign)
+    return FFTPlan(code, (n,), sign, scale, batch, tuning=tuning, flags=flags)
 
 
 def _planner_tuning(planner, code, n, batch, sign):
@@ -367,20 +383,21 @@ def _planner_tuning(planner, code, n, batch, sign):
     raise ArgumentError(f"planner must be None, 'heuristic', 'learned' or 'exhaustive' (got {planner!r})")
 
 
-def plan_fft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, **flags):
+def plan_fft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None, **flags):
     """plan_fft(T, n) (plan.jl:207-208) or plan_fft(x[, region]) (api.jl:135-140).
-    planner= picks the schedule search for 1-D plans of a given length."""
-    return _plan_c2c(T_or_x, n, -1, "one", region, batch, planner)
+    planner= picks the schedule search for 1-D plans of a given length;
+    precision= selects split-precision MFMA stages for FP32 plans."""
+    return _plan_c2c(T_or_x, n, -1, "one", region, batch, planner, precision)
 
 
-def plan_ifft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, **flags):
+def plan_ifft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None, **flags):
     """plan_ifft (plan.jl:209-210, api.jl:147-152): inverse scaled by 1/n."""
-    return _plan_c2c(T_or_x, n, +1, "inv", region, batch, planner)
+    return _plan_c2c(T_or_x, n, +1, "inv", region, batch, planner, precision)
 
 
-def plan_bfft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, **flags):
+def plan_bfft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None, **flags):
     """plan_bfft (plan.jl:212-213, api.jl:141-146): unnormalized inverse."""
-    return _plan_c2c(T_or_x, n, +1, "one", region, batch, planner)
+    return _plan_c2c(T_or_x, n, +1, "one", region, batch, planner, precision)
 
 
 def plan_fft_(T_or_x, n=None, region=None, **kw):

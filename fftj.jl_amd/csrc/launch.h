@@ -22,6 +22,15 @@ hipError_t launch_pass_pow2(int dtype, int dir, const PassDesc& d, const PassDes
 int pass_pow2_resident_groups(int dtype, int dir, const PassDesc& d);
 int pow2_max_threads(int dtype);
 
+// split-precision MFMA pass kernels (FP32): mode is a MatMode (stages_mat.h),
+// mat the operand table launch_mat_table wrote for that mode and direction
+hipError_t launch_pass_mat(int mode, int dir, const PassDesc& d, const PassDesc* dev_desc, const void* x, void* y,
+                           const void* tw, const void* mat, hipStream_t stream);
+int pass_mat_resident_groups(int mode, int dir, const PassDesc& d);
+// bytes of the operand table of a mode; its generator (FP64 on the device)
+int64_t mat_table_bytes(int mode);
+hipError_t launch_mat_table(int mode, int dir, void* out, hipStream_t stream);
+
 hipError_t launch_tw_stage(int dtype, void* out, int ns, int R, int dir, hipStream_t stream);
 hipError_t launch_tw_pow(int dtype, void* out, int64_t count, int64_t mult, int64_t L, int dir,
                          hipStream_t stream);

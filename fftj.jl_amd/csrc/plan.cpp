@@ -27,6 +27,7 @@
 #include "common.h"
 #include "launch.h"
 #include "ragged.h"
+#include "stages_mat.h"
 
 namespace fftjh {
 
@@ -118,6 +119,50 @@ static std::vector<int> default_radices(int64_t n) {
 
 static int esize(int dtype) { return dtype == FFTJH_F32 ? 8 : 16; }
 
+// MFMA chains: powers of two as radix-16 stages after one radix-2/4/8 stage,
+// pairs of threes as radix-9 stages (both on MFMA), the rest on VALU
+static std::vector<int> mat_radices(int64_t n) {
+  int a, cnt[5];
+  std::vector<int> r;
+  if (!factor13(n, a, cnt)) return r;
+  if (a % 4) r.push_back(1 << (a % 4));
+  for (int i = 0; i < a / 4; ++i) r.push_back(16);
+  for (int k = 0; k < cnt[0] / 2; ++k) r.push_back(9);
+  if (cnt[0] % 2) r.push_back(3);
+  for (int i = 1; i < 5; ++i)
+    for (int k = 0; k < cnt[i]; ++k) r.push_back(kOddRadices[i]);
+  return r;
+}
+
+// threads of an MFMA chain: a wave holds mat_tiles_per_wave(R) tiles of 16
+// butterflies in a matrix stage; VALU stages as in pass_threads
+static int pass_threads_mat(int n, int C, const std::vector<int>& radices) {
+  int64_t t = 64;
+  for (int R : radices) {
+    const int64_t nb = static_cast<int64_t>(C) * (n / R);
+    if (mat_radix(R)) {
+      const int64_t tiles = (nb + 15) / 16;
+      const int64_t waves = (tiles + mat_tiles_per_wave(R) - 1) / mat_tiles_per_wave(R);
+      t = std::max<int64_t>(t, 64 * waves);
+    } else {
+      const int bpt = kElemsPerThread / R;
+      t = std::max<int64_t>(t, (nb + bpt - 1) / bpt);
+    }
+  }
+  t = (t + 63) / 64 * 64;
+  return static_cast<int>(std::max<int64_t>(t, C));
+}
+
+constexpr int kMatThreads = kMaxThreads / 2;  // launch bound of the MFMA pass kernels
+
+static bool pass_fits_mat(int n, int C, const std::vector<int>& radices, int* threads) {
+  if (static_cast<int64_t>(lds_elems(C * n)) * 8 > kLdsBytes) return false;
+  const int t = pass_threads_mat(n, C, radices);
+  if (t > kMatThreads) return false;
+  if (threads) *threads = t;
+  return true;
+}
+
 // threads one workgroup needs for C transforms of length n with `radices`
 static int pass_threads(int n, int C, const std::vector<int>& radices) {
   int t = 64;
@@ -191,6 +236,7 @@ struct Step {
   // ST_PASS
   PassDesc d{};
   bool pow2 = false;  // compile-time-chain kernel
+  int mat = kMatNone;  // split-precision MFMA chain (radix 9/16/32 stages on MFMA)
   std::vector<int> radices;
   // other kernels
   int64_t n = 0, m = 0, h = 0, batch = 0, s1 = 0, s2 = 0, keep = 0;
@@ -200,13 +246,14 @@ struct Step {
   int64_t Bi = 1;
   Geometry ga{1, 0, 0}, gb{1, 0, 0};
   // device resources
-  DevBufPtr tw, desc, aux;
+  DevBufPtr tw, desc, aux, mat_tab;
   std::shared_ptr<struct RaggedData> rg;  // ST_RAGGED
 };
 
 struct Layout {
   fftjh_tuning tune{};  // search-space overrides (zero = planner's choice)
   int npass = 0;        // pass launches laid out so far (index into tune.pass)
+  int mat = kMatNone;   // split-precision MFMA mode of every pass (FP32 C2C)
   int dtype = 0, kind = FFTJH_KIND_C2C, sign = -1, rank = 1;
   int64_t dims[8] = {0};
   int64_t batch = 1;
@@ -251,13 +298,15 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   d.scale = scale;
   const fftjh_pass_tuning* pt = L.npass < 4 ? &L.tune.pass[L.npass] : nullptr;
   ++L.npass;
-  s.radices = (n == 1) ? std::vector<int>{} : default_radices(n);
+  s.mat = L.mat;
+  s.radices = (n == 1) ? std::vector<int>{} : L.mat ? mat_radices(n) : default_radices(n);
   if (pt && pt->nradix > 0) {
     if (pt->nradix > kMaxStages) fail(FFTJH_ERR_ARGUMENT, "too many radices in tuning");
     std::vector<int> r(pt->radices, pt->radices + pt->nradix);
     int64_t prod = 1;
     for (int R : r) {
-      if (R != 2 && R != 3 && R != 4 && R != 5 && R != 7 && R != 8 && R != 11 && R != 13 && R != 16)
+      const bool valu = R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 11 || R == 13 || R == 16;
+      if (!valu && !(L.mat && mat_radix(R)))
         fail(FFTJH_ERR_ARGUMENT, "tuning radix " + std::to_string(R) + " has no kernel");
       prod *= R;
     }
@@ -270,7 +319,7 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   {
     int pr[8], pc = 0;
     static const bool no_pow2 = std::getenv("FFTJH_NO_POW2") != nullptr;
-    if (!no_pow2 && !(pt && pt->nradix > 0) && pow2_chain(n, pr, &pc)) {
+    if (!no_pow2 && !L.mat && !(pt && pt->nradix > 0) && pow2_chain(n, pr, &pc)) {
       s.pow2 = true;
       s.radices.assign(pr, pr + pc);
     }
@@ -278,11 +327,15 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   // transforms per workgroup: strided passes take adjacent columns for
   // contiguous HBM runs; row passes group short transforms to fill a workgroup
   const bool colmode = in.stride > 1 || out.stride > 1;
+  auto fits = [&](int C, int* t) {
+    if (L.mat) return pass_fits_mat(n, C, s.radices, t);
+    return s.pow2 ? pow2_fits(L.dtype, n, C, t) : pass_fits(L.dtype, n, C, s.radices, t);
+  };
   int bestC = 0, bestT = 0;
   if (pt && pt->C > 0) {
     int t;
     if ((pt->C & (pt->C - 1)) || pt->C > 64) fail(FFTJH_ERR_ARGUMENT, "tuning C must be a power of two <= 64");
-    if (!(s.pow2 ? pow2_fits(L.dtype, n, pt->C, &t) : pass_fits(L.dtype, n, pt->C, s.radices, &t)))
+    if (!fits(pt->C, &t))
       fail(FFTJH_ERR_ARGUMENT, "tuning C=" + std::to_string(pt->C) + " does not fit one workgroup");
     bestC = pt->C;
     bestT = t;
@@ -298,7 +351,7 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
     }
     for (int C = colmode ? 16 : 64; C >= 1; C >>= 1) {
       int t;
-      if (!(s.pow2 ? pow2_fits(L.dtype, n, C, &t) : pass_fits(L.dtype, n, C, s.radices, &t))) continue;
+      if (!fits(C, &t)) continue;
       // more columns than transforms only to fill one wave (idle columns)
       if (C > 1 && C > batch && !(s.pow2 && (C / 2) * (n / 16) < 64)) continue;
       if (!colmode && t > 256 && C > 1) continue;
@@ -611,8 +664,18 @@ static void check_dtype(int dtype) {
     fail(FFTJH_ERR_ARGUMENT, "dtype must be FFTJH_F32 or FFTJH_F64");
 }
 
+// split-precision mode from plan flags (at most one FFTJH_FLAG_MFMA_* bit; FP32 only)
+static int mat_mode_of(unsigned flags, int dtype) {
+  const unsigned known = FFTJH_FLAG_MFMA_F16 | FFTJH_FLAG_MFMA_BF16 | FFTJH_FLAG_MFMA_BF16X3;
+  if (flags & ~known) fail(FFTJH_ERR_ARGUMENT, "unknown plan flags");
+  if (!flags) return kMatNone;
+  if (flags & (flags - 1)) fail(FFTJH_ERR_ARGUMENT, "at most one split-precision flag");
+  if (dtype != FFTJH_F32) fail(FFTJH_ERR_ARGUMENT, "split-precision MFMA stages are for FP32 plans");
+  return flags == FFTJH_FLAG_MFMA_F16 ? kMatF16 : flags == FFTJH_FLAG_MFMA_BF16 ? kMatBF16 : kMatBF16x3;
+}
+
 static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch, int sign, double scale,
-                         const fftjh_tuning* tune = nullptr) {
+                         const fftjh_tuning* tune = nullptr, unsigned flags = 0) {
   check_dtype(dtype);
   if (rank < 1 || rank > 8) fail(FFTJH_ERR_ARGUMENT, "rank must be in 1..8");
   if (!dims) fail(FFTJH_ERR_ARGUMENT, "dims is NULL");
@@ -627,6 +690,7 @@ static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch
   }
   Layout L;
   if (tune) L.tune = *tune;
+  L.mat = mat_mode_of(flags | (tune ? tune->flags : 0u), dtype);
   L.dtype = dtype;
   L.kind = FFTJH_KIND_C2C;
   L.sign = sign;
@@ -1033,6 +1097,7 @@ static std::string describe(const Layout& L) {
       o << " n=" << s.d.n << " radices=";
       for (size_t i = 0; i < s.radices.size(); ++i) o << (i ? "," : "") << s.radices[i];
       if (s.pow2) o << " kernel=pow2";
+      if (s.mat) o << (s.mat == kMatF16 ? " kernel=mfma_f16" : s.mat == kMatBF16 ? " kernel=mfma_bf16" : " kernel=mfma_bf16x3");
       if (s.d.rows_in) o << " rows_in";
       if (s.d.rows_out) o << " rows_out";
       o << " batch=" << s.d.batch << " C=" << s.d.C << " threads=" << s.d.nthreads << " groups=" << s.d.ngroups
@@ -1113,8 +1178,18 @@ static void materialize(fftjh_plan_s& P) {
       // gain from it; a 1-workgroup-per-CU generic tile loses)
       const bool small_tile = static_cast<int64_t>(lds_elems(s.d.C * s.d.n)) * es <= 48 * 1024;
       const bool persist = persist_env >= 0 ? persist_env != 0 : (s.pow2 || small_tile);
-      const int resident =
-          !persist ? 0 : s.pow2 ? pass_pow2_resident_groups(dt, s.dir, s.d) : pass_resident_groups(dt, s.dir, s.d);
+      const int resident = !persist  ? 0
+                           : s.mat   ? pass_mat_resident_groups(s.mat, s.dir, s.d)
+                           : s.pow2  ? pass_pow2_resident_groups(dt, s.dir, s.d)
+                                     : pass_resident_groups(dt, s.dir, s.d);
+      if (s.mat) {
+        // split DFT-matrix operands of this mode and direction (FP64-generated)
+        std::ostringstream mk;
+        mk << "mat:" << dev << ":" << s.mat << ":" << s.dir;
+        const int mode = s.mat;
+        s.mat_tab = cached(mk.str(), mat_table_bytes(mode),
+                           [&](void* p) { hip_check(launch_mat_table(mode, dir, p, 0), "operand table kernel"); });
+      }
       s.d.grid = resident > 0 ? std::min(resident, s.d.ngroups) : s.d.ngroups;
       s.desc = std::make_shared<DevBuf>(sizeof(PassDesc));
       hip_check(hipMemcpy(s.desc->p, &s.d, sizeof(PassDesc), hipMemcpyHostToDevice), "descriptor upload");
@@ -1203,6 +1278,9 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
   const int dt = P.L.dtype;
   switch (s.kind) {
     case ST_PASS:
+      if (s.mat)
+        return launch_pass_mat(s.mat, s.dir, s.d, static_cast<const PassDesc*>(s.desc->p), ptr[s.src], ptr[s.dst],
+                               s.tw->p, s.mat_tab->p, st);
       if (s.pow2)
         return launch_pass_pow2(dt, s.dir, s.d, static_cast<const PassDesc*>(s.desc->p), ptr[s.src], ptr[s.dst],
                                 s.tw->p, st);
@@ -1290,8 +1368,7 @@ int fftjh_plan_c2c(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, i
                    double scale, unsigned flags) {
   return guarded([&] {
     if (plan) *plan = nullptr;
-    if (flags) fail(FFTJH_ERR_ARGUMENT, "flags must be 0");
-    make_plan(plan, layout_c2c(dtype, rank, dims, batch, sign, scale));
+    make_plan(plan, layout_c2c(dtype, rank, dims, batch, sign, scale, nullptr, flags));
   });
 }
 

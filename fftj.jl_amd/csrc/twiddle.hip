@@ -9,6 +9,7 @@
 #include "butterflies.h"
 #include "common.h"
 #include "launch.h"
+#include "stages_mat.h"
 
 namespace fftjh {
 
@@ -45,6 +46,68 @@ __global__ void k_tw_pow(V2<T>* out, int64_t count, int64_t mult, int64_t L, int
     expi_exact((i % L) * (mult % L), L, dir, &c, &s);
     out[i] = mk<T>(static_cast<T>(c), static_cast<T>(s));
   }
+}
+
+// split-precision DFT-matrix operands (layout: stages_mat.h): the entry of
+// output row m and input point p is w^{m p}, w = exp(dir*2*pi*i/R), from
+// an exact reduction of m*p mod R; each value is split in FP64 into
+// `terms` FP16 or BF16 parts (hi first) whose sum carries its significance
+__global__ void k_mat_table(uint16_t* out, int mode, int dir) {
+  const int terms = mat_terms(mode);
+  const int total = mat_entries(terms) * 8;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+    const int e = idx & 7, ent = idx >> 3;
+    int R, rel;
+    if (ent < mat_base(16, terms)) {
+      R = 9;
+      rel = ent;
+    } else if (ent < mat_base(32, terms)) {
+      R = 16;
+      rel = ent - mat_base(16, terms);
+    } else {
+      R = 32;
+      rel = ent - mat_base(32, terms);
+    }
+    const int KT = mat_kt(R);
+    const int lane = rel & 63;
+    int q = rel >> 6;
+    const int term = q % terms;
+    q /= terms;
+    const int comp = q & 1;
+    q >>= 1;
+    const int t = q % KT, h = q / KT;
+    const int m = 16 * h + (lane & 15);
+    const int p = 16 * t + 4 * (lane >> 4) + (e >> 1);
+    const int part = e & 1;
+    double v = 0.0;
+    if (m < R && p < R) {
+      double c, s;
+      expi_exact(static_cast<int64_t>(m) * p, R, dir, &c, &s);
+      // Re(y) = sum Re(w) xr - Im(w) xi ;  Im(y) = sum Im(w) xr + Re(w) xi
+      v = comp == 0 ? (part == 0 ? c : -s) : (part == 0 ? s : c);
+    }
+    uint16_t bits = 0;
+    for (int k = 0; k <= term; ++k) {
+      if (mode == kMatF16) {
+        const _Float16 hv = static_cast<_Float16>(static_cast<float>(v));
+        bits = __builtin_bit_cast(uint16_t, hv);
+        v -= static_cast<double>(hv);
+      } else {
+        const __bf16 bv = static_cast<__bf16>(static_cast<float>(v));
+        bits = __builtin_bit_cast(uint16_t, bv);
+        v -= static_cast<double>(static_cast<float>(bv));
+      }
+    }
+    out[idx] = bits;
+  }
+}
+
+int64_t mat_table_bytes(int mode) { return static_cast<int64_t>(mat_entries(mat_terms(mode))) * 16; }
+
+hipError_t launch_mat_table(int mode, int dir, void* out, hipStream_t stream) {
+  if (mode < kMatF16 || mode > kMatBF16x3) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mat_table, dim3(64), dim3(256), 0, stream, static_cast<uint16_t*>(out), mode, dir);
+  return hipGetLastError();
 }
 
 static dim3 grid_for(int64_t count) {

@@ -47,6 +47,16 @@ extern "C" {
 #define FFTJH_KIND_C2R 2
 #define FFTJH_KIND_RAGGED 3
 
+/* Plan flags (complex FP32 plans; at most one).  Radix-9, -16 and -32 stages
+ * run as DFT-matrix products on gfx950 MFMA with split operands and FP32
+ * accumulation; no reference counterpart (the reference is CPU-only):
+ *   FFTJH_FLAG_MFMA_F16     FP16 hi + lo, per-butterfly power-of-two scale
+ *   FFTJH_FLAG_MFMA_BF16    BF16 hi + lo (16 significant bits)
+ *   FFTJH_FLAG_MFMA_BF16X3  BF16 hi + mid + lo (24 significant bits) */
+#define FFTJH_FLAG_MFMA_F16 0x1u
+#define FFTJH_FLAG_MFMA_BF16 0x2u
+#define FFTJH_FLAG_MFMA_BF16X3 0x4u
+
 typedef struct fftjh_plan_s* fftjh_plan;
 /* hipStream_t, passed as an opaque pointer (NULL = the null stream) */
 typedef void* fftjh_stream;
@@ -57,8 +67,9 @@ typedef void* fftjh_stream;
  *   plan_fft / plan_ifft / plan_bfft(x::Array{Complex{T},N})  api.jl:135-152
  * (all dims transformed, dims[0] fastest).  batch > 1 runs that many
  * independent transforms stored back to back (an extension: the reference
- * has no batch API and loops over 1-D plans).  flags must be 0 (the
- * reference accepts and ignores FFTW planner flags, api.jl:107-108). */
+ * has no batch API and loops over 1-D plans).  flags: 0, or one
+ * FFTJH_FLAG_MFMA_* bit (the reference accepts and ignores FFTW planner
+ * flags, api.jl:107-108; these are this library's own). */
 int fftjh_plan_c2c(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
                    double scale, unsigned flags);
 
@@ -69,7 +80,8 @@ int fftjh_plan_c2c(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, i
 typedef struct fftjh_pass_tuning {
   int32_t C;            /* transforms per workgroup, a power of two <= 64 */
   int32_t nradix;       /* > 0: use radices[0..nradix) as the pass's stage chain */
-  int32_t radices[16];  /* each in {2,3,4,5,7,8,11,13,16}, product = pass length */
+  int32_t radices[16];  /* each in {2,3,4,5,7,8,11,13,16} (and 9, 32 with an
+                           MFMA flag), product = pass length */
   int32_t xcd_remap;    /* 0 planner, 1 off, 2 on */
 } fftjh_pass_tuning;
 
@@ -78,6 +90,8 @@ typedef struct fftjh_tuning {
                                    contiguous row pass (run last), then outward;
                                    zero-terminated, product = n */
   fftjh_pass_tuning pass[4];    /* by launch order of the pass kernels */
+  uint32_t flags;               /* FFTJH_FLAG_* as for fftjh_plan_c2c */
+  uint32_t pad_;
 } fftjh_tuning;
 
 /* fftjh_plan_c2c with explicit search-space choices (NULL = planner's). */

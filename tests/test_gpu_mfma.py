@@ -1,0 +1,122 @@
+"""Split-precision MFMA stages: radix-9/16/32 stages as DFT-matrix products
+on gfx950 MFMA with split operands, against the FP64 oracle on the same
+seeded inputs.  FP16 hi/lo (scaled) and BF16 hi/mid/lo meet the FP32 bar
+3e-6*log2(N); BF16 hi/lo (16 significant bits) is held to its own stated
+bound, 4e-5*log2(N)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from conftest import F, inputs_c, rel_l2, to_dev, to_host, tol_for
+
+pytestmark = pytest.mark.gpu
+MODES = ["f16", "bf16", "bf16x3"]
+
+
+def tol_mode(mode, n):
+    if mode == "bf16":
+        return 4e-5 * max(1.0, float(np.log2(max(n, 2))))
+    return tol_for("f32", n)
+
+
+def _run(mode, n, batch, x, sign=-1, tuning=None):
+    if tuning is not None:
+        t = F._lib.make_tuning(tuning.get("splits", ()), tuning.get("passes", ()), F._lib.PRECISION_FLAGS[mode])
+        p = F.FFTPlan(0, (n,), sign, 1.0, batch, tuning=t)
+    else:
+        mk = F.plan_fft if sign < 0 else F.plan_bfft
+        p = mk(torch.float32, n, batch=batch, precision=mode)
+    assert "kernel=mfma_" + mode + "\n" in p.describe() or "kernel=mfma_" + mode + " " in p.describe()
+    return to_host(p * to_dev(x))
+
+
+SIZES = [(16, 3), (256, 5), (4096, 2), (1024, 4), (81, 7), (729, 3), (1458, 2), (9 * 16 * 4, 3), (48, 9),
+         (1 << 16, 2), (3 ** 7 * 2, 2), (2 ** 14 * 9, 1), (5 * 16 * 16, 3), (7 * 9 * 16, 2)]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n,batch", SIZES)
+def test_mfma_parity(mode, n, batch):
+    x = inputs_c(n + batch, n * batch, "f32")
+    ref = O.fft(x.astype(np.complex128).reshape(batch, n))
+    y = _run(mode, n, batch, x).reshape(batch, n)
+    err = rel_l2(y, ref)
+    assert err <= tol_mode(mode, n), (mode, n, batch, err)
+    refb = O.fft(x.astype(np.complex128).reshape(batch, n), sign=1)
+    yb = _run(mode, n, batch, x, sign=1).reshape(batch, n)
+    assert rel_l2(yb, refb) <= tol_mode(mode, n)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n,radices", [(1024, [32, 32]), (32, [32]), (512, [16, 32]), (288, [32, 9])])
+def test_mfma_radix32(mode, n, radices):
+    batch = 3
+    x = inputs_c(n, n * batch, "f32")
+    ref = O.fft(x.astype(np.complex128).reshape(batch, n))
+    y = _run(mode, n, batch, x, tuning={"passes": [{"radices": radices}]}).reshape(batch, n)
+    assert rel_l2(y, ref) <= tol_mode(mode, n), (mode, n, radices)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("kind", ["constant", "impulse", "huge", "tiny", "ramp_range"])
+def test_mfma_dynamic_range(mode, kind):
+    """FP16 needs its per-butterfly scale: a constant input puts N in one
+    output bin, far beyond FP16's 65504; tiny and huge inputs leave its range."""
+    n, batch = 4096, 2
+    if kind == "constant":
+        x = np.ones(n * batch, np.complex64) * (3.0 + 1.0j)
+    elif kind == "impulse":
+        x = np.zeros(n * batch, np.complex64)
+        x[5] = 1.0
+        x[n + 77] = -2.0j
+    elif kind == "huge":
+        x = (inputs_c(3, n * batch, "f32") * np.float32(1e30)).astype(np.complex64)
+    elif kind == "tiny":
+        x = (inputs_c(4, n * batch, "f32") * np.float32(1e-30)).astype(np.complex64)
+    else:
+        x = (inputs_c(5, n * batch, "f32") * np.logspace(-6, 6, n * batch).astype(np.float32)).astype(np.complex64)
+    ref = O.fft(x.astype(np.complex128).reshape(batch, n))
+    y = _run(mode, n, batch, x).reshape(batch, n)
+    assert np.all(np.isfinite(y))
+    assert rel_l2(y, ref) <= tol_mode(mode, n), (mode, kind, rel_l2(y, ref))
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_mfma_ifft_and_round_trip(mode):
+    n, batch = 1 << 12, 4
+    x = inputs_c(11, n * batch, "f32")
+    p = F.plan_fft(torch.float32, n, batch=batch, precision=mode)
+    pi = F.plan_ifft(torch.float32, n, batch=batch, precision=mode)
+    y = p * to_dev(x)
+    back = to_host(pi * y)
+    assert rel_l2(back, x) <= 2 * tol_mode(mode, n)
+    q = F.inv(p)
+    assert rel_l2(to_host(q * y), x) <= 2 * tol_mode(mode, n)
+
+
+# the two configurations of the split-precision study: N = 2^20 and
+# N = 3^12 * 4, batch 64; the error is taken over 4 of the 64 transforms
+# (the oracle's share), all 64 being computed
+CONFIGS = [(1 << 20, {"splits": [1024, 1024]}), (1 << 20, None), (3 ** 12 * 4, None), (3 ** 12 * 4, {"splits": [1458, 1458]})]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n,tuning", CONFIGS)
+def test_mfma_study_configs(mode, n, tuning):
+    batch, check_rows = 64, 4
+    x = inputs_c(20 + n % 97, n * batch, "f32")
+    y = _run(mode, n, batch, x, tuning=tuning).reshape(batch, n)
+    rows = np.linspace(0, batch - 1, check_rows).astype(int)
+    ref = O.fft(x.reshape(batch, n)[rows].astype(np.complex128))
+    err = rel_l2(y[rows], ref)
+    # plain FP32 chain on the same inputs, for the record
+    yp = to_host(F.plan_fft(torch.float32, n, batch=batch) * to_dev(x)).reshape(batch, n)
+    err_plain = rel_l2(yp[rows], ref)
+    out = os.environ.get("FFTJH_MFMA_REPORT")
+    if out:
+        with open(out, "a") as fh:
+            fh.write(f"{mode} n={n} batch={batch} tuning={tuning} rel_l2={err:.3e} plain_fp32={err_plain:.3e}\n")
+    assert err <= tol_mode(mode, n), (mode, n, err, err_plain)

@@ -104,3 +104,31 @@ def test_region_check():
     x = torch.zeros(4, 4, dtype=torch.complex64)
     with pytest.raises(F.ArgumentError, match="all dims"):
         F.plan_fft(x, region=(1,))
+
+
+def test_mfma_flags_and_schedules():
+    """Split-precision flags: one FP32 mode at a time; the chain takes radix
+    16 and 9 stages on MFMA (radix 32 by tuning), the rest on VALU."""
+    L = _lib
+    for bits, name in ((L.FLAG_MFMA_F16, "mfma_f16"), (L.FLAG_MFMA_BF16, "mfma_bf16"),
+                       (L.FLAG_MFMA_BF16X3, "mfma_bf16x3")):
+        d = L.describe_tuned(L.F32, (4096,), 4, -1, L.make_tuning((), (), bits))
+        assert f"kernel={name} " in d and "radices=16,16,16" in d
+    d = L.describe_tuned(L.F32, (3 ** 6 * 2,), 2, -1, L.make_tuning((), (), L.FLAG_MFMA_F16))
+    assert "radices=2,9,9,9" in d
+    d = L.describe_tuned(L.F32, (1024,), 2, -1, L.make_tuning((), [{"radices": [32, 32]}], L.FLAG_MFMA_BF16))
+    assert "radices=32,32 kernel=mfma_bf16 " in d
+    with pytest.raises(L.ArgumentError):
+        L.describe_tuned(L.F64, (4096,), 1, -1, L.make_tuning((), (), L.FLAG_MFMA_F16))
+    with pytest.raises(L.ArgumentError):
+        L.describe_tuned(L.F32, (4096,), 1, -1, L.make_tuning((), (), L.FLAG_MFMA_F16 | L.FLAG_MFMA_BF16))
+    with pytest.raises(L.ArgumentError):
+        L.describe_tuned(L.F32, (4096,), 1, -1, L.make_tuning((), (), 0x100))
+    with pytest.raises(L.ArgumentError):  # radix 32 has no VALU kernel
+        L.describe_tuned(L.F32, (1024,), 1, -1, L.make_tuning((), [{"radices": [32, 32]}]))
+    # every MFMA pass stays within the 512-thread bound of its kernel
+    for n in (1 << 20, 3 ** 12 * 4, 1 << 16, 1458, 4374, 9 * 16 * 16 * 16):
+        d = L.describe_tuned(L.F32, (n,), 64, -1, L.make_tuning((), (), L.FLAG_MFMA_BF16X3))
+        for ln in d.splitlines():
+            if "threads=" in ln:
+                assert int(ln.split("threads=")[1].split()[0]) <= 512
