@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip the split-precision and learned-planner legs")
+    ap.add_argument("--split-only", action="store_true",
+                    help="run only the split-precision leg (one JSON line; for kernel traces)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 3 forward plans of the workload, then 3 copies of the same bytes "
                          "(the calibration kernel of known traffic); no timing, no JSON")
@@ -150,6 +153,61 @@ def ragged_leg(F, quick=False):
             "candidates": [[c, round(ms, 4)] for c, ms in results]}
 
 
+def split_precision_leg(F):
+    """N = 2^20 and N = 3^12*4, batch 64, forward: the plain FP32 chain next to
+    the split-precision MFMA chains (FP16 hi/lo, BF16 hi/lo, BF16 hi/mid/lo).
+    Accuracy against the FP64 oracle is measured by the tests; here the
+    round trip forward -> inverse of each mode is reported."""
+    import numpy as np
+    import torch
+
+    out = []
+    for n in (1 << 20, 3 ** 12 * 4):
+        batch = 64
+        x = torch.from_numpy(F.rng.uniform_complex(2, n * batch).astype(np.complex64)).to("cuda:0")
+        y = torch.empty_like(x)
+        z = torch.empty_like(x)
+        for mode in (None, "f16", "bf16", "bf16x3"):
+            pf = F.plan_fft(torch.float32, n, batch=batch, precision=mode)
+            pi = F.plan_ifft(torch.float32, n, batch=batch, precision=mode)
+            ms = F.tune.time_plan(pf, y, x, iters=20)
+            pi(z, y)
+            rt = (torch.linalg.vector_norm(z - x) / torch.linalg.vector_norm(x)).item()
+            launches = pf.profile_launches(y, x, iters=10)
+            out.append({"n": n, "batch": batch, "precision": mode or "fp32", "ms": round(ms, 4),
+                        "gflops": round(5.0 * n * math.log2(n) * batch / (ms * 1e-3) / 1e9, 1),
+                        "gbs_transform": round(16.0 * n * batch / (ms * 1e-3) / 1e9, 1),
+                        "launch_ms": [round(v, 5) for v in launches], "roundtrip_rel_err": rt})
+            del pf, pi
+        del x, y, z
+    return out
+
+
+def planner_leg(F):
+    """Held-out lengths of the learned planner's training set: the default
+    schedule vs the one the trained cost model picks, timed now."""
+    import numpy as np
+    import torch
+
+    P = F.planner
+    data = P.load_dataset()
+    lengths = sorted({int(n) for n in data["n"][P.held_out_mask(data)]})[::8]
+    lp = P.LearnedPlanner()
+    rows = []
+    for n in lengths:
+        batch = P.batch_for(n)
+        x = torch.randn(n * batch, dtype=torch.complex64, device="cuda:0")
+        y = torch.empty_like(x)
+        t_def = P.time_plan(P.make_plan(n, batch, {}), y, x) * 1e3
+        p, cand, _ = lp.choose(n, batch)
+        t_l = P.time_plan(p, y, x) * 1e3
+        rows.append((n, t_def, t_l))
+    r = np.array(rows, dtype=np.float64)
+    return {"held_out_lengths": len(rows), "geomean_speedup_learned_over_default":
+            round(float(np.exp(np.mean(np.log(r[:, 1] / r[:, 2])))), 4),
+            "cases": [[int(a), round(b, 2), round(c, 2)] for a, b, c in rows]}
+
+
 def main():
     args = parse()
     if args.pmc_probe:
@@ -164,6 +222,9 @@ def main():
     import __graft_entry__
 
     F = __graft_entry__.load_package()
+    if args.split_only:
+        print(json.dumps({"split_precision": split_precision_leg(F)}), flush=True)
+        return
     n, batch, desc = WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -241,6 +302,9 @@ def main():
     }
     if not args.no_ragged:
         result["ragged"] = ragged_leg(F)
+    if not args.no_extras:
+        result["split_precision"] = split_precision_leg(F)
+        result["learned_planner"] = planner_leg(F)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(n, batch, args.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -19,8 +19,12 @@ template <> struct MaxThreads<double> { static constexpr int value = kMaxThreads
 // radices 9, 16 and 32 (operand table `mat`); the other radices run on VALU.
 // (the MFMA chains keep twice the registers per lane: at most 512 threads)
 constexpr int kMatMaxThreads = kMaxThreads / 2;
+#ifndef FFTJH_MAT_WAVES
+#define FFTJH_MAT_WAVES 1
+#endif
 template <typename T, int DIR, int MAT = kMatNone>
 __global__ void __launch_bounds__(MAT != kMatNone ? kMatMaxThreads : MaxThreads<T>::value)
+__attribute__((amdgpu_waves_per_eu(MAT != kMatNone ? FFTJH_MAT_WAVES : 1)))
 k_pass(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<T>* tw, const void* mat) {
   extern __shared__ __align__(16) unsigned char smem[];
   V2<T>* lds = reinterpret_cast<V2<T>*>(smem);
@@ -62,22 +66,22 @@ k_pass(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<T>* t
       if constexpr (MAT != kMatNone) {
         if (st.radix == 9 || st.radix == 16 || st.radix == 32) {
           const int lane = tid & 63, wave = tid >> 6, nwaves = d.nthreads >> 6;
-          if (st.radix == 9) run_stage_mat<DIR, MAT, 9>(n, C, d.logC, st, lds, tw, mat, lane, wave, nwaves);
-          else if (st.radix == 16) run_stage_mat<DIR, MAT, 16>(n, C, d.logC, st, lds, tw, mat, lane, wave, nwaves);
-          else run_stage_mat<DIR, MAT, 32>(n, C, d.logC, st, lds, tw, mat, lane, wave, nwaves);
+          if (st.radix == 9) run_stage_mat<DIR, MAT, 9>(n, C, d.logC, st, as_lds<T>(lds), as_tw<T>(tw), mat, lane, wave, nwaves);
+          else if (st.radix == 16) run_stage_mat<DIR, MAT, 16>(n, C, d.logC, st, as_lds<T>(lds), as_tw<T>(tw), mat, lane, wave, nwaves);
+          else run_stage_mat<DIR, MAT, 32>(n, C, d.logC, st, as_lds<T>(lds), as_tw<T>(tw), mat, lane, wave, nwaves);
           continue;
         }
       }
       switch (st.radix) {
-        case 2: run_stage<T, DIR, 2>(n, C, st, lds, tw, c, jt, jstep); break;
-        case 3: run_stage<T, DIR, 3>(n, C, st, lds, tw, c, jt, jstep); break;
-        case 4: run_stage<T, DIR, 4>(n, C, st, lds, tw, c, jt, jstep); break;
-        case 5: run_stage<T, DIR, 5>(n, C, st, lds, tw, c, jt, jstep); break;
-        case 7: run_stage<T, DIR, 7>(n, C, st, lds, tw, c, jt, jstep); break;
-        case 8: run_stage<T, DIR, 8>(n, C, st, lds, tw, c, jt, jstep); break;
-        case 11: run_stage<T, DIR, 11>(n, C, st, lds, tw, c, jt, jstep); break;
-        case 13: run_stage<T, DIR, 13>(n, C, st, lds, tw, c, jt, jstep); break;
-        default: run_stage<T, DIR, 16>(n, C, st, lds, tw, c, jt, jstep); break;
+        case 2: run_stage<T, DIR, 2>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 3: run_stage<T, DIR, 3>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 4: run_stage<T, DIR, 4>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 5: run_stage<T, DIR, 5>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 7: run_stage<T, DIR, 7>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 8: run_stage<T, DIR, 8>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 11: run_stage<T, DIR, 11>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 13: run_stage<T, DIR, 13>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        default: run_stage<T, DIR, 16>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
       }
     }
     // store, with the multi-pass twiddle and the scale

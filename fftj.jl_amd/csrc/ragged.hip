@@ -28,7 +28,7 @@ namespace {
 constexpr int kRT = kRaggedThreads;
 
 template <typename T, int DIR>
-__device__ void tile_stages(V2<T>* lds, const RaggedSched& sc, int C, int logC, const V2<T>* tw) {
+__device__ void tile_stages(LdsPtr<T> lds, const RaggedSched& sc, int C, int logC, TwPtr<T> tw) {
   const int tid = static_cast<int>(threadIdx.x);
   const int c = tid & (C - 1), jt = tid >> logC, jstep = kRT >> logC;
   for (int s = 0; s < sc.nstages; ++s) {
@@ -50,14 +50,17 @@ __device__ void tile_stages(V2<T>* lds, const RaggedSched& sc, int C, int logC, 
 // length-n transform src -> dst (both contiguous, global) as two tile phases
 // through `mid` (global, n elements); dst may equal src
 template <typename T, int DIR>
-__device__ void two_level(V2<T>* lds, const RaggedItem& it, const RaggedSched* scheds, const V2<T>* tw,
-                          const V2<T>* src, V2<T>* mid, V2<T>* dst, T scale) {
+__device__ void two_level(LdsPtr<T> lds, const RaggedItem& it, const RaggedSched* scheds, TwPtr<T> tw,
+                          const V2<T>* src_, V2<T>* mid_, V2<T>* dst_, T scale) {
+  const TwPtr<T> src = as_tw<T>(src_);
+  const GPtr<T> mid = as_g<T>(mid_);
+  const GPtr<T> dst = as_g<T>(dst_);
   const int tid = static_cast<int>(threadIdx.x);
   const int n1 = it.n1, n2 = it.n2;
   const RaggedSched& sa = scheds[it.sA];
   const RaggedSched& sb = scheds[it.sB];
-  const V2<T>* thi = tw + it.tw_hi;
-  const V2<T>* tlo = tw + it.tw_lo;
+  TwPtr<T> thi = tw + it.tw_hi;
+  TwPtr<T> tlo = tw + it.tw_lo;
   const uint32_t lo_mask = (1u << it.lo_bits) - 1u;
   // phase A: columns j1 (stride n1) of length n2, twiddle w_n^{j1*k2}, mid[j1 + n1*k2]
   {
@@ -66,7 +69,9 @@ __device__ void two_level(V2<T>* lds, const RaggedItem& it, const RaggedSched* s
       for (int f = tid; f < C * n2; f += kRT) {
         const int c = f & (C - 1), i = f >> logC;
         const int col = c0 + c;
-        lds[lpad(i * C + c)] = col < n1 ? src[col + static_cast<int64_t>(n1) * i] : mk<T>(T(0), T(0));
+        V2<T> v = mk<T>(T(0), T(0));
+        if (col < n1) v = ldv<T>(src, col + static_cast<int64_t>(n1) * i);
+        stv(lds, lpad(i * C + c), v);
       }
       __syncthreads();
       tile_stages<T, DIR>(lds, sa, C, logC, tw);
@@ -75,8 +80,8 @@ __device__ void two_level(V2<T>* lds, const RaggedItem& it, const RaggedSched* s
         const int col = c0 + c;
         if (col < n1) {
           const uint32_t e = static_cast<uint32_t>(col) * static_cast<uint32_t>(k2);
-          const V2<T> w = cmul<T>(thi[e >> it.lo_bits], tlo[e & lo_mask]);
-          mid[col + static_cast<int64_t>(n1) * k2] = cmul<T>(lds[lpad(k2 * C + c)], w);
+          const V2<T> w = cmul<T>(ldv<T>(thi, e >> it.lo_bits), ldv<T>(tlo, e & lo_mask));
+          stv(mid, col + static_cast<int64_t>(n1) * k2, cmul<T>(ldv<T>(lds, lpad(k2 * C + c)), w));
         }
       }
       __syncthreads();
@@ -91,14 +96,16 @@ __device__ void two_level(V2<T>* lds, const RaggedItem& it, const RaggedSched* s
       for (int f = tid; f < C * n1; f += kRT) {
         const int c = f & (C - 1), i = f >> logC;
         const int row = r0 + c;
-        lds[lpad(i * C + c)] = row < n2 ? mid[static_cast<int64_t>(row) * n1 + i] : mk<T>(T(0), T(0));
+        V2<T> v = mk<T>(T(0), T(0));
+        if (row < n2) v = ldv<T>(mid, static_cast<int64_t>(row) * n1 + i);
+        stv(lds, lpad(i * C + c), v);
       }
       __syncthreads();
       tile_stages<T, DIR>(lds, sb, C, logC, tw);
       for (int f = tid; f < C * n1; f += kRT) {
         const int c = f & (C - 1), k1 = f >> logC;
         const int row = r0 + c;
-        if (row < n2) dst[row + static_cast<int64_t>(n2) * k1] = cscale<T>(lds[lpad(k1 * C + c)], scale);
+        if (row < n2) stv(dst, row + static_cast<int64_t>(n2) * k1, cscale<T>(ldv<T>(lds, lpad(k1 * C + c)), scale));
       }
       __syncthreads();
     }
@@ -143,6 +150,8 @@ k_ragged(const RaggedItem* __restrict__ items, const int32_t* __restrict__ wg_pt
          const T* __restrict__ corr, const V2<T>* x, V2<T>* y, V2<T>* ws, const int64_t* __restrict__ wg_ws) {
   extern __shared__ __align__(16) unsigned char smem[];
   V2<T>* lds = reinterpret_cast<V2<T>*>(smem);
+  const LdsPtr<T> l3 = as_lds<T>(lds);
+  const TwPtr<T> t1 = as_tw<T>(tw);
   const int tid = static_cast<int>(threadIdx.x);
   const int wg = static_cast<int>(blockIdx.x);
   // this workgroup's workspace: two-level items use n elements, two-level
@@ -158,11 +167,11 @@ k_ragged(const RaggedItem* __restrict__ items, const int32_t* __restrict__ wg_pt
       const RaggedSched& sc = scheds[it.sA];
       for (int i = tid; i < n; i += kRT) lds[lpad(i)] = xi[i];
       __syncthreads();
-      tile_stages<T, DIR>(lds, sc, 1, 0, tw);
+      tile_stages<T, DIR>(l3, sc, 1, 0, t1);
       for (int k = tid; k < n; k += kRT) yi[k] = cscale<T>(lds[lpad(k)], scale);
       __syncthreads();
     } else if (it.kind == kRaggedTwoLevel) {
-      two_level<T, DIR>(lds, it, scheds, tw, xi, wsA, yi, scale);
+      two_level<T, DIR>(l3, it, scheds, t1, xi, wsA, yi, scale);
     } else {
       // resampling: corr carries the scale; the m-point transform is unscaled
       const int m = it.m, hp = (n + 1) / 2, hi0 = m - n + hp;
@@ -173,7 +182,7 @@ k_ragged(const RaggedItem* __restrict__ items, const int32_t* __restrict__ wg_pt
           lds[lpad(j)] = src >= 0 ? cscale<T>(xi[src], cr[src]) : mk<T>(T(0), T(0));
         }
         __syncthreads();
-        tile_stages<T, DIR>(lds, scheds[it.sA], 1, 0, tw);
+        tile_stages<T, DIR>(l3, scheds[it.sA], 1, 0, t1);
         interp<T, true>(it, lds, yi);
         __syncthreads();
       } else {
@@ -185,7 +194,7 @@ k_ragged(const RaggedItem* __restrict__ items, const int32_t* __restrict__ wg_pt
         }
         __threadfence_block();
         __syncthreads();
-        two_level<T, DIR>(lds, it, scheds, tw, grid, mid, grid, T(1));
+        two_level<T, DIR>(l3, it, scheds, t1, grid, mid, grid, T(1));
         interp<T, false>(it, grid, yi);
         __threadfence_block();
         __syncthreads();

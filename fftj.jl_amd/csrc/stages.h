@@ -10,6 +10,35 @@ namespace fftjh {
 // rows a thread moves per unrolled batch of the load phase
 constexpr int kIoUnroll = 8;
 
+// The out-of-line stages take LDS and the twiddle table through
+// address-space-typed pointers: ds_read/ds_write and global_load with the
+// base in a register.  (A generic pointer to the kernel's dynamic LDS would
+// be resolved per access through the per-kernel LDS offset table, a scalar
+// load and wait in front of every LDS instruction.)
+// Elements behind these pointers are plain two-lane vectors (no
+// constructors, so they are copied across address spaces on both compiler
+// passes); ldv/stv convert to and from V2.
+template <typename T> using NV2 = T __attribute__((ext_vector_type(2)));
+template <typename T> using LdsPtr = __attribute__((address_space(3))) NV2<T>*;
+template <typename T> using TwPtr = const __attribute__((address_space(1))) NV2<T>*;
+template <typename T> using GPtr = __attribute__((address_space(1))) NV2<T>*;
+template <typename T, typename P> __device__ __forceinline__ V2<T> ldv(P p, int64_t i) {
+  const NV2<T> e = p[i];
+  return mk<T>(e.x, e.y);
+}
+template <typename P, typename V> __device__ __forceinline__ void stv(P p, int64_t i, const V& v) {
+  p[i] = {v.x, v.y};
+}
+// (the base goes through an empty asm so that it stays a run-time argument:
+// constant-propagated into the stage it would turn back into the table form)
+template <typename T> __device__ __forceinline__ LdsPtr<T> as_lds(V2<T>* p) {
+  unsigned a = static_cast<unsigned>(reinterpret_cast<uintptr_t>((LdsPtr<T>)(p)));
+  asm volatile("" : "+s"(a));
+  return (LdsPtr<T>)(static_cast<uintptr_t>(a));
+}
+template <typename T> __device__ __forceinline__ TwPtr<T> as_tw(const V2<T>* p) { return (TwPtr<T>)(p); }
+template <typename T> __device__ __forceinline__ GPtr<T> as_g(V2<T>* p) { return (GPtr<T>)(p); }
+
 // LDS slot of flat index f = i*C + c: one pad element every 16.  With 8- or
 // 16-byte elements this turns the stride-R scatter of the early stages (and
 // the ns-blocked scatter of the later ones) into the minimum number of bank
@@ -22,8 +51,8 @@ __device__ __forceinline__ int lpad(int f) { return f + (f >> 4); }
 // twiddle loads are issued first so their latency overlaps the LDS reads and
 // the barrier.
 template <typename T, int DIR, int R>
-__device__ __noinline__ void run_stage(const int n, const int C, const StageDesc& s, V2<T>* lds,
-                                       const V2<T>* tw, const int c, const int jt,
+__device__ __noinline__ void run_stage(const int n, const int C, const StageDesc s, LdsPtr<T> lds,
+                                       TwPtr<T> tw, const int c, const int jt,
                                        const int jstep) {
   constexpr int BPT = kElemsPerThread / R;
   const int nr = n / R;
@@ -39,9 +68,9 @@ __device__ __noinline__ void run_stage(const int n, const int C, const StageDesc
       const int q = static_cast<int>(__umulhi(static_cast<unsigned>(j), s.ns_magic));
       const int k = j - q * ns;
       kk[b] = k;
-      const V2<T>* twp = tw + (s.tw_off + k);
+      TwPtr<T> twp = tw + (s.tw_off + k);
 #pragma unroll
-      for (int r = 1; r < R; ++r) w[b][r - 1] = twp[(r - 1) * ns];
+      for (int r = 1; r < R; ++r) w[b][r - 1] = ldv<T>(twp, (r - 1) * ns);
     }
   }
 #pragma unroll
@@ -49,7 +78,7 @@ __device__ __noinline__ void run_stage(const int n, const int C, const StageDesc
     const int j = jt + b * jstep;
     if (j < nr) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = lds[lpad((j + r * nr) * C + c)];
+      for (int r = 0; r < R; ++r) v[b][r] = ldv<T>(lds, lpad((j + r * nr) * C + c));
     }
   }
   __syncthreads();
@@ -64,7 +93,7 @@ __device__ __noinline__ void run_stage(const int n, const int C, const StageDesc
       dft<T, DIR, R>(v[b]);
       const int base = (j - kk[b]) * R + kk[b];
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lpad((base + r * ns) * C + c)] = v[b][r];
+      for (int r = 0; r < R; ++r) stv(lds, lpad((base + r * ns) * C + c), v[b][r]);
     }
   }
   __syncthreads();

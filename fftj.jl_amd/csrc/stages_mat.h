@@ -87,23 +87,26 @@ template <int MODE> struct MatBf16 {
 template <> struct MatOps<kMatBF16> : MatBf16<kMatBF16> {};
 template <> struct MatOps<kMatBF16x3> : MatBf16<kMatBF16x3> {};
 
-// acc += A*B over the split terms, smallest products first
+// zr += Ar*B, zi += Ai*B over the split terms, smallest products first; the
+// two components alternate so that consecutive MFMAs are independent
 template <typename Ops>
-__device__ __forceinline__ mat_f4 mat_dot(const typename Ops::V (&a)[Ops::terms], const typename Ops::V (&b)[Ops::terms],
-                                          mat_f4 acc) {
+__device__ __forceinline__ void mat_dot2(const typename Ops::V (&ar)[Ops::terms], const typename Ops::V (&ai)[Ops::terms],
+                                         const typename Ops::V (&b)[Ops::terms], mat_f4& zr, mat_f4& zi) {
   if constexpr (Ops::terms == 2) {
-    acc = Ops::mma(a[1], b[0], acc);
-    acc = Ops::mma(a[0], b[1], acc);
-    acc = Ops::mma(a[0], b[0], acc);
+    constexpr int ia[3] = {1, 0, 0}, ib[3] = {0, 1, 0};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      zr = Ops::mma(ar[ia[q]], b[ib[q]], zr);
+      zi = Ops::mma(ai[ia[q]], b[ib[q]], zi);
+    }
   } else {
-    acc = Ops::mma(a[2], b[0], acc);
-    acc = Ops::mma(a[0], b[2], acc);
-    acc = Ops::mma(a[1], b[1], acc);
-    acc = Ops::mma(a[1], b[0], acc);
-    acc = Ops::mma(a[0], b[1], acc);
-    acc = Ops::mma(a[0], b[0], acc);
+    constexpr int ia[6] = {2, 0, 1, 1, 0, 0}, ib[6] = {0, 2, 1, 0, 1, 0};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      zr = Ops::mma(ar[ia[q]], b[ib[q]], zr);
+      zi = Ops::mma(ai[ia[q]], b[ib[q]], zi);
+    }
   }
-  return acc;
 }
 
 // One radix-R stage (R in {9, 16, 32}) over the C columns in LDS, the same
@@ -111,8 +114,8 @@ __device__ __forceinline__ mat_f4 mat_dot(const typename Ops::V (&a)[Ops::terms]
 // column beta % 16 of tile beta / 16; wave w takes tiles w, w + nwaves, ...
 // All 64 lanes run every MFMA (inactive columns carry zeros).
 template <int DIR, int MODE, int R>
-__device__ __noinline__ void run_stage_mat(const int n, const int C, const int logC, const StageDesc& s,
-                                           V2<float>* lds, const V2<float>* tw, const void* mat, const int lane,
+__device__ __noinline__ void run_stage_mat(const int n, const int C, const int logC, const StageDesc s,
+                                           LdsPtr<float> lds, TwPtr<float> tw, const void* mat, const int lane,
                                            const int wave, const int nwaves) {
   using Ops = MatOps<MODE>;
   using V = typename Ops::V;
@@ -124,6 +127,16 @@ __device__ __noinline__ void run_stage_mat(const int n, const int C, const int l
   const int ns = s.ns;
   const int nb = nr * C;
   const int g = lane >> 4, col = lane & 15;
+  const __attribute__((address_space(1))) V* A = (const __attribute__((address_space(1))) V*)(mat) + mat_base(R, T);
+  // R <= 16: one K-tile, both components' operands held for the whole stage
+  // (their loads overlap the gather and the barrier)
+  V a1[2][T];
+  if constexpr (KT == 1) {
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp)
+#pragma unroll
+      for (int t = 0; t < T; ++t) a1[comp][t] = A[(comp * T + t) * 64 + lane];
+  }
   V2<float> v[TPW][P];
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
@@ -137,22 +150,18 @@ __device__ __noinline__ void run_stage_mat(const int n, const int C, const int l
       for (int q = 0; q < P; ++q) {
         const int r = 16 * (q >> 2) + 4 * g + (q & 3);
         if (r < R) {
-          V2<float> a = lds[lpad((j + r * nr) * C + c)];
-          if (r > 0 && ns > 1) a = cmul<float>(a, tw[s.tw_off + (r - 1) * ns + k]);
+          V2<float> a = ldv<float>(lds, lpad((j + r * nr) * C + c));
+          if (r > 0 && ns > 1) a = cmul<float>(a, ldv<float>(tw, s.tw_off + (r - 1) * ns + k));
           v[u][q] = a;
         }
       }
     }
   }
   __syncthreads();
-  const V* A = static_cast<const V*>(mat) + mat_base(R, T);
 #pragma unroll
   for (int u = 0; u < TPW; ++u) {
     const int tile = wave + u * nwaves;
     if (tile * 16 >= nb) break;  // uniform across the wave
-    // the operands are read again for every tile (L1 hits): kept live across
-    // tiles they would not fit the register budget of a 1024-thread group
-    asm volatile("" ::: "memory");
     int ex = 0;
     if constexpr (Ops::scaled) {
       float mx = 0.f;
@@ -179,19 +188,23 @@ __device__ __noinline__ void run_stage_mat(const int n, const int C, const int l
     const int base = (j - k) * R + k;
 #pragma unroll
     for (int h = 0; h < KT; ++h) {
-      mat_f4 acc[2];
-#pragma unroll
-      for (int comp = 0; comp < 2; ++comp) {
-        if constexpr (KT > 1) asm volatile("" ::: "memory");  // one (h, comp) block of operands live at a time
-        mat_f4 z = {0.f, 0.f, 0.f, 0.f};
+      mat_f4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+      if constexpr (KT == 1) {
+        mat_dot2<Ops>(a1[0], a1[1], b[0], acc[0], acc[1]);
+      } else {
+        // R = 32: the operands of one output tile at a time are read (L1
+        // hits) -- all four blocks live would not fit the register budget
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int t = 0; t < KT; ++t) {
-          V at[T];
+          V ar[T], ai[T];
 #pragma unroll
-          for (int tt = 0; tt < T; ++tt) at[tt] = A[(((h * KT + t) * 2 + comp) * T + tt) * 64 + lane];
-          z = mat_dot<Ops>(at, b[t], z);
+          for (int tt = 0; tt < T; ++tt) {
+            ar[tt] = A[(((h * KT + t) * 2 + 0) * T + tt) * 64 + lane];
+            ai[tt] = A[(((h * KT + t) * 2 + 1) * T + tt) * 64 + lane];
+          }
+          mat_dot2<Ops>(ar, ai, b[t], acc[0], acc[1]);
         }
-        acc[comp] = z;
       }
       if (beta < nb) {
 #pragma unroll
@@ -200,7 +213,7 @@ __device__ __noinline__ void run_stage_mat(const int n, const int C, const int l
           if (m < R) {
             const float re = Ops::scaled ? ldexpf(acc[0][i], ex) : acc[0][i];
             const float im = Ops::scaled ? ldexpf(acc[1][i], ex) : acc[1][i];
-            lds[lpad((base + m * ns) * C + c)] = mk<float>(re, im);
+            stv(lds, lpad((base + m * ns) * C + c), mk<float>(re, im));
           }
         }
       }
