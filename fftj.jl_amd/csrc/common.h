@@ -54,7 +54,8 @@ struct PassDesc {
   Geometry out;
   // multi-pass twiddle at the store: output k of transform b is multiplied
   // by w_P^e, e = q*(tw_acoef*((b / Bi) % B2) + tw_bcoef*k), q = (b % Bi) / tw_qdiv,
-  // P = tw_period (0 = none); evaluated in FP64 from exact integers
+  // P = tw_period (0 = none); the exponent is reduced exactly in integers,
+  // the value read from the tables below
   int64_t tw_period;
   int64_t tw_qdiv;
   int64_t tw_acoef;
@@ -68,6 +69,21 @@ struct PassDesc {
   // base(g*C) + c*cdelta (the planner checks that this holds)
   int64_t in_cdelta;
   int64_t out_cdelta;
+  // batch < 2^31: the batch split in 32-bit arithmetic, x / D = (x * m) >> s
+  // (FastDiv, passcommon.h) for D = Bi, B2
+  int32_t fast_batch;
+  uint32_t bi_m, b2_m;
+  int32_t bi_s, b2_s;
+  // multi-pass twiddle start values from two FP64 tables (double2):
+  // w_P^e = ptw[e >> ptw_sh] * ptw[ptw_lo + (e & (2^ptw_sh - 1))]
+  int32_t ptw_sh;
+  int64_t ptw_lo;
+  const void* ptw;
+  uint32_t qd_m;     // FastDiv by tw_qdiv
+  int32_t qd_s;
+  int64_t tw_ar;     // tw_acoef mod P
+  int64_t tw_br;     // tw_bcoef mod P
+  double tw_invp;    // 1 / P
 };
 
 // LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)

@@ -89,9 +89,9 @@ k_pass(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<T>* t
       V2<T>* yb = y + ix.base(d.out);
       if (d.tw_period) {
         PostTwiddle pt;
-        pt.init<DIR>(d, ix, jt, jstep);
+        pt.init(d, ix, jt, jstep);
         for (int k = jt; k < n; k += jstep) {
-          V2<T> v = cmul<T>(lds[lpad(k * C + c)], mk<T>(static_cast<T>(pt.w.x), static_cast<T>(pt.w.y)));
+          V2<T> v = cmul<T>(lds[lpad(k * C + c)], pt.value<T>());
           if (scaled) v = cscale<T>(v, scale);
           yb[k * ost] = v;
           pt.advance();

@@ -246,7 +246,7 @@ struct Step {
   int64_t Bi = 1;
   Geometry ga{1, 0, 0}, gb{1, 0, 0};
   // device resources
-  DevBufPtr tw, desc, aux, mat_tab;
+  DevBufPtr tw, desc, aux, mat_tab, ptw;
   std::shared_ptr<struct RaggedData> rg;  // ST_RAGGED
 };
 
@@ -265,6 +265,15 @@ struct Layout {
 
 static void need_scratch(Layout& L, int role, int64_t elems) {
   if (role >= RS0) L.scratch_elems[role - RS0] = std::max(L.scratch_elems[role - RS0], elems);
+}
+
+// x / D = (x * m) >> s for 0 <= x < 2^31, 1 <= D < 2^31 (fdiv in passcommon.h)
+static void fastdiv_params(int64_t D, uint32_t* m, int32_t* s) {
+  int l = 0;
+  while ((int64_t(1) << l) < D) ++l;
+  *s = 31 + l;
+  const unsigned __int128 p = static_cast<unsigned __int128>(1) << *s;
+  *m = static_cast<uint32_t>((p + static_cast<unsigned __int128>(D) - 1) / static_cast<unsigned __int128>(D));
 }
 
 static uint32_t magic_for(int ns) {
@@ -293,6 +302,23 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   d.tw_qdiv = tw.qdiv;
   d.tw_acoef = tw.acoef;
   d.tw_bcoef = tw.bcoef;
+  const int64_t lim = int64_t(1) << 31;
+  d.fast_batch = (batch < lim && Bi < lim && B2 < lim && tw.qdiv >= 1 && tw.qdiv < lim) ? 1 : 0;
+  if (d.fast_batch) {
+    fastdiv_params(Bi, &d.bi_m, &d.bi_s);
+    fastdiv_params(B2, &d.b2_m, &d.b2_s);
+    fastdiv_params(tw.qdiv, &d.qd_m, &d.qd_s);
+  }
+  if (tw.period) {
+    const int64_t P = tw.period;
+    d.tw_ar = tw.acoef % P;
+    d.tw_br = tw.bcoef % P;
+    d.tw_invp = 1.0 / static_cast<double>(P);
+    int bits = 0;
+    while ((int64_t(1) << bits) < P) ++bits;
+    d.ptw_sh = (bits + 1) / 2;
+    d.ptw_lo = ((P - 1) >> d.ptw_sh) + 1;  // entries of the high table
+  }
   d.in = in;
   d.out = out;
   d.scale = scale;
@@ -1182,6 +1208,17 @@ static void materialize(fftjh_plan_s& P) {
                            : s.mat   ? pass_mat_resident_groups(s.mat, s.dir, s.d)
                            : s.pow2  ? pass_pow2_resident_groups(dt, s.dir, s.d)
                                      : pass_resident_groups(dt, s.dir, s.d);
+      if (s.d.tw_period) {
+        // multi-pass twiddle tables (FP64): w_P^(i*2^sh) for i < ptw_lo, then w_P^i for i < 2^sh
+        const int64_t P = s.d.tw_period, hi = s.d.ptw_lo, lo = int64_t(1) << s.d.ptw_sh;
+        std::ostringstream pk;
+        pk << "ptw:" << dev << ":" << dir << ":" << P;  // FP64 for every plan precision
+        s.ptw = cached(pk.str(), (hi + lo) * 16, [&](void* p) {
+          hip_check(launch_tw_pow(FFTJH_F64, p, hi, lo, P, dir, 0), "twiddle kernel");
+          hip_check(launch_tw_pow(FFTJH_F64, static_cast<char*>(p) + hi * 16, lo, 1, P, dir, 0), "twiddle kernel");
+        });
+        s.d.ptw = s.ptw->p;
+      }
       if (s.mat) {
         // split DFT-matrix operands of this mode and direction (FP64-generated)
         std::ostringstream mk;

@@ -147,12 +147,20 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
   const GroupWalk walk(d);
 
   V2<T> v[kE], pf[kE];
+  const int64_t istep = static_cast<int64_t>(J) * ist, ostep = static_cast<int64_t>(J) * ost;
   auto load = [&](int g, V2<T> (&dst)[kE]) {
     const int64_t b = static_cast<int64_t>(g) * C + c;
-    const bool valid = b < d.batch;
-    const V2<T>* xb = x + (BatchIndex(d, b).base(d.in) + jt * ist);
+    if (b < d.batch) {
+      const V2<T>* p = x + (BatchIndex(d, b).base(d.in) + jt * ist);
 #pragma unroll
-    for (int u = 0; u < kE; ++u) dst[u] = valid ? xb[u * J * ist] : mk<T>(T(0), T(0));
+      for (int u = 0; u < kE; ++u) {
+        dst[u] = *p;
+        p += istep;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kE; ++u) dst[u] = mk<T>(T(0), T(0));
+    }
   };
 
   // Row-major staging for contiguous rows when C columns share a workgroup:
@@ -218,7 +226,7 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
       const BatchIndex ix(d, b);
       // the multi-pass twiddle's base and step, formed while v is dead
       PostTwiddle pt;
-      if (IO != 2 && d.tw_period) pt.init<DIR>(d, ix, jt, J);
+      if (IO != 2 && d.tw_period) pt.init(d, ix, jt, J);
       stage_gather<T, N, R>(v, lds, C, c, jt, J);
       __syncthreads();
       stage_twiddle<T, R, NS>(v, tw + OFF, jt, J);
@@ -235,14 +243,18 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
         if (d.tw_period) {
 #pragma unroll
           for (int u = 0; u < kE; ++u) {
-            V2<T> val = cmul<T>(v[u], mk<T>(static_cast<T>(pt.w.x), static_cast<T>(pt.w.y)));
+            V2<T> val = cmul<T>(v[u], pt.value<T>());
             if (scaled) val = cscale<T>(val, scale);
-            yb[u * J * ost] = val;
+            *yb = val;
+            yb += ostep;
             pt.advance();
           }
         } else {
 #pragma unroll
-          for (int u = 0; u < kE; ++u) yb[u * J * ost] = scaled ? cscale<T>(v[u], scale) : v[u];
+          for (int u = 0; u < kE; ++u) {
+            *yb = scaled ? cscale<T>(v[u], scale) : v[u];
+            yb += ostep;
+          }
         }
        }
       }
