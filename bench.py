@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the split-precision and learned-planner legs")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="measure the device's streaming copy rate at the 2^24 footprint and exit")
     ap.add_argument("--split-only", action="store_true",
                     help="run only the split-precision leg (one JSON line; for kernel traces)")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -66,6 +68,32 @@ def pmc_probe(args):
         y.copy_(x)
     torch.cuda.synchronize()
     print("pmc probe done", flush=True)
+
+
+def copy_probe():
+    """Streaming ceiling on this device for the 2^24 footprint: torch copies
+    of 128 MiB among three rotating buffers (as the three passes of the
+    2^24 plan touch three arrays), and between two fixed buffers."""
+    import torch
+    n = 1 << 24
+    b = [torch.zeros(n, dtype=torch.complex64, device="cuda:0") for _ in range(3)]
+    out = {}
+    for name, pairs in (("rotating_3x128MiB", [(0, 1), (1, 2), (2, 0)]), ("fixed_2x128MiB", [(0, 1)])):
+        for i in range(6):
+            s, d = pairs[i % len(pairs)]
+            b[d].copy_(b[s])
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        it = 60
+        e0.record()
+        for i in range(it):
+            s, d = pairs[i % len(pairs)]
+            b[d].copy_(b[s])
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / it
+        out[name] = {"us_per_copy": round(us, 2), "gbs": round(16.0 * n / (us * 1e-6) / 1e9, 1)}
+    print(json.dumps({"copy_probe": out}), flush=True)
 
 
 def cpu_baseline(n, batch, target_s):
@@ -212,6 +240,9 @@ def main():
     args = parse()
     if args.pmc_probe:
         pmc_probe(args)
+        return
+    if args.copy_probe:
+        copy_probe()
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "

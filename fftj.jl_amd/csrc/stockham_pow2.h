@@ -10,13 +10,13 @@
 //   stages    LDS -> registers -> LDS (two barriers each);
 //   last      LDS -> registers -> HBM, with scale and four-step twiddle.
 // Two LDS round trips for a 4096-point transform instead of five, no calls,
-// everything unrolled at compile time.  The kernel is persistent: while the
-// later stages of group g run, the loads of the next group are in flight.
+// everything unrolled at compile time.  The kernel is persistent: each
+// workgroup walks several groups (FFTJH_P2_PREFETCH=1 also loads the next
+// group's registers during the middle stages; measured no faster, off).
 //
-// Four-step twiddle w_N^{j1*k} for k = jt + u*J: w = A * S^u with
-// A = w_N^{j1*jt} and S = w_N^{j1*J} from FP64 sincospi of exact integer
-// angles, the powers formed in FP64 and rounded once, so no table loads
-// sit in the store phase.
+// Four-step twiddle w_N^{j1*k} for k = jt + u*J: w = A * S^u with A and S
+// each a product of two FP64 table entries at exact integer exponents, the
+// powers formed in FP64 and rounded once (PostTwiddle, passcommon.h).
 #include "butterflies.h"
 #include "common.h"
 #include "launch.h"
