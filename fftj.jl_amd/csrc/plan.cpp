@@ -623,6 +623,36 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
     std::vector<int64_t> S(P + 1, 1), M(P + 2, 1);
     for (int d = 2; d <= P; ++d) S[d] = S[d - 1] * F[d - 2];
     for (int d = P - 1; d >= 0; --d) M[d] = M[d + 1] * F[d];  // M[d] = prod F[d..P-1] (0-based)
+    // (measured on 2^24: the in-place middle pass runs from the Infinity
+    // Cache, 70 -> 55 us; FFTJH_MID_INPLACE=0 restores the rotating layout)
+    static const bool mid_inplace = [] {
+      const char* e = std::getenv("FFTJH_MID_INPLACE");
+      return !e || std::atoi(e) != 0;
+    }();
+    if (P == 3 && mid_inplace) {
+      // Three passes through one intermediate z: x -> z (digit j3 -> k3 in
+      // place), z -> z (j2 -> k2 in place, twiddle w_n^{j1 (k3 + N3 k2)}),
+      // z -> y (rows j1 at j1 + N1 (k2 + N2 k3), written to natural order
+      // k3 + N3 k2 + N2 N3 k1).  The middle pass works inside one array.
+      const int64_t N1 = F[0], N2 = F[1], N3 = F[2];
+      PassTw t3;
+      t3.period = N2 * N3;
+      t3.qdiv = N1;
+      t3.acoef = 1;
+      t3.bcoef = 1;
+      add_pass(L, dir, static_cast<int>(N3), batch * (n / N3), N1 * N2, geo(N1 * N2, 1, n, n), geo(N1 * N2, 1, n, n),
+               1.0, src, z, t3, 1);
+      PassTw t2;
+      t2.period = n;
+      t2.qdiv = 1;
+      t2.acoef = 1;
+      t2.bcoef = N3;
+      add_pass(L, dir, static_cast<int>(N2), batch * (n / N2), N1, geo(N1, 1, n, N1 * N2), geo(N1, 1, n, N1 * N2), 1.0,
+               z, z, t2, N3);
+      add_pass(L, dir, static_cast<int>(N1), batch * (n / N1), N3, geo(1, N1 * N2, n, N1), geo(N2 * N3, 1, n, N3),
+               scale, z, dst, PassTw(), N2);
+      return;
+    }
     int cur = src;
     for (int d = P; d >= 1; --d) {
       const int64_t Nd = F[d - 1];

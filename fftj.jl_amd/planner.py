@@ -296,6 +296,7 @@ class CostModel:
         self.nparams = int(lib().fftjh_mlp_param_count(h))
         self.mu = np.zeros(in_features, np.float32)
         self.sd = np.ones(in_features, np.float32)
+        self.log_range = (-np.inf, np.inf)  # training targets' range (log us)
         self.set_params(init_params(in_features, hidden, seed))
 
     def __del__(self):
@@ -367,6 +368,8 @@ class CostModel:
         X = np.asarray(X, np.float32)
         self.mu = X.mean(0).astype(np.float32)
         self.sd = (X.std(0) + 1e-3).astype(np.float32)
+        tt = np.asarray(t, np.float64)
+        self.log_range = (float(tt.min()), float(tt.max()))
         Xd = torch.from_numpy(self.normalize(X)).to("cuda")
         td = torch.as_tensor(np.asarray(t, np.float32)).to("cuda")
         g = torch.Generator(device="cpu").manual_seed(seed)
@@ -386,7 +389,8 @@ class CostModel:
 
     def save(self, path):
         np.savez(path, params=self.get_params(), mu=self.mu, sd=self.sd,
-                 shape=np.array([self.in_features, self.hidden], np.int64))
+                 shape=np.array([self.in_features, self.hidden], np.int64),
+                 log_range=np.array(self.log_range, np.float64))
 
     @classmethod
     def load(cls, path):
@@ -394,6 +398,8 @@ class CostModel:
             m = cls(int(z["shape"][0]), int(z["shape"][1]))
             m.set_params(z["params"])
             m.mu, m.sd = z["mu"].astype(np.float32), z["sd"].astype(np.float32)
+            if "log_range" in z.files:
+                m.log_range = tuple(float(v) for v in z["log_range"])
         return m
 
 
@@ -463,17 +469,32 @@ class LearnedPlanner:
         cand, us = self.rank(n, batch, sign)[0]
         return make_plan(n, batch, cand, sign=sign), cand, us
 
+    # resampled lengths: the m-point transform's cost times this factor for
+    # the deconvolution and interpolation edges (the model saw no resampled
+    # schedules: the training lengths are 13-smooth)
+    RESAMPLE_FACTOR = 1.5
+
     def per_transform_us(self, sizes):
         """Predicted microseconds per transform for each length (default
         plans at the training batch) -- per-item costs for the ragged
-        planner's workgroup balance."""
+        planner's workgroup balance.  A non-smooth length is priced as the
+        smooth m its plan resamples onto; predictions are clamped to the
+        range of the training timings."""
         sizes = [int(s) for s in sizes]
-        feats = []
+        core, factor = [], []
         for n in sizes:
-            b = batch_for(n)
-            feats.append(features(schedule(n, b, {}), n, b))
-        pred = self.model.predict(np.array(feats))
-        return np.exp(pred.astype(np.float64)) / np.array([batch_for(n) for n in sizes], np.float64)
+            m = n
+            if not smooth13(n):
+                d = schedule(n, 1, {})
+                m = int(d.split(" m=")[1].split()[0]) if " m=" in d else n
+            core.append(m)
+            factor.append(self.RESAMPLE_FACTOR if m != n else 1.0)
+        feats = [features(schedule(m, batch_for(m), {}), m, batch_for(m)) for m in core]
+        pred = self.model.predict(np.array(feats)).astype(np.float64)
+        lo, hi = self.model.log_range
+        pred = np.clip(pred, lo, hi)
+        per = np.exp(pred) / np.array([batch_for(m) for m in core], np.float64)
+        return per * np.array(factor)
 
 
 _LEARNED = None

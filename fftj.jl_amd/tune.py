@@ -31,9 +31,32 @@ def ragged_candidates(resident=1024):
         yield {"radix_pref": rp, "tile_max": tm, "groups": g}
 
 
+def model_costs(sizes, planner=None, tile_max=4096):
+    """Per-transform costs for the ragged planner's workgroup balance from the
+    learned cost model: predicted microseconds per transform of each length
+    (batched plans of that length, one prediction per distinct length), plus
+    the ragged kernel's fixed cost per item, which batched timings do not
+    show: the planner's own per-item constants (element units: 2000 for a
+    one-tile item, 6000 two-level, 3000 / 8000 resampled) converted to
+    microseconds by the model's median cost per n*log2(n)."""
+    from . import planner as _pl
+    lp = planner if planner is not None else _pl.default_learned()
+    sizes = np.asarray(sizes, dtype=np.int64)
+    uniq, inv = np.unique(sizes, return_inverse=True)
+    us = lp.per_transform_us(uniq)
+    smooth = np.array([_pl.smooth13(int(n)) for n in uniq])
+    nl = uniq * np.log2(np.maximum(uniq, 2))
+    scale = float(np.median(us[smooth] / nl[smooth])) if smooth.any() else float(np.median(us / nl))
+    fixed = np.where(smooth, np.where(uniq <= tile_max, 2000.0, 6000.0), np.where(uniq <= tile_max // 2, 3000.0, 8000.0))
+    return (us + scale * fixed)[inv]
+
+
 def autotune_ragged(T, sizes, offsets=None, *, sign=-1, normalized=True, x=None, y=None, costs=None,
-                    candidates=None, iters=5, budget=None):
-    """Returns (best_plan, best_config, results); results lists (config, ms)."""
+                    candidates=None, iters=5, budget=None, seed_model=False):
+    """Returns (best_plan, best_config, results); results lists (config, ms).
+    seed_model=True adds, for every configuration, a candidate whose
+    workgroup balance uses the learned cost model's per-transform costs
+    (config key "costs": "model"; the others use the planner's own)."""
     code = _code(T)
     sizes = np.asarray(sizes, dtype=np.int64)
     extent = int(sizes.sum()) if offsets is None else int((np.asarray(offsets) + sizes).max())
@@ -45,10 +68,15 @@ def autotune_ragged(T, sizes, offsets=None, *, sign=-1, normalized=True, x=None,
     cands = list(candidates if candidates is not None else ragged_candidates())
     if budget is not None:
         cands = cands[:budget]
+    mcosts = model_costs(sizes) if seed_model else None
+    if seed_model:
+        cands = [dict(c, costs="model") for c in cands] + cands
     results = []
     best = (None, None, float("inf"))
     for cfg in cands:
-        plan = RaggedPlan(code, sizes, offsets, sign, normalized, tuning=make_tuning(**cfg), costs=costs)
+        knobs = {k: v for k, v in cfg.items() if k != "costs"}
+        use = mcosts if cfg.get("costs") == "model" else costs
+        plan = RaggedPlan(code, sizes, offsets, sign, normalized, tuning=make_tuning(**knobs), costs=use)
         ms = time_plan(plan, y, x, iters=iters)
         results.append((cfg, ms))
         if ms < best[2]:

@@ -85,3 +85,25 @@ def test_ragged_benchmark_batch():
     loop(y2, xd)
     e = int(offs[299] + sizes[299])
     assert rel_l2(to_host(y2)[:e], y[:e]) <= tol_for("f32", 65536)
+
+
+def test_ragged_autotune_seeded_by_cost_model():
+    """The learned cost model's per-transform costs seed the autotuner's
+    workgroup balance; whatever it picks computes the same transforms."""
+    sizes = F.ragged.benchmark_sizes()[:1500]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    mc = F.tune.model_costs(sizes)
+    assert mc.shape == sizes.shape and np.all(np.isfinite(mc)) and np.all(mc > 0)
+    big, small = int(np.argmax(sizes)), int(np.argmin(sizes))
+    assert mc[big] > mc[small]
+    x = F.rng.uniform_complex(4, int(sizes.sum())).astype(np.complex64)
+    xd = to_dev(x)
+    yd = torch.empty_like(xd)
+    plan, cfg, results = F.tune.autotune_ragged(torch.float32, sizes, x=xd, y=yd, iters=2, budget=2,
+                                                 seed_model=True)
+    assert len(results) == 4 and sum(1 for c, _ in results if c.get("costs") == "model") == 2
+    plan(yd, xd)
+    y = to_host(yd)
+    for i in list(range(0, len(sizes), 97)) + [big]:
+        n, o = int(sizes[i]), int(offs[i])
+        assert rel_l2(y[o:o + n], O.fft(x[o:o + n].astype(np.complex128))) <= tol_for("f32", n), (n, cfg)

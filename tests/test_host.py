@@ -66,9 +66,11 @@ def test_schedule_config2():
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [4096], batch=16384))
     assert len(s) == 1 and "batch=16384" in s[0]
     # (b) 2^24: three 256-point passes (nested four-step), twiddles fused
-    # into the first two, natural-order output from the third
+    # into the first two, natural-order output from the third; the middle
+    # pass works in place on the one intermediate
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1 << 24]))
     assert len(s) == 3 and all("n=256" in p and "kernel=pow2" in p for p in s)
+    assert s[0].startswith("pass x->s0") and s[1].startswith("pass s0->s0") and s[2].startswith("pass s0->y")
     assert "in_stride=65536" in s[0] and "twiddle=w_65536" in s[0]
     assert "in_stride=256" in s[1] and "twiddle=w_16777216" in s[1]
     assert "in_stride=1 " in s[2] and "out_stride=65536" in s[2] and "twiddle" not in s[2]
