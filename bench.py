@@ -211,6 +211,53 @@ def split_precision_leg(F):
     return out
 
 
+def configs_leg(F):
+    """Every configuration of the parity matrix, forward, timed once each
+    (HIP events, inputs resident): ms, GFLOP/s (5 N log2 N per complex
+    transform, 2.5 N log2 N per real one) and GB/s on the algorithmic bytes
+    (one read + one write of the transform's input and output)."""
+    import numpy as np
+    import torch
+
+    cases = [("c2c_f64_1024", "c2c", torch.float64, (1024,), 1), ("c2c_f32_4096x16384", "c2c", torch.float32, (4096,), 16384),
+             ("c2c_f32_2pow24", "c2c", torch.float32, (1 << 24,), 1), ("c2c_f32_1e6x8", "c2c", torch.float32, (10 ** 6,), 8),
+             ("c2c_f32_65537x4", "c2c", torch.float32, (65537,), 4), ("c2c_f32_1000003x4", "c2c", torch.float32, (1000003,), 4),
+             ("r2c_f32_2pow22x4", "r2c", torch.float32, (1 << 22,), 4), ("c2r_f32_2pow22x4", "c2r", torch.float32, (1 << 22,), 4),
+             ("c2c_f32_256cubed", "c2c", torch.float32, (256, 256, 256), 1)]
+    out = []
+    for name, kind, T, dims, batch in cases:
+        n = int(np.prod(dims))
+        cdt = torch.complex64 if T == torch.float32 else torch.complex128
+        es = 8 if T == torch.float32 else 16
+        if kind == "c2c":
+            x = torch.randn(batch * n, dtype=cdt, device="cuda:0")
+            if len(dims) > 1:
+                x = x.reshape(dims)
+                p = F.plan_fft(x)
+            else:
+                p = F.plan_fft(T, n, batch=batch)
+            y = torch.empty_like(x)
+            bytes_ = 2 * es * n * batch
+            flops = 5.0 * n * math.log2(n) * batch
+        elif kind == "r2c":
+            x = torch.randn(batch * n, dtype=T, device="cuda:0")
+            p = F.plan_rfft(T, n, batch=batch)
+            y = torch.empty(batch * (n // 2 + 1), dtype=cdt, device="cuda:0")
+            bytes_ = (es // 2) * n * batch + es * (n // 2 + 1) * batch
+            flops = 2.5 * n * math.log2(n) * batch
+        else:
+            x = torch.randn(batch * (n // 2 + 1), dtype=cdt, device="cuda:0")
+            p = F.plan_irfft(T, n, batch=batch)
+            y = torch.empty(batch * n, dtype=T, device="cuda:0")
+            bytes_ = (es // 2) * n * batch + es * (n // 2 + 1) * batch
+            flops = 2.5 * n * math.log2(n) * batch
+        ms = F.tune.time_plan(p, y, x, iters=10)
+        out.append({"config": name, "ms": round(ms, 4), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),
+                    "gbs": round(bytes_ / (ms * 1e-3) / 1e9, 1), "launches": p.describe().count("\n  ")})
+        del p, x, y
+    return out
+
+
 def planner_leg(F):
     """Held-out lengths of the learned planner's training set: the default
     schedule vs the one the trained cost model picks, timed now."""
@@ -334,6 +381,7 @@ def main():
     if not args.no_ragged:
         result["ragged"] = ragged_leg(F)
     if not args.no_extras:
+        result["configs"] = configs_leg(F)
         result["split_precision"] = split_precision_leg(F)
         result["learned_planner"] = planner_leg(F)
     if not args.no_cpu:
