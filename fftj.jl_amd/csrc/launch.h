@@ -53,8 +53,8 @@ hipError_t launch_rs_corr(int dtype, void* corr, int64_t n, int64_t m, int w, do
 // writes contiguous rows of m; interp reads rows of m and writes geometry gy
 hipError_t launch_rs_deconv(int dtype, const void* x, void* g, const void* corr, int64_t n, int64_t m,
                             int64_t batch, int64_t Bi, Geometry gx, hipStream_t st);
-hipError_t launch_rs_interp(int dtype, const void* G, void* y, int64_t n, int64_t m, int w, double beta,
-                            int64_t batch, int64_t Bi, Geometry gy, hipStream_t st);
+hipError_t launch_rs_interp(int dtype, const void* G, void* y, const void* coef, int pieces, int degree, int64_t n,
+                            int64_t m, int w, int64_t batch, int64_t Bi, Geometry gy, hipStream_t st);
 hipError_t launch_strided_copy(int dtype, const void* src, void* dst, int64_t n, int64_t batch, int64_t Bi,
                                Geometry gs, Geometry gd, hipStream_t st);
 

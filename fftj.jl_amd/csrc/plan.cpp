@@ -241,6 +241,7 @@ struct Step {
   // other kernels
   int64_t n = 0, m = 0, h = 0, batch = 0, s1 = 0, s2 = 0, keep = 0;
   int w = 0;
+  int pieces = 0, degree = 0;  // ST_RS_INTERP: piecewise-polynomial weights
   double beta = 0.0, pre = 1.0, scale = 1.0;
   // strided sides of rs_deconv (ga), rs_interp (gb), strided_copy (ga -> gb)
   int64_t Bi = 1;
@@ -553,6 +554,79 @@ static void gauss_legendre(int nq, std::vector<double>& z, std::vector<double>& 
     z[i - 1] = x;
     wq[i - 1] = 2.0 / ((1.0 - x * x) * dp * dp);
   }
+}
+
+// Interpolation weights as piecewise polynomials in the offset delta = r/n
+// of an output from its grid point (resample.jl:59-170 chooses the same
+// representation for long transforms): delta's range [0, 1) is cut into
+// `pieces`; on piece p, tap i's weight psi((delta + h - 1 - i)/h) as a
+// function of t = 2(delta*pieces - p) - 1 in [-1, 1] is interpolated at the
+// degree+1 Chebyshev points (long double), kept as Chebyshev coefficients
+// (evaluated by Clenshaw's recurrence on the device), the degree raised until
+// the error on a dense grid meets the precision's target or stops halving.
+// coef[((p*w + i)*(degree + 1)) + k].
+struct RsWeights {
+  int degree = 0;
+  std::vector<long double> coef;
+};
+
+static RsWeights fit_rs_weights(int dtype, int w, double beta, int pieces) {
+  const int h = w / 2;
+  const long double target = dtype == FFTJH_F32 ? 2e-9L : 2e-16L;
+  const int dmax = dtype == FFTJH_F32 ? 16 : 32;
+  auto psi = [&](long double z) -> long double {
+    const long double a = std::fabs(z);
+    return a >= 1.0L ? 0.0L : std::exp(static_cast<long double>(beta) * (std::sqrt(1.0L - a * a) - 1.0L));
+  };
+  const long double pi = 3.141592653589793238462643383279502884L;
+  RsWeights best;
+  long double best_err = 1e30L;
+  for (int d = 2; d <= dmax; ++d) {
+    const int N = d + 1;
+    std::vector<long double> coef(static_cast<size_t>(pieces) * w * N);
+    long double err = 0;
+    for (int p = 0; p < pieces; ++p)
+      for (int i = 0; i < w; ++i) {
+        auto f = [&](long double t) {
+          const long double delta = (p + (t + 1.0L) / 2.0L) / pieces;
+          return psi((delta + (h - 1 - i)) / h);
+        };
+        long double* a = &coef[(static_cast<size_t>(p) * w + i) * N];
+        std::vector<long double> fj(N);
+        for (int j = 0; j < N; ++j) fj[j] = f(std::cos(pi * (j + 0.5L) / N));
+        for (int k = 0; k < N; ++k) {
+          long double sum = 0;
+          for (int j = 0; j < N; ++j) sum += fj[j] * std::cos(pi * k * (j + 0.5L) / N);
+          a[k] = sum * 2.0L / N;
+        }
+        a[0] /= 2.0L;
+        // error on a dense grid, with the coefficients rounded as stored
+        for (int g = 0; g <= 4 * N + 4; ++g) {
+          const long double t = -1.0L + 2.0L * g / (4 * N + 4);
+          long double b1 = 0, b2 = 0;
+          for (int k = d; k >= 1; --k) {
+            const long double ak = dtype == FFTJH_F32 ? static_cast<long double>(static_cast<float>(a[k]))
+                                                      : static_cast<long double>(static_cast<double>(a[k]));
+            const long double b0 = ak + 2.0L * t * b1 - b2;
+            b2 = b1;
+            b1 = b0;
+          }
+          const long double a0 = dtype == FFTJH_F32 ? static_cast<long double>(static_cast<float>(a[0]))
+                                                    : static_cast<long double>(static_cast<double>(a[0]));
+          err = std::max(err, std::fabs(a0 + t * b1 - b2 - f(t)));
+        }
+      }
+    if (err < best_err) {
+      const bool halving = err <= best_err / 2;
+      best.degree = d;
+      best.coef = coef;
+      best_err = err;
+      if (err <= target || !halving) break;
+    } else {
+      break;
+    }
+  }
+  return best;
 }
 
 static bool is_rows(const Geometry& g, int64_t n, int64_t Bi) { return Bi == 1 && g.stride == 1 && g.s_out == n; }
@@ -1316,6 +1390,22 @@ static void materialize(fftjh_plan_s& P) {
       const int dir = s.dir;
       s.aux = cached(key_of("real", {dev, double(dt), double(dir), double(n)}), cnt * es,
                      [&](void* p) { hip_check(launch_tw_pow(dt, p, cnt, 1, n, dir, 0), "twiddle kernel"); });
+    } else if (s.kind == ST_RS_INTERP) {
+      const int pieces = dt == FFTJH_F32 ? 64 : 128;
+      const RsWeights wf = fit_rs_weights(dt, s.w, s.beta, pieces);
+      s.pieces = pieces;
+      s.degree = wf.degree;
+      const size_t cnt = wf.coef.size();
+      s.aux = cached(key_of("rs_wpoly", {dev, double(dt), double(s.w), s.beta, double(pieces)}), cnt * (es / 2),
+                     [&](void* p) {
+                       if (dt == FFTJH_F32) {
+                         std::vector<float> c(wf.coef.begin(), wf.coef.end());
+                         hip_check(hipMemcpy(p, c.data(), cnt * sizeof(float), hipMemcpyHostToDevice), "weights upload");
+                       } else {
+                         std::vector<double> c(wf.coef.begin(), wf.coef.end());
+                         hip_check(hipMemcpy(p, c.data(), cnt * sizeof(double), hipMemcpyHostToDevice), "weights upload");
+                       }
+                     });
     } else if (s.kind == ST_RS_DECONV) {
       const int64_t n = s.n, m = s.m;
       const int w = s.w;
@@ -1368,7 +1458,8 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
     case ST_RS_DECONV:
       return launch_rs_deconv(dt, ptr[s.src], ptr[s.dst], s.aux->p, s.n, s.m, s.batch, s.Bi, s.ga, st);
     case ST_RS_INTERP:
-      return launch_rs_interp(dt, ptr[s.src], ptr[s.dst], s.n, s.m, s.w, s.beta, s.batch, s.Bi, s.gb, st);
+      return launch_rs_interp(dt, ptr[s.src], ptr[s.dst], s.aux->p, s.pieces, s.degree, s.n, s.m, s.w, s.batch, s.Bi,
+                              s.gb, st);
     case ST_STRIDED_COPY:
       return launch_strided_copy(dt, ptr[s.src], ptr[s.dst], s.n, s.batch, s.Bi, s.ga, s.gb, st);
     case ST_RAGGED: {

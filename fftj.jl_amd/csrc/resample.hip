@@ -10,8 +10,11 @@
 //   y_k    = sum_{i<w} psi((delta + h - 1 - i)/h) G[(q - h + 1 + i) mod m],
 //            k*m = q*n + r, delta = r/n, psi(z) = exp(beta(sqrt(1-z^2)-1))
 //            (resample.jl:238-244, 392-424)                         -> k_rs_interp
-// The interpolation weights are evaluated per tap in FP64 from the exact
-// integer remainder r = k*m mod n, instead of being read from a table.
+// The interpolation weights come from piecewise polynomials in delta fitted
+// at plan time (the planner, fit_rs_weights), evaluated in the plan's
+// precision from the exact integer remainder r = k*m mod n.
+#include <algorithm>
+
 #include "butterflies.h"
 #include "common.h"
 #include "launch.h"
@@ -71,29 +74,111 @@ __global__ void k_rs_deconv(const V2<T>* x, V2<T>* g, const T* __restrict__ corr
   }
 }
 
-// G (contiguous rows of m) -> y (transforms at geometry gy)
-template <typename T>
-__global__ void k_rs_interp(const V2<T>* G, V2<T>* y, int64_t n, int64_t m, int w, double beta,
-                            int64_t batch, int64_t Bi, Geometry gy) {
-  const int h = w / 2;
-  GRID_STRIDE(idx, batch * n) {
-    const int64_t b = idx / n, k = idx - b * n;
-    const int64_t km = k * m;
-    const int64_t q = km / n, r = km - q * n;
-    const double delta = static_cast<double>(r) / static_cast<double>(n);
-    int64_t l = q - h + 1;
-    if (l < 0) l += m;
-    const V2<T>* gb = G + b * m;
-    V2<T> acc = mk<T>(T(0), T(0));
-    for (int i = 0; i < w; ++i) {
-      const T wt = static_cast<T>(es_kernel((delta + (h - 1 - i)) / h, beta));
-      int64_t li = l + i;
-      if (li >= m) li -= m;
-      const V2<T> gv = gb[li];
-      acc.x = fma(wt, gv.x, acc.x);
-      acc.y = fma(wt, gv.y, acc.y);
+// G (contiguous rows of m) -> y (transforms at geometry gy).  A wave takes
+// a span of 64*run consecutive outputs of one transform (run <= kInterpRun,
+// chosen so small problems still fill the device), lane l the outputs
+// l + 64u: the quotient and remainder of k*m by n are formed once per lane
+// and then stepped by those of 64*m (stores stay 64-wide runs).
+// The w weights of an output come from the piecewise Chebyshev polynomials
+// the planner fitted (coef[((p*w + i)*(degree+1)) + k]), by Clenshaw's
+// recurrence in T.
+constexpr int kInterpRun = 8;
+
+// weight of tap i from its Chebyshev coefficients a[0..deg] at t
+template <typename T, int DEG>
+__device__ __forceinline__ T clenshaw(const T* a, T t, int deg) {
+  T b1 = T(0), b2 = T(0);
+  if constexpr (DEG > 0) {
+#pragma unroll
+    for (int c = DEG; c >= 1; --c) {
+      const T b0 = fma(T(2) * t, b1, a[c] - b2);
+      b2 = b1;
+      b1 = b0;
     }
-    y[geo_base(gy, Bi, b) + k * gy.stride] = acc;
+  } else {
+    for (int c = deg; c >= 1; --c) {
+      const T b0 = fma(T(2) * t, b1, a[c] - b2);
+      b2 = b1;
+      b1 = b0;
+    }
+  }
+  return fma(t, b1, a[0] - b2);
+}
+
+// W, DEG > 0: compile-time width and degree (the taps and the recurrence
+// unrolled, a piece's coefficients read as one contiguous block); 0: runtime
+// LDS_COEF: the coefficient table (pieces*W*(DEG+1) values) is staged in LDS
+template <typename T, int W, int DEG, bool LDS_COEF>
+__global__ void k_rs_interp(const V2<T>* G, V2<T>* y, const T* __restrict__ coef_g, int pieces, int degree,
+                            int64_t n, int64_t m, int w_rt, int64_t batch, int64_t Bi, Geometry gy, int run) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const T* coef = coef_g;
+  if constexpr (LDS_COEF) {
+    T* c = reinterpret_cast<T*>(smem);
+    const int cnt = pieces * W * (DEG + 1);
+    for (int e = static_cast<int>(threadIdx.x); e < cnt; e += static_cast<int>(blockDim.x)) c[e] = coef_g[e];
+    __syncthreads();
+    coef = c;
+  }
+  const int w = W > 0 ? W : w_rt;
+  const int h = w / 2;
+  const int N = (DEG > 0 ? DEG : degree) + 1;
+  const double pn = static_cast<double>(pieces) / static_cast<double>(n);
+  const int64_t span = 64 * static_cast<int64_t>(run);
+  const int64_t runs = (n + span - 1) / span;
+  const int64_t q64 = (64 * m) / n, r64 = 64 * m - q64 * n;
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int64_t wave = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) >> 6;
+  const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * blockDim.x) >> 6;
+  for (int64_t sp = wave; sp < batch * runs; sp += nwaves) {
+    const int64_t b = sp / runs;
+    int64_t k = (sp - b * runs) * span + lane;
+    if (k >= n) continue;
+    const int64_t km = k * m;
+    int64_t q = km / n, r = km - q * n;
+    const V2<T>* gb = G + b * m;
+    V2<T>* yb = y + geo_base(gy, Bi, b);
+    for (int u = 0; u < run && k < n; ++u) {
+      const double sd = static_cast<double>(r) * pn;  // delta * pieces, in [0, pieces)
+      int p = static_cast<int>(sd);
+      if (p >= pieces) p = pieces - 1;
+      const T t = static_cast<T>(2.0 * (sd - p) - 1.0);
+      const T* cp = coef + static_cast<int64_t>(p) * w * N;
+      int64_t l = q - h + 1;
+      if (l < 0) l += m;
+      V2<T> acc = mk<T>(T(0), T(0));
+      if constexpr (W > 0 && DEG > 0) {
+        T a[W * (DEG + 1)];
+#pragma unroll
+        for (int e = 0; e < W * (DEG + 1); ++e) a[e] = cp[e];
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+          const T wt = clenshaw<T, DEG>(a + i * (DEG + 1), t, DEG);
+          int64_t li = l + i;
+          if (li >= m) li -= m;
+          const V2<T> gv = gb[li];
+          acc.x = fma(wt, gv.x, acc.x);
+          acc.y = fma(wt, gv.y, acc.y);
+        }
+      } else {
+        for (int i = 0; i < w; ++i) {
+          const T wt = clenshaw<T, 0>(cp + i * N, t, N - 1);
+          int64_t li = l + i;
+          if (li >= m) li -= m;
+          const V2<T> gv = gb[li];
+          acc.x = fma(wt, gv.x, acc.x);
+          acc.y = fma(wt, gv.y, acc.y);
+        }
+      }
+      yb[k * gy.stride] = acc;
+      k += 64;
+      r += r64;
+      q += q64;
+      if (r >= n) {
+        r -= n;
+        ++q;
+      }
+    }
   }
 }
 
@@ -132,16 +217,40 @@ hipError_t launch_rs_deconv(int dtype, const void* x, void* g, const void* corr,
   return hipGetLastError();
 }
 
-hipError_t launch_rs_interp(int dtype, const void* G, void* y, int64_t n, int64_t m, int w, double beta,
-                            int64_t batch, int64_t Bi, Geometry gy, hipStream_t st) {
-  const dim3 gr = grid_1d(batch * n);
-  if (dtype == 0)
-    hipLaunchKernelGGL(k_rs_interp<float>, gr, dim3(256), 0, st, static_cast<const float2*>(G),
-                       static_cast<float2*>(y), n, m, w, beta, batch, Bi, gy);
-  else
-    hipLaunchKernelGGL(k_rs_interp<double>, gr, dim3(256), 0, st, static_cast<const double2*>(G),
-                       static_cast<double2*>(y), n, m, w, beta, batch, Bi, gy);
+template <typename T>
+static hipError_t launch_interp_t(const void* G, void* y, const void* coef, int pieces, int degree, int64_t n,
+                                  int64_t m, int w, int64_t batch, int64_t Bi, Geometry gy, hipStream_t st) {
+  // outputs per lane: up to kInterpRun, fewer when that would leave < 8192 waves
+  const int64_t outs = batch * n;
+  int run = static_cast<int>(std::min<int64_t>(kInterpRun, std::max<int64_t>(1, outs / (64 * 8192))));
+  const dim3 gr = grid_1d(batch * ((n + 64 * run - 1) / (64 * run)) * 64);
+  const auto* g = static_cast<const V2<T>*>(G);
+  auto* yy = static_cast<V2<T>*>(y);
+  const auto* c = static_cast<const T*>(coef);
+  // the planner's usual fits: FP32 w = 8 degree 3 (table staged in LDS),
+  // FP64 w = 16 degree 5
+  if (w == 8 && degree == 3) {
+    const size_t lds = static_cast<size_t>(pieces) * 8 * 4 * sizeof(T);
+    if (lds <= 48 * 1024)
+      hipLaunchKernelGGL((k_rs_interp<T, 8, 3, true>), gr, dim3(256), lds, st, g, yy, c, pieces, degree, n, m, w, batch,
+                         Bi, gy, run);
+    else
+      hipLaunchKernelGGL((k_rs_interp<T, 8, 3, false>), gr, dim3(256), 0, st, g, yy, c, pieces, degree, n, m, w, batch,
+                         Bi, gy, run);
+  } else if (w == 16 && degree == 5) {
+    hipLaunchKernelGGL((k_rs_interp<T, 16, 5, false>), gr, dim3(256), 0, st, g, yy, c, pieces, degree, n, m, w, batch,
+                       Bi, gy, run);
+  } else {
+    hipLaunchKernelGGL((k_rs_interp<T, 0, 0, false>), gr, dim3(256), 0, st, g, yy, c, pieces, degree, n, m, w, batch,
+                       Bi, gy, run);
+  }
   return hipGetLastError();
+}
+
+hipError_t launch_rs_interp(int dtype, const void* G, void* y, const void* coef, int pieces, int degree, int64_t n,
+                            int64_t m, int w, int64_t batch, int64_t Bi, Geometry gy, hipStream_t st) {
+  return dtype == 0 ? launch_interp_t<float>(G, y, coef, pieces, degree, n, m, w, batch, Bi, gy, st)
+                    : launch_interp_t<double>(G, y, coef, pieces, degree, n, m, w, batch, Bi, gy, st);
 }
 
 hipError_t launch_strided_copy(int dtype, const void* src, void* dst, int64_t n, int64_t batch, int64_t Bi,
