@@ -223,7 +223,9 @@ def configs_leg(F):
              ("c2c_f32_2pow24", "c2c", torch.float32, (1 << 24,), 1), ("c2c_f32_1e6x8", "c2c", torch.float32, (10 ** 6,), 8),
              ("c2c_f32_65537x4", "c2c", torch.float32, (65537,), 4), ("c2c_f32_1000003x4", "c2c", torch.float32, (1000003,), 4),
              ("r2c_f32_2pow22x4", "r2c", torch.float32, (1 << 22,), 4), ("c2r_f32_2pow22x4", "c2r", torch.float32, (1 << 22,), 4),
-             ("c2c_f32_256cubed", "c2c", torch.float32, (256, 256, 256), 1)]
+             ("c2c_f32_256cubed", "c2c", torch.float32, (256, 256, 256), 1),
+             ("r2c_f32_256cubed", "r2c", torch.float32, (256, 256, 256), 1),
+             ("c2r_f32_256cubed", "c2r", torch.float32, (256, 256, 256), 1)]
     out = []
     for name, kind, T, dims, batch in cases:
         n = int(np.prod(dims))
@@ -240,16 +242,31 @@ def configs_leg(F):
             bytes_ = 2 * es * n * batch
             flops = 5.0 * n * math.log2(n) * batch
         elif kind == "r2c":
+            # N-d: the real array of torch shape dims, halved along the last axis
+            h = dims[-1] // 2 + 1
+            hn = n // dims[-1] * h
             x = torch.randn(batch * n, dtype=T, device="cuda:0")
-            p = F.plan_rfft(T, n, batch=batch)
-            y = torch.empty(batch * (n // 2 + 1), dtype=cdt, device="cuda:0")
-            bytes_ = (es // 2) * n * batch + es * (n // 2 + 1) * batch
+            if len(dims) > 1:
+                x = x.reshape(dims)
+                p = F.plan_rfft(x)
+                y = torch.empty(dims[:-1] + (h,), dtype=cdt, device="cuda:0")
+            else:
+                p = F.plan_rfft(T, n, batch=batch)
+                y = torch.empty(batch * hn, dtype=cdt, device="cuda:0")
+            bytes_ = (es // 2) * n * batch + es * hn * batch
             flops = 2.5 * n * math.log2(n) * batch
         else:
-            x = torch.randn(batch * (n // 2 + 1), dtype=cdt, device="cuda:0")
-            p = F.plan_irfft(T, n, batch=batch)
-            y = torch.empty(batch * n, dtype=T, device="cuda:0")
-            bytes_ = (es // 2) * n * batch + es * (n // 2 + 1) * batch
+            h = dims[-1] // 2 + 1
+            hn = n // dims[-1] * h
+            x = torch.randn(batch * hn, dtype=cdt, device="cuda:0")
+            if len(dims) > 1:
+                x = x.reshape(dims[:-1] + (h,))
+                p = F.plan_irfft(x, dims[-1])
+                y = torch.empty(dims, dtype=T, device="cuda:0")
+            else:
+                p = F.plan_irfft(T, n, batch=batch)
+                y = torch.empty(batch * n, dtype=T, device="cuda:0")
+            bytes_ = (es // 2) * n * batch + es * hn * batch
             flops = 2.5 * n * math.log2(n) * batch
         ms = F.tune.time_plan(p, y, x, iters=10)
         out.append({"config": name, "ms": round(ms, 4), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),

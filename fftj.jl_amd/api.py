@@ -3,13 +3,13 @@ real.jl, ndplan.jl) so a GPU plan stands in for a CPU one.
 
     Julia (reference)                 here
     plan_fft(T, n)                    plan_fft(torch.float32, n)       plan.jl:207-213
-    plan_fft(x::Array)                plan_fft(x)                      api.jl:135-140
-    p(y, x) / p(x) / p * x            p(y, x) / p(x) / p * x           plan.jl:359-398, api.jl:61-63
-    mul!(y, p, x)                     mul_(y, p, x)                    api.jl:62-82
-    plan_fft!(T, n) / InplacePlan     plan_fft_(T, n) / InplacePlan    api.jl:43-52
-    inv(p) -> Scaled(p, s)            inv(p) -> Scaled(p, s)           api.jl:100-122
-    size(p)                           size(p)                          api.jl:85-88, 194-196
-    fft, ifft, bfft, fft!, rfft, ...  fft, ifft, bfft, fft_, rfft, ...  api.jl:124-127, 209-231
+    plan_fft(x::Array)                plan_fft(x)                      api.jl:107-112
+    p(y, x) / p(x) / p * x            p(y, x) / p(x) / p * x           plan.jl:359-398, api.jl:34-35
+    mul!(y, p, x)                     mul_(y, p, x)                    api.jl:34-55
+    plan_fft!(T, n) / InplacePlan     plan_fft_(T, n) / InplacePlan    api.jl:15-24, 42-49
+    inv(p) -> Scaled(p, s)            inv(p) -> Scaled(p, s)           api.jl:72-94
+    size(p)                           size(p)                          api.jl:57-60, 166-168
+    fft, ifft, bfft, fft!, rfft, ...  fft, ifft, bfft, fft_, rfft, ...  api.jl:97-99, 181-203
 
 Data are torch tensors on the GPU (complex64/complex128 for complex sides,
 float32/float64 for real sides), contiguous.  torch is only the memory and
@@ -28,7 +28,7 @@ import math
 import torch
 
 from . import _lib
-from ._lib import ArgumentError, DimensionMismatch, UnsupportedError, check, dims_array, lib
+from ._lib import ArgumentError, DimensionMismatch, check, dims_array, lib
 
 F32, F64 = _lib.F32, _lib.F64
 _CPLX = {F32: torch.complex64, F64: torch.complex128}
@@ -68,14 +68,17 @@ def _require(x, dtype, what):
 
 
 class _Handle:
-    __slots__ = ("ptr",)
+    # keeps its own reference to the destroy entry point: at interpreter exit
+    # module globals may already be cleared when the last plans are collected
+    __slots__ = ("ptr", "_destroy")
 
     def __init__(self, ptr):
         self.ptr = ptr
+        self._destroy = lib().fftjh_plan_destroy
 
     def __del__(self):
-        if self.ptr and _lib._LIB is not None:
-            _lib._LIB.fftjh_plan_destroy(self.ptr)
+        if self.ptr:
+            self._destroy(self.ptr)
             self.ptr = None
 
 
@@ -168,20 +171,41 @@ class FFTPlan(_PlanBase):
 
 
 class RfftPlan(_PlanBase):
-    """Real forward plan (real.jl:6-13): n reals -> n//2+1 complex."""
+    """Real forward plan: n reals -> n//2+1 complex (real.jl:6-13), or, with
+    shape=(s0, ..., s_{N-1}), the N-d plan of that real tensor, halved along
+    the last (contiguous) axis (ndplan.jl:355-418)."""
 
-    def __init__(self, dtype, n, batch=1):
+    def __init__(self, dtype, n, batch=1, shape=None):
         self.dtype_code = dtype
-        self.n = int(n)
         self.batch = int(batch)
+        self.shape = None if shape is None or len(shape) <= 1 else tuple(int(d) for d in shape)
+        self.dims = (int(n),) if self.shape is None else tuple(reversed(self.shape))
+        self.n = self.dims[0]
         h = ctypes.c_void_p()
-        check(lib().fftjh_plan_r2c(ctypes.byref(h), dtype, self.n, self.batch, 0))
+        if self.shape is None:
+            check(lib().fftjh_plan_r2c(ctypes.byref(h), dtype, self.n, self.batch, 0))
+        else:
+            check(lib().fftjh_plan_r2c_nd(ctypes.byref(h), dtype, len(self.dims), dims_array(self.dims),
+                                          self.batch, 0))
         self._h = _Handle(h.value)
+
+    @property
+    def cshape(self):
+        return self.shape[:-1] + (self.n // 2 + 1,)
 
     def __call__(self, y, x=None):
         if x is None:
             x, y = y, None
         _require(x, _REAL[self.dtype_code], "input")
+        if self.shape is not None:
+            if tuple(x.shape) != self.shape:
+                raise DimensionMismatch(f"plan dims {self.shape}, input {tuple(x.shape)}")
+            if y is None:
+                y = torch.empty(self.cshape, dtype=_CPLX[self.dtype_code], device=x.device)
+            _require(y, _CPLX[self.dtype_code], "output")
+            if tuple(y.shape) != self.cshape:
+                raise DimensionMismatch(f"rfft output dims must be {self.cshape}, got {tuple(y.shape)}")
+            return self._exec(y, x)
         if x.numel() != self.n * self.batch:
             raise DimensionMismatch(f"plan length {self.n * self.batch}, input length {x.numel()}")
         nout = (self.n // 2 + 1) * self.batch
@@ -198,22 +222,44 @@ class RfftPlan(_PlanBase):
 
 
 class IrfftPlan(_PlanBase):
-    """Real inverse plan (real.jl:15-23): n//2+1 complex -> n reals;
+    """Real inverse plan: n//2+1 complex -> n reals (real.jl:15-23), or, with
+    shape=(s0, ..., s_{N-1}) the real output's shape, the N-d plan from the
+    (s0, ..., s_{N-1}//2+1) complex tensor (ndplan.jl:435-521).
     normalized=True is irfft, False is brfft."""
 
-    def __init__(self, dtype, n, normalized=True, batch=1):
+    def __init__(self, dtype, n, normalized=True, batch=1, shape=None):
         self.dtype_code = dtype
-        self.n = int(n)
         self.batch = int(batch)
         self.normalized = bool(normalized)
+        self.shape = None if shape is None or len(shape) <= 1 else tuple(int(d) for d in shape)
+        self.dims = (int(n),) if self.shape is None else tuple(reversed(self.shape))
+        self.n = self.dims[0]
         h = ctypes.c_void_p()
-        check(lib().fftjh_plan_c2r(ctypes.byref(h), dtype, self.n, self.batch, int(self.normalized), 0))
+        if self.shape is None:
+            check(lib().fftjh_plan_c2r(ctypes.byref(h), dtype, self.n, self.batch, int(self.normalized), 0))
+        else:
+            check(lib().fftjh_plan_c2r_nd(ctypes.byref(h), dtype, len(self.dims), dims_array(self.dims),
+                                          self.batch, int(self.normalized), 0))
         self._h = _Handle(h.value)
+
+    @property
+    def cshape(self):
+        return self.shape[:-1] + (self.n // 2 + 1,)
 
     def __call__(self, y, X=None):
         if X is None:
             X, y = y, None
         _require(X, _CPLX[self.dtype_code], "input")
+        if self.shape is not None:
+            what = "irfft" if self.normalized else "brfft"
+            if tuple(X.shape) != self.cshape:
+                raise DimensionMismatch(f"{what} input dims must be {self.cshape}, got {tuple(X.shape)}")
+            if y is None:
+                y = torch.empty(self.shape, dtype=_REAL[self.dtype_code], device=X.device)
+            _require(y, _REAL[self.dtype_code], "output")
+            if tuple(y.shape) != self.shape:
+                raise DimensionMismatch(f"plan dims {self.shape}, output {tuple(y.shape)}")
+            return self._exec(y, X)
         nin = (self.n // 2 + 1) * self.batch
         if X.numel() != nin:
             raise DimensionMismatch(f"irfft input must have length {nin}, got {X.numel()}")
@@ -230,7 +276,7 @@ class IrfftPlan(_PlanBase):
 
 
 class InplacePlan:
-    """In-place wrapper (api.jl:43-52): ip * x transforms x in place."""
+    """In-place wrapper (api.jl:15-24, 42-49): ip * x transforms x in place."""
 
     def __init__(self, p):
         self.p = p
@@ -247,7 +293,7 @@ class InplacePlan:
 
 
 class Scaled:
-    """A plan times a constant, as inv returns (api.jl:54-59); .p and .scale
+    """A plan times a constant, as inv returns (api.jl:26-31); .p and .scale
     are public and .p stays the unnormalized plan.  Application runs one
     fused plan whose kernels fold the combined scale into their stores."""
 
@@ -274,7 +320,7 @@ class Scaled:
 
 
 def mul_(y, p, x):
-    """mul!(y, p, x): apply p to x into y and return y (api.jl:61-82)."""
+    """mul!(y, p, x): apply p to x into y and return y (api.jl:34-55)."""
     if isinstance(p, (InplacePlan, Scaled)):
         p(y, x)
         return y
@@ -282,7 +328,7 @@ def mul_(y, p, x):
 
 
 def inv(p):
-    """inv(p) (api.jl:100-122, 198-207): forward -> Scaled(bfft, 1/n) (so
+    """inv(p) (api.jl:72-94, 170-180): forward -> Scaled(bfft, 1/n) (so
     inv(p).p is the unnormalized backward plan); ifft -> plain fft;
     bfft -> Scaled(fft, 1/n); inv(c*P) = inv(P)/c."""
     if isinstance(p, InplacePlan):
@@ -301,7 +347,7 @@ def inv(p):
 
 
 def size(p):
-    """size(p) (api.jl:85-88, 194-196)."""
+    """size(p) (api.jl:57-60, 166-168)."""
     if isinstance(p, (InplacePlan,)):
         return size(p.p)
     if isinstance(p, Scaled):
@@ -309,16 +355,16 @@ def size(p):
     if isinstance(p, FFTPlan):
         return p.shape
     if isinstance(p, RfftPlan):
-        return (p.n,)
+        return (p.n,) if p.shape is None else p.shape
     if isinstance(p, IrfftPlan):
-        return (p.n // 2 + 1,)
+        return (p.n // 2 + 1,) if p.shape is None else p.cshape
     raise ArgumentError(f"size is not defined for {type(p).__name__}")
 
 
 # ------------------------------------------------------------ constructors
 
 def _check_region(region, N):
-    # _checkregion_nd (api.jl:131-133): N-d plans transform all dims
+    # _checkregion_nd (api.jl:103-105): N-d plans transform all dims
     if region is None:
         return
     r = tuple(region) if not isinstance(region, int) else (region,)
@@ -384,24 +430,24 @@ def _planner_tuning(planner, code, n, batch, sign):
 
 
 def plan_fft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None, **flags):
-    """plan_fft(T, n) (plan.jl:207-208) or plan_fft(x[, region]) (api.jl:135-140).
+    """plan_fft(T, n) (plan.jl:207-208) or plan_fft(x[, region]) (api.jl:107-112).
     planner= picks the schedule search for 1-D plans of a given length;
     precision= selects split-precision MFMA stages for FP32 plans."""
     return _plan_c2c(T_or_x, n, -1, "one", region, batch, planner, precision)
 
 
 def plan_ifft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None, **flags):
-    """plan_ifft (plan.jl:209-210, api.jl:147-152): inverse scaled by 1/n."""
+    """plan_ifft (plan.jl:209-210, api.jl:119-124): inverse scaled by 1/n."""
     return _plan_c2c(T_or_x, n, +1, "inv", region, batch, planner, precision)
 
 
 def plan_bfft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None, **flags):
-    """plan_bfft (plan.jl:212-213, api.jl:141-146): unnormalized inverse."""
+    """plan_bfft (plan.jl:212-213, api.jl:113-118): unnormalized inverse."""
     return _plan_c2c(T_or_x, n, +1, "one", region, batch, planner, precision)
 
 
 def plan_fft_(T_or_x, n=None, region=None, **kw):
-    """plan_fft! (api.jl:47-48, 153-154)."""
+    """plan_fft! (api.jl:19-20, 125-126)."""
     return InplacePlan(plan_fft(T_or_x, n, region, **kw))
 
 
@@ -414,14 +460,16 @@ def plan_bfft_(T_or_x, n=None, region=None, **kw):
 
 
 def plan_rfft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, **flags):
-    """plan_rfft(T, n) (real.jl:41-52) or plan_rfft(x) (api.jl:159-164)."""
+    """plan_rfft(T, n) (real.jl:41-52) or plan_rfft(x) (api.jl:131-136)."""
     if isinstance(T_or_x, torch.Tensor):
         x = T_or_x
         if x.is_complex():
             raise ArgumentError("rfft plans need a real array")
         _check_region(region, x.dim())
         if x.dim() > 1:
-            raise UnsupportedError("N-d real plans are outside this build")
+            if any(d < 1 for d in x.shape):
+                raise ArgumentError(f"dimensions must be positive (got {tuple(reversed(x.shape))})")
+            return RfftPlan(_code_of_tensor(x), x.shape[-1], 1, tuple(x.shape))
         return RfftPlan(_code_of_tensor(x), _positive_n(x.numel()), 1)
     return RfftPlan(_code(T_or_x), _positive_n(n), batch)
 
@@ -430,30 +478,34 @@ def _plan_c2r(T_or_X, n, normalized, region, batch):
     if isinstance(T_or_X, torch.Tensor):
         X = T_or_X
         _check_region(region, X.dim())
-        if X.dim() > 1:
-            raise UnsupportedError("N-d real plans are outside this build")
         n = int(n)
-        if X.shape[0] != n // 2 + 1:
+        lead = X.shape[-1] if X.dim() else 1
+        if lead != n // 2 + 1:
             what = "irfft" if normalized else "brfft"
-            raise DimensionMismatch(f"{what} of leading length {n} needs {n // 2 + 1} rows, got {X.shape[0]}")
+            raise DimensionMismatch(f"{what} of leading length {n} needs {n // 2 + 1} rows, got {lead}")
+        if X.dim() > 1:
+            shape = tuple(X.shape[:-1]) + (n,)
+            if any(d < 1 for d in shape):
+                raise ArgumentError(f"dimensions must be positive (got {tuple(reversed(shape))})")
+            return IrfftPlan(_code_of_tensor(X), n, normalized, 1, shape)
         return IrfftPlan(_code_of_tensor(X), _positive_n(n), normalized, 1)
     return IrfftPlan(_code(T_or_X), _positive_n(n), normalized, batch)
 
 
 def plan_irfft(T_or_X, n, region=None, *, batch=1, nthreads=None, **flags):
-    """plan_irfft(T, n) (real.jl:57-70) or plan_irfft(X, n1) (api.jl:174-182)."""
+    """plan_irfft(T, n) (real.jl:57-70) or plan_irfft(X, n1) (api.jl:146-154)."""
     return _plan_c2r(T_or_X, n, True, region, batch)
 
 
 def plan_brfft(T_or_X, n, region=None, *, batch=1, nthreads=None, **flags):
-    """plan_brfft(T, n) (real.jl:72-73) or plan_brfft(X, n1) (api.jl:165-173)."""
+    """plan_brfft(T, n) (real.jl:72-73) or plan_brfft(X, n1) (api.jl:137-145)."""
     return _plan_c2r(T_or_X, n, False, region, batch)
 
 
 # ---------------------------------------------------------------- one-shots
 
 def _as_complex(x):
-    # promoting fallbacks (api.jl:220-231): integers and reals -> float complex
+    # promoting fallbacks (api.jl:192-203): integers and reals -> float complex
     if not x.is_complex():
         if not x.is_floating_point():
             x = x.to(torch.float64)
@@ -483,7 +535,7 @@ def bfft(x):
 
 
 def fft_(x):
-    """fft!(x) (api.jl:125): in place."""
+    """fft!(x) (api.jl:97): in place."""
     return plan_fft_(x) * x
 
 

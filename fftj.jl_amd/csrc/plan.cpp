@@ -857,28 +857,20 @@ static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch
   return L;
 }
 
-static Layout layout_r2c(int dtype, int64_t n, int64_t batch) {
-  check_dtype(dtype);
-  if (n < 1) fail(FFTJH_ERR_ARGUMENT, "transform length must be positive (got " + std::to_string(n) + ")");
-  if (batch < 1) fail(FFTJH_ERR_ARGUMENT, "batch must be positive");
-  Layout L;
-  L.dtype = dtype;
-  L.kind = FFTJH_KIND_R2C;
-  L.sign = -1;
-  L.rank = 1;
-  L.dims[0] = n;
-  L.batch = batch;
-  L.in_elems = n * batch;
-  L.out_elems = (n / 2 + 1) * batch;
+// Real-to-complex rows: `batch` rows of n reals at src (row stride n) ->
+// rows of n/2+1 complex at dst (row stride n/2+1), through the scratch roles
+// in `free` (front first; at least four).  Even n: the half-length complex
+// transform of the reals read as n/2 complex points (real.jl:88-131); odd n:
+// the full complex transform of the widened reals, prefix kept.
+static void add_r2c_rows(Layout& L, int64_t n, int64_t batch, int src, int dst, const std::vector<int>& free) {
   if (n % 2 == 0) {
     const int64_t h = n / 2;
-    // x read as h complex points per row (real.jl:88), transform into S0
-    need_scratch(L, RS0, h * batch);
-    build_c2c_rows(L, h, batch, -1, 1.0, RX, RS0, {RS1, RS2, RS3});
+    need_scratch(L, free[0], h * batch);
+    build_c2c_rows(L, h, batch, -1, 1.0, src, free[0], std::vector<int>(free.begin() + 1, free.end()));
     Step t;
     t.kind = ST_R2C_TWIST;
-    t.src = RS0;
-    t.dst = RY;
+    t.src = free[0];
+    t.dst = dst;
     t.dir = -1;
     t.n = n;
     t.h = h;
@@ -887,49 +879,40 @@ static Layout layout_r2c(int dtype, int64_t n, int64_t batch) {
     t.s2 = h + 1;
     L.steps.push_back(t);
   } else {
-    need_scratch(L, RS0, n * batch);
-    need_scratch(L, RS1, n * batch);
+    need_scratch(L, free[0], n * batch);
+    need_scratch(L, free[1], n * batch);
     Step e;
     e.kind = ST_REAL_TO_COMPLEX;
-    e.src = RX;
-    e.dst = RS0;
+    e.src = src;
+    e.dst = free[0];
     e.n = n;
     e.batch = batch;
     L.steps.push_back(e);
-    build_c2c_rows(L, n, batch, -1, 1.0, RS0, RS1, {RS2, RS3});
+    build_c2c_rows(L, n, batch, -1, 1.0, free[0], free[1], std::vector<int>(free.begin() + 2, free.end()));
     Step c;
     c.kind = ST_COPY_PREFIX;
-    c.src = RS1;
-    c.dst = RY;
+    c.src = free[1];
+    c.dst = dst;
     c.n = n;
     c.keep = n / 2 + 1;
     c.batch = batch;
     L.steps.push_back(c);
   }
-  return L;
 }
 
-static Layout layout_c2r(int dtype, int64_t n, int64_t batch, int normalized) {
-  check_dtype(dtype);
-  if (n < 1) fail(FFTJH_ERR_ARGUMENT, "transform length must be positive (got " + std::to_string(n) + ")");
-  if (batch < 1) fail(FFTJH_ERR_ARGUMENT, "batch must be positive");
-  Layout L;
-  L.dtype = dtype;
-  L.kind = FFTJH_KIND_C2R;
-  L.sign = 1;
-  L.rank = 1;
-  L.dims[0] = n;
-  L.batch = batch;
-  L.scale = normalized ? 1.0 / static_cast<double>(n) : 1.0;
-  L.in_elems = (n / 2 + 1) * batch;
-  L.out_elems = n * batch;
+// Complex-to-real rows: `batch` rows of n/2+1 complex at src -> rows of n
+// reals at dst, scaled by 1/n when normalized (irfft), else 1 (brfft).  src
+// is read once, by the first step, so a scratch src may also be listed in
+// `free` behind the roles that step writes.
+static void add_c2r_rows(Layout& L, int64_t n, int64_t batch, bool normalized, int src, int dst,
+                         const std::vector<int>& free) {
   if (n % 2 == 0) {
     const int64_t h = n / 2;
-    need_scratch(L, RS0, h * batch);
+    need_scratch(L, free[0], h * batch);
     Step p;
     p.kind = ST_C2R_PRE;
-    p.src = RX;
-    p.dst = RS0;
+    p.src = src;
+    p.dst = free[0];
     p.dir = 1;
     p.n = n;
     p.h = h;
@@ -939,25 +922,115 @@ static Layout layout_c2r(int dtype, int64_t n, int64_t batch, int normalized) {
     p.pre = normalized ? 0.5 : 1.0;
     L.steps.push_back(p);
     // inner inverse of length h: scale 1/h (irfft) or 1 (brfft), real.jl:64-65
-    build_c2c_rows(L, h, batch, 1, normalized ? 1.0 / static_cast<double>(h) : 1.0, RS0, RY, {RS1, RS2, RS3});
+    build_c2c_rows(L, h, batch, 1, normalized ? 1.0 / static_cast<double>(h) : 1.0, free[0], dst,
+                   std::vector<int>(free.begin() + 1, free.end()));
   } else {
-    need_scratch(L, RS0, n * batch);
-    need_scratch(L, RS1, n * batch);
+    need_scratch(L, free[0], n * batch);
+    need_scratch(L, free[1], n * batch);
     Step f;
     f.kind = ST_HERMITIAN_FILL;
-    f.src = RX;
-    f.dst = RS0;
+    f.src = src;
+    f.dst = free[0];
     f.n = n;
     f.batch = batch;
     L.steps.push_back(f);
-    build_c2c_rows(L, n, batch, 1, L.scale, RS0, RS1, {RS2, RS3});
+    build_c2c_rows(L, n, batch, 1, normalized ? 1.0 / static_cast<double>(n) : 1.0, free[0], free[1],
+                   std::vector<int>(free.begin() + 2, free.end()));
     Step t;
     t.kind = ST_TAKE_REAL;
-    t.src = RS1;
-    t.dst = RY;
+    t.src = free[1];
+    t.dst = dst;
     t.n = n;
     t.batch = batch;
     L.steps.push_back(t);
+  }
+}
+
+// Strided complex passes along dims[1..rank-1] of the column-major array
+// cdims (dims[0] is the batch within each pass), src -> dst then in place on
+// dst (ndplan.jl:57-118), `scale` on the last pass.  Lays out nothing when
+// every one of those dims is 1.
+static void add_axis_passes(Layout& L, int rank, const int64_t* cdims, int64_t batch, int sign, double scale, int src,
+                            int dst, const std::vector<int>& free) {
+  std::vector<int> axes;
+  for (int i = 1; i < rank; ++i)
+    if (cdims[i] > 1) axes.push_back(i);
+  for (size_t ai = 0; ai < axes.size(); ++ai) {
+    const int ax = axes[ai];
+    int64_t Bi = 1, Bo = batch;
+    for (int i = 0; i < ax; ++i) Bi *= cdims[i];
+    for (int i = ax + 1; i < rank; ++i) Bo *= cdims[i];
+    const int64_t nd = cdims[ax];
+    const double sc = (ai + 1 == axes.size()) ? scale : 1.0;
+    build_c2c(L, nd, Bi * Bo, Bi, geo(Bi, 1, Bi * nd), geo(Bi, 1, Bi * nd), sign, sc, ai == 0 ? src : dst, dst, free);
+  }
+}
+
+static void check_real_request(int dtype, int rank, const int64_t* dims, int64_t batch) {
+  check_dtype(dtype);
+  if (rank < 1 || rank > 8) fail(FFTJH_ERR_ARGUMENT, "rank must be in 1..8");
+  if (!dims) fail(FFTJH_ERR_ARGUMENT, "dims is NULL");
+  for (int i = 0; i < rank; ++i)
+    if (dims[i] < 1)
+      fail(FFTJH_ERR_ARGUMENT, rank == 1 ? "transform length must be positive (got " + std::to_string(dims[i]) + ")"
+                                         : "dimensions must be positive");
+  if (batch < 1) fail(FFTJH_ERR_ARGUMENT, "batch must be positive");
+}
+
+// R2C over all dims of the column-major real array dims (dims[0] halved):
+// real rows along dims[0] into y, then the complex passes of the other dims
+// in place on y (ndplan.jl:355-418).  rank 1 is the 1-D plan (real.jl:41-52).
+static Layout layout_r2c(int dtype, int rank, const int64_t* dims, int64_t batch) {
+  check_real_request(dtype, rank, dims, batch);
+  Layout L;
+  L.dtype = dtype;
+  L.kind = FFTJH_KIND_R2C;
+  L.sign = -1;
+  L.rank = rank;
+  int64_t rest = 1;
+  for (int i = 0; i < rank; ++i) L.dims[i] = dims[i];
+  for (int i = 1; i < rank; ++i) rest *= dims[i];
+  L.batch = batch;
+  const int64_t n1 = dims[0], h1p = n1 / 2 + 1;
+  L.in_elems = n1 * rest * batch;
+  L.out_elems = h1p * rest * batch;
+  add_r2c_rows(L, n1, rest * batch, RX, RY, {RS0, RS1, RS2, RS3});
+  int64_t cdims[8];
+  for (int i = 0; i < rank; ++i) cdims[i] = i == 0 ? h1p : dims[i];
+  add_axis_passes(L, rank, cdims, batch, -1, 1.0, RY, RY, {RS0, RS1, RS2, RS3});
+  return L;
+}
+
+// C2R over all dims (ndplan.jl:435-521): the inverse complex passes of
+// dims[1..] from x into scratch (x is left intact), then the real rows along
+// dims[0] into y.  normalized scales by 1/prod(dims) (irfft), else 1 (brfft).
+static Layout layout_c2r(int dtype, int rank, const int64_t* dims, int64_t batch, int normalized) {
+  check_real_request(dtype, rank, dims, batch);
+  Layout L;
+  L.dtype = dtype;
+  L.kind = FFTJH_KIND_C2R;
+  L.sign = 1;
+  L.rank = rank;
+  int64_t rest = 1;
+  for (int i = 0; i < rank; ++i) L.dims[i] = dims[i];
+  for (int i = 1; i < rank; ++i) rest *= dims[i];
+  L.batch = batch;
+  const int64_t n1 = dims[0], h1p = n1 / 2 + 1;
+  L.scale = normalized ? 1.0 / (static_cast<double>(n1) * static_cast<double>(rest)) : 1.0;
+  L.in_elems = h1p * rest * batch;
+  L.out_elems = n1 * rest * batch;
+  int64_t cdims[8];
+  for (int i = 0; i < rank; ++i) cdims[i] = i == 0 ? h1p : dims[i];
+  // irfft: 1/prod(dims[1..]) on the last complex pass, 1/n1 on the rows
+  const double rest_scale = normalized ? 1.0 / static_cast<double>(rest) : 1.0;
+  bool passes = false;
+  for (int i = 1; i < rank; ++i) passes |= dims[i] > 1;
+  if (passes) {
+    need_scratch(L, RS0, h1p * rest * batch);
+    add_axis_passes(L, rank, cdims, batch, 1, rest_scale, RX, RS0, {RS1, RS2, RS3});
+    add_c2r_rows(L, n1, rest * batch, normalized != 0, RS0, RY, {RS1, RS2, RS3, RS0});
+  } else {
+    add_c2r_rows(L, n1, rest * batch, normalized != 0, RX, RY, {RS0, RS1, RS2, RS3});
   }
   return L;
 }
@@ -1559,7 +1632,7 @@ int fftjh_plan_r2c(fftjh_plan* plan, int dtype, int64_t n, int64_t batch, unsign
   return guarded([&] {
     if (plan) *plan = nullptr;
     if (flags) fail(FFTJH_ERR_ARGUMENT, "flags must be 0");
-    make_plan(plan, layout_r2c(dtype, n, batch));
+    make_plan(plan, layout_r2c(dtype, 1, &n, batch));
   });
 }
 
@@ -1567,7 +1640,24 @@ int fftjh_plan_c2r(fftjh_plan* plan, int dtype, int64_t n, int64_t batch, int no
   return guarded([&] {
     if (plan) *plan = nullptr;
     if (flags) fail(FFTJH_ERR_ARGUMENT, "flags must be 0");
-    make_plan(plan, layout_c2r(dtype, n, batch, normalized));
+    make_plan(plan, layout_c2r(dtype, 1, &n, batch, normalized));
+  });
+}
+
+int fftjh_plan_r2c_nd(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, unsigned flags) {
+  return guarded([&] {
+    if (plan) *plan = nullptr;
+    if (flags) fail(FFTJH_ERR_ARGUMENT, "flags must be 0");
+    make_plan(plan, layout_r2c(dtype, rank, dims, batch));
+  });
+}
+
+int fftjh_plan_c2r_nd(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, int normalized,
+                      unsigned flags) {
+  return guarded([&] {
+    if (plan) *plan = nullptr;
+    if (flags) fail(FFTJH_ERR_ARGUMENT, "flags must be 0");
+    make_plan(plan, layout_c2r(dtype, rank, dims, batch, normalized));
   });
 }
 
@@ -1632,10 +1722,11 @@ int fftjh_describe_request(int kind, int dtype, int rank, const int64_t* dims, i
   return guarded([&] {
     Layout L;
     if (kind == FFTJH_KIND_C2C) L = layout_c2c(dtype, rank, dims, batch, sign, 1.0);
-    else if (kind == FFTJH_KIND_R2C || kind == FFTJH_KIND_C2R) {
-      if (rank != 1 || !dims) fail(FFTJH_ERR_ARGUMENT, "real plans are 1-D");
-      L = kind == FFTJH_KIND_R2C ? layout_r2c(dtype, dims[0], batch) : layout_c2r(dtype, dims[0], batch, 1);
-    } else fail(FFTJH_ERR_ARGUMENT, "unknown kind");
+    else if (kind == FFTJH_KIND_R2C)
+      L = layout_r2c(dtype, rank, dims, batch);
+    else if (kind == FFTJH_KIND_C2R)
+      L = layout_c2r(dtype, rank, dims, batch, 1);
+    else fail(FFTJH_ERR_ARGUMENT, "unknown kind");
     copy_out(describe(L), buf, len);
   });
 }

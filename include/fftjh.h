@@ -13,7 +13,7 @@
  *   - data: interleaved complex (re, im) of the plan's precision; N-d arrays
  *     column-major, dims[0] fastest (ndplan.jl:7-31);
  *   - execution is alias safe: y == x is an in-place transform
- *     (api.jl:39-40);
+ *     (api.jl:11-13);
  *   - errors are reported before any work is queued; codes 1 and 2 map to
  *     Julia's ArgumentError and DimensionMismatch (plan.jl:171, 361-362).
  *
@@ -64,7 +64,7 @@ typedef void* fftjh_stream;
 /* Complex plan.  rank 1 with dims[0] = n replaces
  *   plan_fft / plan_ifft / plan_bfft(::Type{T}, n)        plan.jl:207-213
  * rank >= 2 replaces the array-based N-d constructors
- *   plan_fft / plan_ifft / plan_bfft(x::Array{Complex{T},N})  api.jl:135-152
+ *   plan_fft / plan_ifft / plan_bfft(x::Array{Complex{T},N})  api.jl:107-124
  * (all dims transformed, dims[0] fastest).  batch > 1 runs that many
  * independent transforms stored back to back (an extension: the reference
  * has no batch API and loops over 1-D plans).  flags: 0, or one
@@ -129,9 +129,27 @@ int fftjh_plan_r2c(fftjh_plan* plan, int dtype, int64_t n, int64_t batch, unsign
  * normalized == 0 replaces plan_brfft(::Type{T}, n)        real.jl:72-73 */
 int fftjh_plan_c2r(fftjh_plan* plan, int dtype, int64_t n, int64_t batch, int normalized, unsigned flags);
 
+/* N-d real-to-complex plan over all dims of the column-major real array
+ * dims[0..rank-1] (dims[0] fastest, halved to dims[0]/2+1 complex), `batch`
+ * such arrays back to back.  Real rows along dims[0], then the complex
+ * passes of the other dims in place on y.  rank 1 is fftjh_plan_r2c.
+ * Replaces plan_rfft(X::Array{T,N})                      ndplan.jl:355-418,
+ * api.jl:131-136 */
+int fftjh_plan_r2c_nd(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, unsigned flags);
+
+/* N-d complex-to-real plan: (dims[0]/2+1, dims[1], ...) complex -> dims
+ * reals; the complex passes of dims[1..] run from x into plan scratch (x is
+ * not written), then the real rows along dims[0].  normalized != 0 scales by
+ * 1/prod(dims) (irfft), normalized == 0 by 1 (brfft).
+ * Replaces plan_irfft(X::Array{Complex{T},N}, n1) / plan_brfft(X, n1)
+ *                                                        ndplan.jl:435-521,
+ * api.jl:137-154 */
+int fftjh_plan_c2r_nd(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, int normalized,
+                      unsigned flags);
+
 /* Apply the plan: y = P x.  Replaces p(y, x) / mul!(y, p, x)
  * (plan.jl:359-396, fourstep.jl:338-356, real.jl:80-211, ndplan.jl:329-335,
- * api.jl:61-69).  No length checks (see fftjh_execute_checked). */
+ * api.jl:34-55).  No length checks (see fftjh_execute_checked). */
 int fftjh_execute(fftjh_plan plan, void* y, const void* x, fftjh_stream stream);
 
 /* As fftjh_execute, after checking the element counts of y and x against
@@ -141,7 +159,7 @@ int fftjh_execute(fftjh_plan plan, void* y, const void* x, fftjh_stream stream);
 int fftjh_execute_checked(fftjh_plan plan, void* y, int64_t y_elems, const void* x, int64_t x_elems,
                           fftjh_stream stream);
 
-/* Shape queries.  Replaces size(p) (api.jl:85-88, 194-196): kind, dtype,
+/* Shape queries.  Replaces size(p) (api.jl:57-60, 166-168): kind, dtype,
  * sign, rank, dims, batch, scale and the element counts of one whole call's
  * input and output. */
 int fftjh_plan_info(fftjh_plan plan, int* kind, int* dtype, int* sign, int* rank, int64_t* dims /* [8] */,

@@ -128,6 +128,36 @@ def ndfft(x, dims, sign=-1, scale=1.0, prec="f64"):
     return y
 
 
+def _axis_fft(y, ax, sign, prec):
+    # complex transform of every line along axis ax (rows of the oracle's fft)
+    return np.moveaxis(fft(np.ascontiguousarray(np.moveaxis(y, ax, -1)), sign, 1.0, prec), -1, ax)
+
+
+def ndrfft(x, prec="f64"):
+    """N-d real forward transform of a row-major real array (shape s0..s_{N-1}
+    is the column-major array of the reversed dims), halved along the last,
+    contiguous axis: real rows along that axis first, then the complex
+    transform of every other axis (ndplan.jl:370-418: _real_pencils! into
+    the intermediate, then _exec_nd! of dims 2..N)."""
+    y = rfft(x, prec)
+    for ax in range(np.ndim(x) - 1):
+        y = _axis_fft(y, ax, -1, prec)
+    return np.ascontiguousarray(y)
+
+
+def ndirfft(X, n, normalized=True, prec="f64"):
+    """N-d complex-to-real inverse (ndplan.jl:478-521): the complex inverse
+    of every axis but the last, scaled by 1/prod of those dims when
+    normalized (restscale, ndplan.jl:456), then the real rows along the
+    last axis (1/n when normalized, brfft otherwise)."""
+    y = _cplx(X, prec)
+    for ax in range(y.ndim - 1):
+        y = _axis_fft(y, ax, +1, prec)
+    if normalized and y.ndim > 1:
+        y = (y / np.prod(y.shape[:-1])).astype(_NP_CPLX[prec])
+    return irfft(y, n, normalized, prec)
+
+
 def dft_naive(x, sign=-1, prec="ld"):
     x = _cplx(x, prec)
     y = np.empty_like(x)

@@ -96,6 +96,42 @@ def test_dimension_and_argument_errors():
         p(torch.zeros(8, dtype=torch.complex64, device="cuda:0"))
 
 
+def test_nd_real_api():
+    """N-d real plans from example arrays (api.jl:131-154): size(p), mul!,
+    one-shots (runtests.jl:385-387) and the dimension errors."""
+    dev = "cuda:0"
+    Ar = to_dev(F.rng.normal(11, 240).reshape(12, 20).astype(np.float32))
+    X = F.rfft(Ar)
+    assert X.shape == (12, 11)
+    tol = 200 * np.finfo(np.float32).eps
+    assert rel_l2(to_host(X), O.ndrfft(to_host(Ar).astype(np.float64))) <= tol
+    assert rel_l2(to_host(F.irfft(X, 20)), to_host(Ar)) <= 2 * tol
+    assert rel_l2(to_host(F.brfft(X, 20)), 240 * to_host(Ar)) <= 2 * tol
+    pr = F.plan_rfft(Ar)
+    assert F.size(pr) == (12, 20)
+    Y = torch.empty(12, 11, dtype=torch.complex64, device=dev)
+    assert F.mul_(Y, pr, Ar) is Y and torch.equal(Y, X)
+    pi = F.plan_irfft(X, 20)
+    assert F.size(pi) == (12, 11)
+    with pytest.raises(F.DimensionMismatch):
+        pr(torch.zeros(20, 12, dtype=torch.float32, device=dev))
+    with pytest.raises(F.DimensionMismatch):
+        pr(torch.empty(12, 10, dtype=torch.complex64, device=dev), Ar)
+    with pytest.raises(F.DimensionMismatch):   # leading length 21 needs 11 rows
+        F.plan_irfft(X, 21 + 2)
+    with pytest.raises(F.DimensionMismatch):
+        pi(torch.empty(11, 11, dtype=torch.complex64, device=dev))
+    with pytest.raises(F.DimensionMismatch):
+        pi(torch.empty(12, 21, dtype=torch.float32, device=dev), X)
+    with pytest.raises(F.ArgumentError):
+        F.plan_rfft(Ar, region=(1,))
+    # odd leading length: 21 -> 11 rows as well
+    x21 = to_dev(F.rng.normal(12, 4 * 21).reshape(4, 21).astype(np.float64))
+    X21 = F.rfft(x21)
+    assert X21.shape == (4, 11)
+    assert rel_l2(to_host(F.irfft(X21, 21)), to_host(x21)) <= 1e-13
+
+
 def test_graph_capture_replays():
     """Execution allocates nothing and is capture safe: a captured plan
     replays to the eager result on every replay."""

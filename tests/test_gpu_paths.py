@@ -97,6 +97,67 @@ def test_nd(prec, dims):
     assert rel_l2(yb, n * x.astype(np.complex128)) <= 2 * tol
 
 
+# N-d real plans (runtests.jl:358-370 on the same shapes, plus odd, ragged,
+# non-smooth and long halved / strided axes): torch shape reversed(dims),
+# halved along the last axis
+
+REAL_SHAPES = SHAPES[:13] + [(9, 4, 3), (17, 33), (1, 1), (20, 1, 1), (2**15, 4), (3, 2**15), (1000, 6), (7, 5, 11)]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("dims", REAL_SHAPES)
+def test_nd_real(prec, dims):
+    shape = tuple(reversed(dims))
+    n1 = dims[0]
+    x = F.rng.normal(len(dims) * 7 + n1, int(np.prod(dims))).astype(real_np(prec)).reshape(shape)
+    xd = to_dev(x)
+    pr = F.plan_rfft(xd)
+    assert F.size(pr) == shape
+    Xd = pr * xd
+    X = to_host(Xd)
+    assert X.shape == shape[:-1] + (n1 // 2 + 1,)
+    tol = 1e-13 if prec == "f64" else 200 * np.finfo(np.float32).eps   # runtests.jl:332 tier
+    ref = O.ndrfft(x.astype(np.float64))
+    assert rel_l2(X, ref) <= tol
+    pbr = F.plan_brfft(Xd, n1)
+    assert F.size(pbr) == X.shape
+    bref = O.ndirfft(X.astype(np.complex128), n1, normalized=False)
+    assert rel_l2(to_host(pbr * Xd), bref) <= tol
+    back = to_host(F.plan_irfft(Xd, n1) * Xd)
+    assert rel_l2(back, x) <= 2 * tol
+    assert np.array_equal(to_host(Xd), X)   # the inverse leaves its input intact
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_nd_real_arbitrary_spectrum(prec):
+    # a complex input that is not the spectrum of a real array: the inverse
+    # follows the reference's pre-pass (imaginary parts of the DC and Nyquist
+    # bins enter the even-length pre-pass), as the oracle restates it
+    for shape, n1 in (((6, 5, 9), 16), ((4, 7, 6), 11)):
+        X = F.rng.uniform_complex(n1, int(np.prod(shape))).astype(cplx_np(prec)).reshape(shape)
+        tol = 1e-13 if prec == "f64" else 200 * np.finfo(np.float32).eps
+        for normalized in (True, False):
+            p = (F.plan_irfft if normalized else F.plan_brfft)(to_dev(X), n1)
+            y = to_host(p * to_dev(X))
+            assert y.shape == shape[:-1] + (n1,)
+            ref = O.ndirfft(X.astype(np.complex128), n1, normalized=normalized)
+            assert rel_l2(y, ref) <= tol
+
+
+def test_nd_real_256cubed_properties():
+    # full size: the half spectrum is the prefix of the complex plan's
+    # spectrum of the same data, and irfft inverts it
+    shape = (256, 256, 256)
+    x = F.rng.normal(3, 256 ** 3).astype(np.float32).reshape(shape)
+    xd = to_dev(x)
+    Xd = F.rfft(xd)
+    Y = F.fft(xd.to(torch.complex64))
+    tol = 200 * np.finfo(np.float32).eps
+    assert rel_l2(to_host(Xd), to_host(Y[..., :129])) <= tol
+    assert rel_l2(to_host(F.irfft(Xd, 256)), x) <= 2 * tol
+    assert rel_l2(to_host(Xd), O.ndrfft(x.astype(np.float64))) <= tol
+
+
 def test_config3_nd_256cubed():
     dims = (256, 256, 256)
     n = 256 ** 3

@@ -101,6 +101,28 @@ def test_nd_long_and_nonsmooth_axes():
     assert s[1].startswith("rs_deconv") and s[-1].startswith("rs_interp")
 
 
+def test_nd_real_schedules():
+    """N-d real plans: real rows along dims[0] then in-place complex passes
+    of the other dims (R2C); complex passes into scratch then the real rows
+    (C2R), irfft scale split 1/prod(dims[1:]) and 1/dims[0]."""
+    s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F32, [64, 48]))
+    assert s[0].startswith("pass x->s0") and "n=32 " in s[0] and s[1].startswith("r2c_twist s0->y")
+    assert s[2].startswith("pass y->y") and "n=48 " in s[2] and "in_stride=33 " in s[2]
+    s = _passes(_lib.describe_request(_lib.KIND_C2R, _lib.F32, [64, 48]))
+    assert s[0].startswith("pass x->s0") and "n=48 " in s[0] and "scale=0.0208333" in s[0]
+    assert s[1].startswith("c2r_pre s0->s1") and s[2].startswith("pass s1->y") and "scale=0.03125" in s[2]
+    # odd leading length, three dims
+    s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F64, [9, 4, 3]))
+    assert [p.split()[0] for p in s] == ["real_to_complex", "pass", "copy_prefix", "pass", "pass"]
+    s = _passes(_lib.describe_request(_lib.KIND_C2R, _lib.F64, [9, 4, 3]))
+    assert [p.split()[0] for p in s] == ["pass", "pass", "hermitian_fill", "pass", "take_real"]
+    # every dim past the first is 1: the rows alone, straight from x
+    s = _passes(_lib.describe_request(_lib.KIND_C2R, _lib.F32, [8, 1, 1]))
+    assert s[0].startswith("c2r_pre x->s0") and len(s) == 2
+    with pytest.raises(F.ArgumentError, match="positive"):
+        _lib.describe_request(_lib.KIND_R2C, _lib.F32, [8, 0])
+
+
 def test_region_check():
     import torch
     x = torch.zeros(4, 4, dtype=torch.complex64)

@@ -177,6 +177,44 @@ def test_nd_golden(golden, shape):
     assert rel_l2(y, golden[f"nd_{tag}_y"].ravel()) < 1e-14
 
 
+# N-d real: the oracle composes its own 1-D rfft/irfft and the complex
+# transform of the other axes; pinned against the full complex N-d oracle
+# (the forward half-spectrum of a real array is the prefix of its complex
+# spectrum along the halved axis), numpy's rfftn/irfftn, and round trips
+
+@pytest.mark.parametrize("shape", [(48, 64), (17, 33), (11, 16), (16, 11), (1, 8), (8, 1), (5, 3, 128),
+                                   (14, 10, 12), (3, 40, 24), (4, 3, 9), (6, 7)])
+def test_nd_real_oracle(shape):
+    rng = np.random.default_rng(len(shape) * 1000 + shape[-1])
+    x = rng.standard_normal(shape)
+    n1 = shape[-1]
+    y = O.ndrfft(x)
+    assert y.shape == shape[:-1] + (n1 // 2 + 1,)
+    full = O.ndfft(x.astype(np.complex128).ravel(), list(reversed(shape))).reshape(shape)
+    assert rel_l2(y, full[..., : n1 // 2 + 1]) < 1e-14
+    assert rel_l2(y, np.fft.rfftn(x)) < 1e-14
+    assert rel_l2(O.ndirfft(y, n1), x) < 1e-14
+    assert rel_l2(O.ndirfft(y, n1), np.fft.irfftn(y, s=shape, axes=tuple(range(len(shape))))) < 1e-14
+    assert rel_l2(O.ndirfft(y, n1, normalized=False), x.size * x) < 1e-14  # brfft = prod(dims) irfft
+
+
+def test_nd_real_planted_mistakes():
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((6, 5, 10))
+    ref = np.fft.rfftn(x)
+    # an axis left untransformed
+    y = O.rfft(x)
+    y = O._axis_fft(y, 1, -1, "f64")
+    assert rel_l2(y, ref) > 0.1
+    # the halved axis taken as the first (slowest) one instead of the last
+    wrong = np.moveaxis(O.ndrfft(np.ascontiguousarray(np.moveaxis(x, 0, -1))), -1, 0)
+    assert wrong.shape != ref.shape or rel_l2(wrong, ref) > 0.1
+    # the inverse of the other axes with the forward sign
+    Y = O.ndrfft(x)
+    z = O._axis_fft(O._axis_fft(Y, 0, -1, "f64"), 1, -1, "f64") / 30
+    assert rel_l2(O.irfft(z, 10), x) > 0.1
+
+
 # ---------------------------------------------------------------- RNG spec
 
 def test_splitmix64_kat(fj):
