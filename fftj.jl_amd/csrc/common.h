@@ -84,6 +84,13 @@ struct PassDesc {
   int64_t tw_ar;     // tw_acoef mod P
   int64_t tw_br;     // tw_bcoef mod P
   double tw_invp;    // 1 / P
+  // power-of-two row passes fused with the real edges of an even-length
+  // real transform of n = 2*N reals (N the pass length): 1 = R2C (twist of
+  // the half-length spectrum into N+1 bins at the store), 2 = C2R (pre-pass
+  // of N+1 bins into the half-length input at the load); rtw holds
+  // w_k = exp(dir*2*pi*i*k/n), k = 0..N, in the plan's precision
+  int32_t real_io;
+  const void* rtw;
 };
 
 // LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)

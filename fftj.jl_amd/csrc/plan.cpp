@@ -199,6 +199,28 @@ static bool pow2_fits(int dtype, int n, int C, int* threads) {
   return true;
 }
 
+// Rows per workgroup of a power-of-two row pass fused with the real edges
+// (both HBM sides staged through LDS, so no coalescing cap on C): fill up to
+// 256 threads; 0 when the length has no fused kernel
+static int real_fused_C(int dtype, int64_t h, int64_t batch, int* threads) {
+  static const bool off = [] {
+    const char* e = std::getenv("FFTJH_REAL_FUSED");
+    return (e && std::atoi(e) == 0) || std::getenv("FFTJH_NO_POW2") != nullptr;
+  }();
+  int pr[8], pc = 0;
+  if (off || h > (1 << 14) || !pow2_chain(static_cast<int>(h), pr, &pc)) return 0;
+  const int n = static_cast<int>(h);
+  for (int C = 64; C >= 1; C >>= 1) {
+    int t;
+    if (!pow2_fits(dtype, n, C, &t) || (C > 1 && t > 256)) continue;
+    // more rows than transforms only to fill one wave
+    if (C > 1 && C > batch && (C / 2) * (n / 16) >= 64) continue;
+    if (threads) *threads = t;
+    return C;
+  }
+  return 0;
+}
+
 static bool single_pass_ok(int dtype, int64_t n) {
   if (n == 1) return true;
   if (n > (1 << 16)) return false;
@@ -288,7 +310,7 @@ struct PassTw {
 };
 
 static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geometry in, Geometry out,
-                     double scale, int src, int dst, PassTw tw = PassTw(), int64_t B2 = 1) {
+                     double scale, int src, int dst, PassTw tw = PassTw(), int64_t B2 = 1, int real_io = 0) {
   Step s;
   s.kind = ST_PASS;
   s.src = src;
@@ -359,7 +381,11 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
     return s.pow2 ? pow2_fits(L.dtype, n, C, t) : pass_fits(L.dtype, n, C, s.radices, t);
   };
   int bestC = 0, bestT = 0;
-  if (pt && pt->C > 0) {
+  if (real_io) {
+    // rows of a real transform's half-length spectrum, real edges fused
+    bestC = real_fused_C(L.dtype, n, batch, &bestT);
+    if (!bestC || !s.pow2) fail(FFTJH_ERR_UNSUPPORTED, "internal: no fused real-edge kernel for this length");
+  } else if (pt && pt->C > 0) {
     int t;
     if ((pt->C & (pt->C - 1)) || pt->C > 64) fail(FFTJH_ERR_ARGUMENT, "tuning C must be a power of two <= 64");
     if (!fits(pt->C, &t))
@@ -422,6 +448,14 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
       d.rows_in = (in.stride == 1 && out.stride > 1) ? 1 : 0;
       d.rows_out = (out.stride == 1 && in.stride > 1 && d.tw_period == 0) ? 1 : 0;
     }
+  }
+  if (real_io) {
+    // row r of a group at base(g*C) + r*row stride on both sides
+    d.real_io = real_io;
+    d.rows_in = d.rows_out = 0;
+    d.in_cdelta = in.s_out;
+    d.out_cdelta = out.s_out;
+    d.xcd_remap = 0;
   }
   if (pt && pt->xcd_remap > 0) d.xcd_remap = pt->xcd_remap == 2 ? 1 : 0;
   int ns = 1, off = 0;
@@ -863,7 +897,12 @@ static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch
 // transform of the reals read as n/2 complex points (real.jl:88-131); odd n:
 // the full complex transform of the widened reals, prefix kept.
 static void add_r2c_rows(Layout& L, int64_t n, int64_t batch, int src, int dst, const std::vector<int>& free) {
-  if (n % 2 == 0) {
+  if (n % 2 == 0 && !L.mat && real_fused_C(L.dtype, n / 2, batch, nullptr)) {
+    // one launch: the reals read as n/2 complex points, the half-length
+    // transform, the twist at the store
+    const int64_t h = n / 2;
+    add_pass(L, -1, static_cast<int>(h), batch, 1, geo(1, 0, h), geo(1, 0, h + 1), 1.0, src, dst, PassTw(), 1, 1);
+  } else if (n % 2 == 0) {
     const int64_t h = n / 2;
     need_scratch(L, free[0], h * batch);
     build_c2c_rows(L, h, batch, -1, 1.0, src, free[0], std::vector<int>(free.begin() + 1, free.end()));
@@ -906,7 +945,14 @@ static void add_r2c_rows(Layout& L, int64_t n, int64_t batch, int src, int dst, 
 // `free` behind the roles that step writes.
 static void add_c2r_rows(Layout& L, int64_t n, int64_t batch, bool normalized, int src, int dst,
                          const std::vector<int>& free) {
-  if (n % 2 == 0) {
+  if (n % 2 == 0 && !L.mat && real_fused_C(L.dtype, n / 2, batch, nullptr)) {
+    // one launch: the pre-pass at the load, the inverse half-length
+    // transform, the reals stored as n/2 complex points; the scale carries
+    // the pre-factor (1/2 * 1/h = 1/n for irfft, 1 for brfft)
+    const int64_t h = n / 2;
+    add_pass(L, 1, static_cast<int>(h), batch, 1, geo(1, 0, h + 1), geo(1, 0, h),
+             normalized ? 1.0 / static_cast<double>(n) : 1.0, src, dst, PassTw(), 1, 2);
+  } else if (n % 2 == 0) {
     const int64_t h = n / 2;
     need_scratch(L, free[0], h * batch);
     Step p;
@@ -1301,6 +1347,7 @@ static std::string describe(const Layout& L) {
       for (size_t i = 0; i < s.radices.size(); ++i) o << (i ? "," : "") << s.radices[i];
       if (s.pow2) o << " kernel=pow2";
       if (s.mat) o << (s.mat == kMatF16 ? " kernel=mfma_f16" : s.mat == kMatBF16 ? " kernel=mfma_bf16" : " kernel=mfma_bf16x3");
+      if (s.d.real_io) o << (s.d.real_io == 1 ? " real=r2c" : " real=c2r");
       if (s.d.rows_in) o << " rows_in";
       if (s.d.rows_out) o << " rows_out";
       o << " batch=" << s.d.batch << " C=" << s.d.C << " threads=" << s.d.nthreads << " groups=" << s.d.ngroups
@@ -1403,6 +1450,13 @@ static void materialize(fftjh_plan_s& P) {
         const int mode = s.mat;
         s.mat_tab = cached(mk.str(), mat_table_bytes(mode),
                            [&](void* p) { hip_check(launch_mat_table(mode, dir, p, 0), "operand table kernel"); });
+      }
+      if (s.d.real_io) {
+        // w_k = exp(dir*2*pi*i*k/n), k = 0..h, n = 2h (real.jl:26-39)
+        const int64_t h = s.d.n, n = 2 * h;
+        s.aux = cached(key_of("real", {dev, double(dt), double(dir), double(n)}), (h + 1) * es,
+                       [&](void* p) { hip_check(launch_tw_pow(dt, p, h + 1, 1, n, dir, 0), "twiddle kernel"); });
+        s.d.rtw = s.aux->p;
       }
       s.d.grid = resident > 0 ? std::min(resident, s.d.ngroups) : s.d.ngroups;
       s.desc = std::make_shared<DevBuf>(sizeof(PassDesc));

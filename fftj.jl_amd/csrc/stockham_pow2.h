@@ -128,9 +128,16 @@ __device__ __forceinline__ void middle_stages(V2<T> (&v)[kE], V2<T>* lds, const 
 
 // IO: 0 = registers on both HBM sides, 1 = rows staged through LDS on the
 // way in, 2 = on the way out (separate instantiations keep the plain
-// kernel's register allocation small)
+// kernel's register allocation small); 3 = R2C rows (register I/O, the
+// real twist at the store), 4 = C2R rows (the real pre-pass at the load)
+#ifndef FFTJH_P2_WPE
+#define FFTJH_P2_WPE 0
+#endif
 template <typename T, int DIR, int N, int R1, int R2, int R3, int R4, int IO>
 __global__ void __launch_bounds__(MaxThreadsP2<T>::value)
+#if FFTJH_P2_WPE > 0
+__attribute__((amdgpu_waves_per_eu(FFTJH_P2_WPE)))
+#endif
 k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<T>* tw) {
   using Ch = Radices<R1, R2, R3, R4>;
   constexpr int NST = Ch::count;
@@ -197,6 +204,33 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     }
   };
 
+  // C2R edge (IO 4, real.jl:149-193): thread (c, jt) forms its 16 inputs
+  // Z_j, j = jt + u*J, of row b from the bins X_j and X_{N-j} (both read
+  // from HBM; the second read of a row hits L2) in registers:
+  //   Z_j = xe + i*(w_j*xd),  xe, xd = X_j +- conj(X_{N-j}),  w_j = e^{+2 pi i j/2N}
+  // (irfft's pre-factor 1/2 is a power of two and rides, exactly, on the
+  // store scale)
+  auto load_pre = [&](int g, V2<T> (&dst)[kE]) {
+    const int64_t b = static_cast<int64_t>(g) * C + c;
+    if (b < d.batch) {
+      const V2<T>* xr = x + BatchIndex(d, b).base(d.in);
+      const V2<T>* w = static_cast<const V2<T>*>(d.rtw);
+#pragma unroll
+      for (int u = 0; u < kE; ++u) {
+        const int j = jt + u * J;
+        const V2<T> a = xr[j];
+        const V2<T> xc = xr[N - j];
+        const V2<T> cx = mk<T>(xc.x, -xc.y);
+        const V2<T> xe = cadd<T>(a, cx);
+        const V2<T> t = cmul<T>(csub<T>(a, cx), w[j]);
+        dst[u] = mk<T>(xe.x - t.y, xe.y + t.x);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kE; ++u) dst[u] = mk<T>(T(0), T(0));
+    }
+  };
+
   int g = walk.start;
   if (g >= walk.end) return;
   if constexpr (IO == 1) {
@@ -204,6 +238,8 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     __syncthreads();
     stage_gather<T, N, R1>(v, lds, C, c, jt, J);
     __syncthreads();
+  } else if constexpr (IO == 4) {
+    load_pre(g, v);
   } else {
     load(g, v);
   }
@@ -226,7 +262,7 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
       const BatchIndex ix(d, b);
       // the multi-pass twiddle's base and step, formed while v is dead
       PostTwiddle pt;
-      if (IO != 2 && d.tw_period) pt.init(d, ix, jt, J);
+      if (IO <= 1 && d.tw_period) pt.init(d, ix, jt, J);
       stage_gather<T, N, R>(v, lds, C, c, jt, J);
       __syncthreads();
       stage_twiddle<T, R, NS>(v, tw + OFF, jt, J);
@@ -236,6 +272,35 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
         stage_scatter<T, R, N / R>(v, lds, C, c, jt, J);
         __syncthreads();
         store_rows(g);
+        __syncthreads();
+      } else if constexpr (IO == 3) {
+        // R2C edge (real.jl:88-131): v[u] = Z_k, k = jt + u*J; the partner
+        // bins Z_{N-k} come through LDS (same column, lanes contiguous)
+        //   y_0, y_N = Re Z_0 +- Im Z_0,  y_k = xe + w_k*(-i*xo),
+        //   xe, xo = (Z_k +- conj(Z_{N-k}))/2,  w_k = e^{-2 pi i k/2N}
+#pragma unroll
+        for (int u = 0; u < kE; ++u) lds[lpad2((jt + u * J) * C + c)] = v[u];
+        __syncthreads();
+        if (b < d.batch) {
+          V2<T>* yr = y + ix.base(d.out);
+          const V2<T>* w = static_cast<const V2<T>*>(d.rtw);
+#pragma unroll
+          for (int u = 0; u < kE; ++u) {
+            const int k = jt + u * J;
+            const V2<T> z = v[u];
+            if (u == 0 && jt == 0) {
+              yr[0] = mk<T>(z.x + z.y, T(0));
+              yr[N] = mk<T>(z.x - z.y, T(0));
+            } else {
+              const V2<T> zc = lds[lpad2((N - k) * C + c)];
+              const V2<T> cz = mk<T>(zc.x, -zc.y);
+              const T half = T(0.5);
+              const V2<T> xe = cscale<T>(cadd<T>(z, cz), half);
+              const V2<T> xo = cscale<T>(csub<T>(z, cz), half);
+              yr[k] = cadd<T>(xe, cmul<T>(mk<T>(xo.y, -xo.x), w[k]));
+            }
+          }
+        }
         __syncthreads();
       } else {
        if (b < d.batch) {
@@ -268,6 +333,8 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
       __syncthreads();
       stage_gather<T, N, R1>(v, lds, C, c, jt, J);
       __syncthreads();
+    } else if constexpr (IO == 4) {
+      load_pre(gn, v);
     } else {
       load(gn, v);
     }
@@ -296,6 +363,15 @@ hipError_t launch_p2_io(const PassDesc& d, const PassDesc* dd, const void* x, vo
 template <typename T, int DIR, int N, int R1, int R2, int R3, int R4>
 hipError_t launch_p2(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
                      hipStream_t st) {
+  if (d.real_io) {
+    // the real edges exist in one direction each: R2C forward, C2R inverse
+    if constexpr (DIR == -1) {
+      if (d.real_io == 1) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 3>(d, dd, x, y, tw, st);
+    } else {
+      if (d.real_io == 2) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 4>(d, dd, x, y, tw, st);
+    }
+    return hipErrorInvalidValue;
+  }
   if (d.rows_in) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 1>(d, dd, x, y, tw, st);
   if (d.rows_out) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 2>(d, dd, x, y, tw, st);
   return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 0>(d, dd, x, y, tw, st);
@@ -317,6 +393,14 @@ int resident_p2_io(const PassDesc& d) {
 
 template <typename T, int DIR, int N, int R1, int R2, int R3, int R4>
 int resident_p2(const PassDesc& d) {
+  if (d.real_io) {
+    if constexpr (DIR == -1) {
+      if (d.real_io == 1) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 3>(d);
+    } else {
+      if (d.real_io == 2) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 4>(d);
+    }
+    return 0;
+  }
   if (d.rows_in) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 1>(d);
   if (d.rows_out) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 2>(d);
   return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 0>(d);

@@ -60,6 +60,33 @@ def test_rfft_irfft(prec, n):
     assert rel_l2(yb, O.irfft(ref, n, normalized=False)) <= tol_for(prec, n)
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [64, 128, 512, 2048, 8192, 16384])
+@pytest.mark.parametrize("batch", [1, 3, 37])
+def test_real_rows_batched(prec, n, batch):
+    # batched rows: power-of-two half lengths take the one-launch kernel with
+    # the real edges fused (partial last groups included); the rest the
+    # two-launch path
+    T = torch_dtype(prec)
+    x = F.rng.normal(n + batch, n * batch).astype(real_np(prec)).reshape(batch, n)
+    p = F.plan_rfft(T, n, batch=batch)
+    Xd = p(to_dev(x))
+    X = to_host(Xd).reshape(batch, n // 2 + 1)
+    ref = O.rfft(x.astype(np.float64))
+    assert rel_l2(X, ref) <= tol_for(prec, n)
+    for b in (0, batch - 1):
+        assert rel_l2(X[b], ref[b]) <= tol_for(prec, n)
+    back = to_host(F.plan_irfft(T, n, batch=batch)(Xd)).reshape(batch, n)
+    assert rel_l2(back, x) <= 2 * tol_for(prec, n)
+    # brfft of an arbitrary spectrum (imaginary DC / Nyquist parts enter the
+    # even-length pre-pass, as in the reference)
+    Z = F.rng.uniform_complex(n, (n // 2 + 1) * batch).astype(cplx_np(prec)).reshape(batch, n // 2 + 1)
+    yb = to_host(F.plan_brfft(T, n, batch=batch)(to_dev(Z))).reshape(batch, n)
+    assert rel_l2(yb, O.irfft(Z.astype(np.complex128), n, normalized=False)) <= tol_for(prec, n)
+    if "real=" in p.describe():
+        assert p.describe().count("\n  ") == 1
+
+
 def test_config3_rfft_2pow22_x4():
     n, batch = 1 << 22, 4
     x = F.rng.normal(0, n * batch).astype(np.float32).reshape(batch, n)

@@ -105,12 +105,18 @@ def test_nd_real_schedules():
     """N-d real plans: real rows along dims[0] then in-place complex passes
     of the other dims (R2C); complex passes into scratch then the real rows
     (C2R), irfft scale split 1/prod(dims[1:]) and 1/dims[0]."""
+    # power-of-two half length: one launch for the rows, real edges fused
     s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F32, [64, 48]))
-    assert s[0].startswith("pass x->s0") and "n=32 " in s[0] and s[1].startswith("r2c_twist s0->y")
-    assert s[2].startswith("pass y->y") and "n=48 " in s[2] and "in_stride=33 " in s[2]
+    assert s[0].startswith("pass x->y") and "n=32 " in s[0] and "real=r2c" in s[0]
+    assert s[1].startswith("pass y->y") and "n=48 " in s[1] and "in_stride=33 " in s[1] and len(s) == 2
     s = _passes(_lib.describe_request(_lib.KIND_C2R, _lib.F32, [64, 48]))
     assert s[0].startswith("pass x->s0") and "n=48 " in s[0] and "scale=0.0208333" in s[0]
-    assert s[1].startswith("c2r_pre s0->s1") and s[2].startswith("pass s1->y") and "scale=0.03125" in s[2]
+    assert s[1].startswith("pass s0->y") and "real=c2r" in s[1] and "scale=0.015625" in s[1] and len(s) == 2
+    # other even lengths: the half-length transform, then the twist
+    s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F32, [96, 48]))
+    assert s[0].startswith("pass x->s0") and "n=48 " in s[0] and s[1].startswith("r2c_twist s0->y")
+    s = _passes(_lib.describe_request(_lib.KIND_C2R, _lib.F32, [96, 48]))
+    assert s[1].startswith("c2r_pre s0->s1") and s[2].startswith("pass s1->y") and "scale=0.0208333" in s[2]
     # odd leading length, three dims
     s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F64, [9, 4, 3]))
     assert [p.split()[0] for p in s] == ["real_to_complex", "pass", "copy_prefix", "pass", "pass"]
@@ -119,6 +125,8 @@ def test_nd_real_schedules():
     # every dim past the first is 1: the rows alone, straight from x
     s = _passes(_lib.describe_request(_lib.KIND_C2R, _lib.F32, [8, 1, 1]))
     assert s[0].startswith("c2r_pre x->s0") and len(s) == 2
+    s = _passes(_lib.describe_request(_lib.KIND_C2R, _lib.F32, [64, 1, 1]))
+    assert s[0].startswith("pass x->y") and "real=c2r" in s[0] and len(s) == 1
     with pytest.raises(F.ArgumentError, match="positive"):
         _lib.describe_request(_lib.KIND_R2C, _lib.F32, [8, 0])
 
