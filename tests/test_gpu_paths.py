@@ -103,7 +103,8 @@ def test_config3_rfft_2pow22_x4():
 # ------------------------------------------------------------------- N-d (A11)
 
 SHAPES = [(64, 48), (256, 256), (33, 17), (16, 11), (11, 16), (8, 1), (1, 8), (128, 3, 5), (16, 16, 16),
-          (12, 10, 14), (32, 64), (60, 50), (24, 40, 3), (1024, 512), (4, 2**12), (2**12, 4), (256, 33, 17), (3, 2**17), (2**17, 2), (19, 2**15)]
+          (12, 10, 14), (32, 64), (60, 50), (24, 40, 3), (1024, 512), (4, 2**12), (2**12, 4), (256, 33, 17), (3, 2**17), (2**17, 2), (19, 2**15),
+          (64, 4096), (3, 8192), (16, 5, 4096), (8, 6000), (7, 12288), (4096, 4096)]
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])

@@ -93,9 +93,18 @@ def test_schedule_config3():
 
 
 def test_nd_long_and_nonsmooth_axes():
-    # a long strided axis is gathered into rows, transformed, scattered back
+    # a long strided axis runs the multi-pass schedule on its column block
+    # directly (offsets scaled by the column count), no gather / scatter
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [2, 1 << 20]))
-    assert s[1].startswith("strided_copy") and s[-1].startswith("strided_copy")
+    assert len(s) == 4 and not any(p.startswith("strided_copy") for p in s)
+    assert s[1].startswith("pass y->y") and "in_stride=16384 " in s[1] and "C=16 " in s[1]
+    assert s[2].startswith("pass y->s0") and s[3].startswith("pass s0->y") and "out_stride=32768" in s[3]
+    # a strided axis of 4096 or more splits into 16-column passes
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [4096, 4096]))
+    assert len(s) == 3 and "n=64 " in s[1] and "C=16 " in s[1] and "in_stride=262144 " in s[1]
+    assert "in_stride=4096 " in s[2] and "out_stride=262144" in s[2]
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F64, [1024, 2048]))
+    assert len(s) == 2
     # a non-smooth axis resamples with strided deconvolution / interpolation
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [33, 17]))
     assert s[1].startswith("rs_deconv") and s[-1].startswith("rs_interp")
