@@ -44,6 +44,8 @@ def parse():
                     help="measure the device's streaming copy rate at the 2^24 footprint and exit")
     ap.add_argument("--split-only", action="store_true",
                     help="run only the split-precision leg (one JSON line; for kernel traces)")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="run only the configurations leg (one JSON line; for kernel traces)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 3 forward plans of the workload, then 3 copies of the same bytes "
                          "(the calibration kernel of known traffic); no timing, no JSON")
@@ -319,6 +321,9 @@ def main():
     F = __graft_entry__.load_package()
     if args.split_only:
         print(json.dumps({"split_precision": split_precision_leg(F)}), flush=True)
+        return
+    if args.configs_only:
+        print(json.dumps({"configs": configs_leg(F)}), flush=True)
         return
     n, batch, desc = WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
