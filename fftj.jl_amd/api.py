@@ -389,8 +389,19 @@ def _precision_flags(precision):
     return _lib.PRECISION_FLAGS[precision]
 
 
-def _plan_c2c(T_or_x, n, sign, scale_kind, region, batch, planner=None, precision=None):
-    flags = _precision_flags(precision)
+def _route_flags(route):
+    """route=None (the planner's choice) or, for a length that is not
+    13-smooth, "composite" (one exact step over its smooth part around the
+    resampled rest, composite.jl) or "resample" (the whole length resampled)."""
+    if route is None:
+        return 0
+    if route not in _lib.ROUTE_FLAGS:
+        raise ArgumentError(f"route must be None, 'composite' or 'resample' (got {route!r})")
+    return _lib.ROUTE_FLAGS[route]
+
+
+def _plan_c2c(T_or_x, n, sign, scale_kind, region, batch, planner=None, precision=None, route=None):
+    flags = _precision_flags(precision) | _route_flags(route)
     if isinstance(T_or_x, torch.Tensor):
         x = T_or_x
         if not x.is_complex():
@@ -429,21 +440,24 @@ def _planner_tuning(planner, code, n, batch, sign):
     raise ArgumentError(f"planner must be None, 'heuristic', 'learned' or 'exhaustive' (got {planner!r})")
 
 
-def plan_fft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None, **flags):
+def plan_fft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None,
+              route=None, **flags):
     """plan_fft(T, n) (plan.jl:207-208) or plan_fft(x[, region]) (api.jl:107-112).
     planner= picks the schedule search for 1-D plans of a given length;
     precision= selects split-precision MFMA stages for FP32 plans."""
-    return _plan_c2c(T_or_x, n, -1, "one", region, batch, planner, precision)
+    return _plan_c2c(T_or_x, n, -1, "one", region, batch, planner, precision, route)
 
 
-def plan_ifft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None, **flags):
+def plan_ifft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None,
+               route=None, **flags):
     """plan_ifft (plan.jl:209-210, api.jl:119-124): inverse scaled by 1/n."""
-    return _plan_c2c(T_or_x, n, +1, "inv", region, batch, planner, precision)
+    return _plan_c2c(T_or_x, n, +1, "inv", region, batch, planner, precision, route)
 
 
-def plan_bfft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None, **flags):
+def plan_bfft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None,
+               route=None, **flags):
     """plan_bfft (plan.jl:212-213, api.jl:113-118): unnormalized inverse."""
-    return _plan_c2c(T_or_x, n, +1, "one", region, batch, planner, precision)
+    return _plan_c2c(T_or_x, n, +1, "one", region, batch, planner, precision, route)
 
 
 def plan_fft_(T_or_x, n=None, region=None, **kw):

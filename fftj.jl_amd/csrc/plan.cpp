@@ -276,6 +276,7 @@ struct Layout {
   fftjh_tuning tune{};  // search-space overrides (zero = planner's choice)
   int npass = 0;        // pass launches laid out so far (index into tune.pass)
   int mat = kMatNone;   // split-precision MFMA mode of every pass (FP32 C2C)
+  unsigned route = 0;   // FFTJH_FLAG_ROUTE_* for non-smooth lengths (0 = planner)
   int dtype = 0, kind = FFTJH_KIND_C2C, sign = -1, rank = 1;
   int64_t dims[8] = {0};
   int64_t batch = 1;
@@ -702,6 +703,24 @@ static bool split_column(int dtype, int64_t n) {
   return n >= 4096;
 }
 
+// Composite route for a non-smooth length n with 13-smooth part s > 1 (the
+// caller checks that one pass holds s): the route flag decides, else the
+// measured rule.  The composite route pays one extra pass (the length-s
+// step) and gains when the resampled length-p transforms fit one pass where
+// the whole length's would not.  Measured on one MI355X (composite / full
+// resampling time, F32 and F64): 48*1009 0.87 / 0.92, 100*1009 0.89 / 0.94,
+// 1000*1009 0.77 / 1.07, 1024*1031 0.96 / 1.01, 256*4099 1.02 / 0.90; it
+// loses with a small smooth part (2*10007 1.31, 16*10007 1.11), a length-p
+// step of more than one pass (64*10007 1.09), a tiny p (32*17 1.95) or a
+// step too long for 16 columns per workgroup (4096*17 1.78).
+static bool use_composite(const Layout& L, int64_t n, int64_t s) {
+  if (L.route & FFTJH_FLAG_ROUTE_COMPOSITE) return true;
+  if (L.route & FFTJH_FLAG_ROUTE_RESAMPLE) return false;
+  const int64_t p = n / s;
+  if (s < 32 || s > 1024 || p < 512) return false;
+  return single_pass_ok(L.dtype, rs_params(L.dtype, p).m) && !single_pass_ok(L.dtype, rs_params(L.dtype, n).m);
+}
+
 static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry gin, Geometry gout, int dir,
                       double scale, int src, int dst, std::vector<int> free) {
   const bool split_asked = L.tune.splits[0] > 0 && L.npass == 0;
@@ -822,7 +841,30 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
     }
     return;
   }
-  // not 13-smooth: resample onto a smooth m (resample.jl:258-276)
+  // not 13-smooth, n = p*s with s its 13-smooth part: one exact
+  // Cooley-Tukey step around the resampled length p (composite.jl:1-101),
+  //   X[k2 + s*k1] = sum_j1 w_p^(j1 k1) w_n^(j1 k2) sum_j2 w_s^(j2 k2) x[j1 + p*j2]:
+  // a pass of p length-s transforms at stride p with the twiddle w_n^(j1 k2)
+  // at its store, in place into dst, then the s length-p transforms over j1
+  // resampled from dst (rows k2) back into dst at stride s (outputs k1)
+  {
+    int64_t sm = 1;
+    for (int f : {2, 3, 5, 7, 11, 13})
+      while ((n / sm) % f == 0) sm *= f;
+    const int64_t p = n / sm;
+    const bool ok = sm > 1 && is_rows(gin, n, Bi) && is_rows(gout, n, Bi) && single_pass_ok(L.dtype, sm);
+    if (ok && use_composite(L, n, sm)) {
+      PassTw t;
+      t.period = n;
+      t.qdiv = 1;
+      t.acoef = 0;
+      t.bcoef = 1;
+      add_pass(L, dir, static_cast<int>(sm), batch * p, p, geo(p, 1, n), geo(p, 1, n), 1.0, src, dst, t);
+      build_c2c(L, p, batch * sm, sm, geo(1, p, n), geo(sm, 1, n), dir, scale, dst, dst, free);
+      return;
+    }
+  }
+  // otherwise resample onto a smooth m (resample.jl:258-276)
   if (free.size() < 2) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for resampling");
   const RsParams rp = rs_params(L.dtype, n);
   const int g = free[0];
@@ -869,9 +911,13 @@ static void check_dtype(int dtype) {
 }
 
 // split-precision mode from plan flags (at most one FFTJH_FLAG_MFMA_* bit; FP32 only)
+static const unsigned kRouteFlags = FFTJH_FLAG_ROUTE_COMPOSITE | FFTJH_FLAG_ROUTE_RESAMPLE;
+
 static int mat_mode_of(unsigned flags, int dtype) {
   const unsigned known = FFTJH_FLAG_MFMA_F16 | FFTJH_FLAG_MFMA_BF16 | FFTJH_FLAG_MFMA_BF16X3;
-  if (flags & ~known) fail(FFTJH_ERR_ARGUMENT, "unknown plan flags");
+  if (flags & ~(known | kRouteFlags)) fail(FFTJH_ERR_ARGUMENT, "unknown plan flags");
+  if ((flags & kRouteFlags) == kRouteFlags) fail(FFTJH_ERR_ARGUMENT, "at most one route flag");
+  flags &= known;
   if (!flags) return kMatNone;
   if (flags & (flags - 1)) fail(FFTJH_ERR_ARGUMENT, "at most one split-precision flag");
   if (dtype != FFTJH_F32) fail(FFTJH_ERR_ARGUMENT, "split-precision MFMA stages are for FP32 plans");
@@ -895,6 +941,7 @@ static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch
   Layout L;
   if (tune) L.tune = *tune;
   L.mat = mat_mode_of(flags | (tune ? tune->flags : 0u), dtype);
+  L.route = (flags | (tune ? tune->flags : 0u)) & kRouteFlags;
   L.dtype = dtype;
   L.kind = FFTJH_KIND_C2C;
   L.sign = sign;

@@ -57,6 +57,17 @@ extern "C" {
 #define FFTJH_FLAG_MFMA_BF16 0x2u
 #define FFTJH_FLAG_MFMA_BF16X3 0x4u
 
+/* Route flags for a 1-D length that is not 13-smooth, n = p * s with s its
+ * 13-smooth part (at most one; 0 = the planner's choice):
+ *   FFTJH_FLAG_ROUTE_COMPOSITE  one exact Cooley-Tukey step over s, then the
+ *                               length-p transforms resampled (the
+ *                               reference's CompositePlan, composite.jl:1-101);
+ *                               applies when s > 1 and one pass holds s
+ *   FFTJH_FLAG_ROUTE_RESAMPLE   the whole length resampled onto a smooth m
+ *                               (ResamplePlan, resample.jl:258-320) */
+#define FFTJH_FLAG_ROUTE_COMPOSITE 0x10u
+#define FFTJH_FLAG_ROUTE_RESAMPLE 0x20u
+
 typedef struct fftjh_plan_s* fftjh_plan;
 /* hipStream_t, passed as an opaque pointer (NULL = the null stream) */
 typedef void* fftjh_stream;
@@ -68,8 +79,9 @@ typedef void* fftjh_stream;
  * (all dims transformed, dims[0] fastest).  batch > 1 runs that many
  * independent transforms stored back to back (an extension: the reference
  * has no batch API and loops over 1-D plans).  flags: 0, or one
- * FFTJH_FLAG_MFMA_* bit (the reference accepts and ignores FFTW planner
- * flags, api.jl:107-108; these are this library's own). */
+ * FFTJH_FLAG_MFMA_* bit and/or one FFTJH_FLAG_ROUTE_* bit (the reference
+ * accepts and ignores FFTW planner flags, api.jl:107-108; these are this
+ * library's own). */
 int fftjh_plan_c2c(fftjh_plan* plan, int dtype, int rank, const int64_t* dims, int64_t batch, int sign,
                    double scale, unsigned flags);
 

@@ -194,3 +194,41 @@ def test_config3_nd_256cubed():
     y = to_host(F.fft(xd))
     ref = O.ndfft(x.astype(np.complex128).ravel(), list(dims)).reshape(dims)
     assert rel_l2(y, ref) <= 200 * np.finfo(np.float32).eps
+
+
+# ------------------------------------------------- composite route (F3)
+
+COMPOSITE_N = [2 * 10007, 6 * 1009, 64 * 10007, 17 * 4096, 48 * 1009, 32 * 17, 12 * 65537, 2 * 1000003]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", COMPOSITE_N)
+def test_composite_route(prec, n):
+    # n = p*s, s the 13-smooth part: one exact step over s around the
+    # resampled length p (composite.jl:1-101), against the oracle, with both
+    # routes agreeing
+    batch = 2
+    x = inputs_c(n + 1, n * batch, prec).reshape(batch, n)
+    T = torch_dtype(prec)
+    p = F.plan_fft(T, n, batch=batch, route="composite")
+    d = p.describe()
+    assert d.splitlines()[1].strip().startswith("pass x->y") and "rs_deconv" in d
+    y = to_host(p(to_dev(x))).reshape(batch, n)
+    ref = O.fft(x.astype(np.complex128))
+    assert rel_l2(y, ref) <= tol_for(prec, n)
+    assert rel_l2(y[1], ref[1]) <= tol_for(prec, n)
+    yr = to_host(F.plan_fft(T, n, batch=batch, route="resample")(to_dev(x))).reshape(batch, n)
+    assert rel_l2(y, yr) <= 2 * tol_for(prec, n)
+    back = to_host(F.plan_ifft(T, n, batch=batch, route="composite")(to_dev(y.astype(x.dtype)))).reshape(batch, n)
+    assert rel_l2(back, x) <= 2 * tol_for(prec, n)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [48 * 1009, 100 * 1009, 1000 * 1009, 1024 * 1031])
+def test_default_route_nonsmooth(prec, n):
+    # lengths the planner sends down the composite route by default
+    x = inputs_c(n + 3, n, prec)
+    p = F.plan_fft(torch_dtype(prec), n)
+    assert p.describe().splitlines()[1].strip().startswith("pass x->y")
+    y = to_host(p(to_dev(x)))
+    assert rel_l2(y, O.fft(x.astype(np.complex128))) <= tol_for(prec, n)

@@ -140,6 +140,32 @@ def test_nd_real_schedules():
         _lib.describe_request(_lib.KIND_R2C, _lib.F32, [8, 0])
 
 
+def test_route_flags():
+    """Non-smooth lengths: the composite route is one pass over the smooth
+    part (twiddle w_n at its store, in place into y) and the resampled rest;
+    the resample route resamples the whole length; at most one route bit."""
+    L = _lib
+    n = 64 * 10007
+    s = _passes(L.describe_tuned(L.F32, (n,), 1, -1, L.make_tuning((), (), L.FLAG_ROUTE_COMPOSITE)))
+    assert s[0].startswith("pass x->y n=64 ") and "in_stride=10007 " in s[0] and f"twiddle=w_{n}" in s[0]
+    assert s[1].startswith("rs_deconv y->") and "n=10007 " in s[1] and s[-1].startswith("rs_interp") and "->y" in s[-1]
+    s = _passes(L.describe_tuned(L.F32, (n,), 1, -1, L.make_tuning((), (), L.FLAG_ROUTE_RESAMPLE)))
+    assert s[0].startswith("rs_deconv x->") and f"n={n} " in s[0]
+    # the planner's own choice: composite where the resampled length-p
+    # transforms fit one pass and the whole length's would not
+    s = _passes(L.describe_request(L.KIND_C2C, L.F32, [100 * 1009]))
+    assert s[0].startswith("pass x->y n=100 ") and s[1].startswith("rs_deconv")
+    s = _passes(L.describe_request(L.KIND_C2C, L.F32, [2 * 10007]))
+    assert s[0].startswith("rs_deconv x->")
+    # a prime has no smooth part: both routes resample it
+    s = _passes(L.describe_tuned(L.F32, (65537,), 1, -1, L.make_tuning((), (), L.FLAG_ROUTE_COMPOSITE)))
+    assert s[0].startswith("rs_deconv")
+    with pytest.raises(L.ArgumentError):
+        L.describe_tuned(L.F32, (n,), 1, -1, L.make_tuning((), (), L.FLAG_ROUTE_COMPOSITE | L.FLAG_ROUTE_RESAMPLE))
+    with pytest.raises(F.ArgumentError):
+        F.plan_fft("f32", n, route="bluestein")
+
+
 def test_region_check():
     import torch
     x = torch.zeros(4, 4, dtype=torch.complex64)
