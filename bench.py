@@ -46,6 +46,9 @@ def parse():
                     help="run only the split-precision leg (one JSON line; for kernel traces)")
     ap.add_argument("--configs-only", action="store_true",
                     help="run only the configurations leg (one JSON line; for kernel traces)")
+    ap.add_argument("--cpu-configs", action="store_true",
+                    help="time every configuration on the host's cores through the oracle's restatements "
+                         "of the reference's routes (one JSON line; no GPU)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 3 forward plans of the workload, then 3 copies of the same bytes "
                          "(the calibration kernel of known traffic); no timing, no JSON")
@@ -137,6 +140,84 @@ def cpu_baseline(n, batch, target_s):
         "sample": f"{reps} forward+inverse steps of the same workload on {threads} threads "
                   f"(oracle FP32 port of FFTJ's threaded {'FourStepPlan' if batch == 1 else 'per-transform FFTPlan loop'})",
     }
+
+
+def cpu_configs_leg(F, per_config_s=3.0):
+    """Every configuration of the configurations leg, forward, through the
+    oracle's restatements of the reference's routes on this host's cores
+    (bounded samples: about per_config_s seconds each).  FP32 arithmetic
+    except the FP64 configuration; threads as the reference threads them
+    (batches across cores, the four-step threaded, the N-d plans serial in
+    this restatement)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    from oracle import oracle as O
+
+    threads = min(os.cpu_count() or 1, 16)
+    pool = ThreadPoolExecutor(threads)
+    rng = F.rng
+
+    def rows(fn, X):
+        # one transform per task: the C calls release the GIL
+        return list(pool.map(fn, list(X)))
+
+    c64 = lambda seed, count: rng.uniform_complex(seed, count).astype(np.complex64)  # noqa: E731
+    r32 = lambda seed, count: rng.normal(seed, count).astype(np.float32)  # noqa: E731
+    cases = []
+    x1024 = rng.uniform_complex(0, 1024)
+    cases.append(("c2c_f64_1024", lambda: O.fft(x1024, prec="f64"), 1, "FFTPlan chain, serial"))
+    xb = c64(0, 4096 * 16384).reshape(16384, 4096)
+    cases.append(("c2c_f32_4096x16384", lambda: O.baseline_fft_batch(xb, prec="f32", nthreads=threads), threads,
+                  "loop of 1-D FFTPlans across cores"))
+    x24 = c64(0, 1 << 24)
+    cases.append(("c2c_f32_2pow24", lambda: O.fourstep(x24, 8192, 2048, prec="f32", nthreads=threads), threads,
+                  "FourStepPlan 8192x2048, threaded"))
+    x6 = c64(0, 8 * 10 ** 6).reshape(8, 10 ** 6)
+    cases.append(("c2c_f32_1e6x8", lambda: O.baseline_fft_batch(x6, prec="f32", nthreads=min(8, threads)),
+                  min(8, threads), "1-D plan per transform across cores"))
+    for n in (65537, 1000003):
+        xp = c64(0, 4 * n).reshape(4, n)
+        cases.append((f"c2c_f32_{n}x4", (lambda xp=xp: rows(lambda r: O.resample(r, prec="f32"), xp)), 4,
+                      "ResamplePlan per transform across cores (the reference routes these primes to HvdhPlan)"))
+    xr = r32(0, 4 << 22).reshape(4, 1 << 22)
+    cases.append(("r2c_f32_2pow22x4", lambda: O.baseline_rfft_batch(xr, prec="f32", nthreads=4), 4,
+                  "RfftPlan per transform across cores, inner 2^21 as one chain (the reference runs it "
+                  "as a threaded FourStepPlan)"))
+    Xr = O.baseline_rfft_batch(xr, prec="f32", nthreads=4)
+    cases.append(("c2r_f32_2pow22x4", lambda: O.baseline_irfft_batch(Xr, 1 << 22, prec="f32", nthreads=4), 4,
+                  "IrfftPlan per transform across cores, inner 2^21 as one chain"))
+    x3 = c64(0, 1 << 24)
+    cases.append(("c2c_f32_256cubed", lambda: O.baseline_ndfft(x3, [256] * 3, prec="f32", nthreads=threads),
+                  threads, "NdPlan axis passes, pencils across cores"))
+    xr3 = r32(0, 1 << 24).reshape(65536, 256)
+
+    def r2c3():
+        Y = O.baseline_rfft_batch(xr3, prec="f32", nthreads=threads)
+        return O.baseline_ndfft(Y, [129, 256, 256], prec="f32", nthreads=threads, first=1)
+
+    cases.append(("r2c_f32_256cubed", r2c3, threads, "RfftNdPlan: real rows, then axis passes, across cores"))
+    X3 = r2c3()
+
+    def c2r3():
+        Z = O.baseline_ndfft(X3, [129, 256, 256], sign=1, scale=1.0 / 65536, prec="f32", nthreads=threads, first=1)
+        return O.baseline_irfft_batch(Z.reshape(65536, 129), 256, prec="f32", nthreads=threads)
+
+    cases.append(("c2r_f32_256cubed", c2r3, threads, "IrfftNdPlan: axis passes, then real rows, across cores"))
+    out = []
+    for name, fn, cores, route in cases:
+        t0 = time.perf_counter()
+        fn()
+        first = time.perf_counter() - t0
+        reps = max(1, min(20, int(per_config_s / max(first, 1e-4))))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        out.append({"config": name, "ms": round(ms, 3), "cores": cores, "reps": reps, "route": route})
+        print(json.dumps(out[-1]), file=sys.stderr, flush=True)
+    pool.shutdown()
+    return out
 
 
 def ragged_leg(F, quick=False):
@@ -309,6 +390,10 @@ def main():
         return
     if args.copy_probe:
         copy_probe()
+        return
+    if args.cpu_configs:
+        import __graft_entry__
+        print(json.dumps({"cpu_configs": cpu_configs_leg(__graft_entry__.load_package())}), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "

@@ -43,6 +43,10 @@ def lib():
             getattr(L, f"oracle_ndfft_{p}").argtypes = [ctypes.c_int, vp, ctypes.c_int, T, vp, vp]
             getattr(L, f"dft_naive_{p}").argtypes = [i64, ctypes.c_int, vp, vp]
             getattr(L, f"baseline_fft_batch_{p}").argtypes = [i64, i64, ctypes.c_int, T, vp, vp, ctypes.c_int]
+            getattr(L, f"baseline_ndfft_{p}").argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, T, vp,
+                                                          ctypes.c_int]
+            getattr(L, f"baseline_real_batch_{p}").argtypes = [i64, i64, ctypes.c_int, ctypes.c_int, vp, vp,
+                                                               ctypes.c_int]
         L.oracle_chain_radices.argtypes = [i64, ctypes.c_int, vp]
         L.oracle_nextsmooth.argtypes = [i64]
         L.oracle_nextsmooth.restype = i64
@@ -172,6 +176,35 @@ def baseline_fft_batch(x, sign=-1, scale=1.0, prec="f32", nthreads=1):
     y = np.empty_like(x)
     _check(getattr(lib(), f"baseline_fft_batch_{prec}")(n, x.size // n, sign, scale, _ptr(x), _ptr(y), nthreads),
            "baseline")
+    return y
+
+
+def baseline_ndfft(x, dims, sign=-1, scale=1.0, prec="f32", nthreads=1, first=0):
+    """The reference's N-d plan on nthreads threads (column-major dims, axes
+    from `first`); returns a transformed copy of x."""
+    y = _cplx(x, prec).copy()
+    d = np.asarray(dims, dtype=np.int64)
+    _check(getattr(lib(), f"baseline_ndfft_{prec}")(len(dims), _ptr(d), first, sign, scale, _ptr(y), nthreads),
+           "baseline_ndfft")
+    return y
+
+
+def baseline_rfft_batch(x, prec="f32", nthreads=1):
+    """rfft of the rows of x (even length) on nthreads threads."""
+    x = np.ascontiguousarray(x, dtype=_NP_REAL[prec])
+    n = x.shape[-1]
+    y = np.empty(x.shape[:-1] + (n // 2 + 1,), dtype=_NP_CPLX[prec])
+    _check(getattr(lib(), f"baseline_real_batch_{prec}")(n, x.size // n, 0, 0, _ptr(x), _ptr(y), nthreads),
+           "baseline_real_batch")
+    return y
+
+
+def baseline_irfft_batch(X, n, normalized=True, prec="f32", nthreads=1):
+    """irfft (brfft when not normalized) of the rows of X on nthreads threads."""
+    X = _cplx(X, prec)
+    y = np.empty(X.shape[:-1] + (n,), dtype=_NP_REAL[prec])
+    _check(getattr(lib(), f"baseline_real_batch_{prec}")(n, X.size // X.shape[-1], 1, int(normalized), _ptr(X),
+                                                         _ptr(y), nthreads), "baseline_real_batch")
     return y
 
 

@@ -650,6 +650,183 @@ int F(baseline_fft_batch)(int64_t n, int64_t batch, int D, T scale, const T* x, 
   return 0;
 }
 
+/* the reference's N-d plan on nt threads (ndplan.jl:57-118, 270-327): axis
+ * by axis from `first`, the pencils of an axis in contiguous chunks over
+ * the threads (threads.jl:15-38), one chain per thread; in place on y, the
+ * scale on the last transformed axis.  13-smooth dims only. */
+typedef struct {
+  int64_t nd, Bi, lo, hi;
+  int D;
+  T scale;
+  T* y;
+  int err;
+} F(NdJob);
+
+static void* F(nd_worker)(void* arg) {
+  F(NdJob)* j = (F(NdJob)*)arg;
+  F(Chain) ch;
+  if (F(chain_init)(&ch, j->nd, j->D)) {
+    j->err = 1;
+    return NULL;
+  }
+  const int64_t nd = j->nd, Bi = j->Bi;
+  T* buf = (T*)malloc(sizeof(T) * 4 * (size_t)nd);
+  T *are = buf, *aim = buf + nd, *bre = buf + 2 * nd, *bim = buf + 3 * nd;
+  for (int64_t p = j->lo; p < j->hi; ++p) {
+    const int64_t bi = p % Bi, bo = p / Bi;
+    T* base = j->y + 2 * (bi + bo * Bi * nd);
+    for (int64_t i = 0; i < nd; ++i) {
+      are[i] = base[2 * i * Bi];
+      aim[i] = base[2 * i * Bi + 1];
+    }
+    int which = F(chain_run)(&ch, are, aim, bre, bim, 1, j->D);
+    T* rr = which ? bre : are;
+    T* ri = which ? bim : aim;
+    for (int64_t i = 0; i < nd; ++i) {
+      base[2 * i * Bi] = rr[i] * j->scale;
+      base[2 * i * Bi + 1] = ri[i] * j->scale;
+    }
+  }
+  free(buf);
+  F(chain_free)(&ch);
+  return NULL;
+}
+
+int F(baseline_ndfft)(int rank, const int64_t* dims, int first, int D, T scale, T* y, int nt) {
+  if (nt < 1) nt = 1;
+  if (nt > 256) nt = 256;
+  int last = -1;
+  for (int i = first; i < rank; ++i) {
+    if (!is_smooth13(dims[i])) return -1;
+    if (dims[i] > 1) last = i;
+  }
+  for (int ax = first; ax < rank; ++ax) {
+    const int64_t nd = dims[ax];
+    if (nd == 1) continue;
+    int64_t Bi = 1, Bo = 1;
+    for (int i = 0; i < ax; ++i) Bi *= dims[i];
+    for (int i = ax + 1; i < rank; ++i) Bo *= dims[i];
+    pthread_t th[256];
+    F(NdJob) jobs[256];
+    int used = 0;
+    for (int i = 0; i < nt; ++i) {
+      int64_t lo, hi;
+      chunk_range(Bi * Bo, nt, i, 1, &lo, &hi);
+      if (lo >= hi) continue;
+      F(NdJob) j = {nd, Bi, lo, hi, D, ax == last ? scale : (T)1, y, 0};
+      jobs[used] = j;
+      pthread_create(&th[used], NULL, F(nd_worker), &jobs[used]);
+      ++used;
+    }
+    int err = 0;
+    for (int i = 0; i < used; ++i) {
+      pthread_join(th[i], NULL);
+      err |= jobs[i].err;
+    }
+    if (err) return -1;
+  }
+  return 0;
+}
+
+/* rfft / irfft rows of an even length n on nt threads (real.jl:80-131,
+ * 144-193 per row): one half-length chain and twist table per thread */
+typedef struct {
+  int64_t n, lo, hi;
+  int inverse, normalized, err;
+  const T* x;
+  T* y;
+} F(RealJob);
+
+static void* F(real_worker)(void* arg) {
+  F(RealJob)* j = (F(RealJob)*)arg;
+  const int64_t n = j->n, h = n / 2;
+  const int D = j->inverse ? 1 : -1;
+  F(Chain) ch;
+  if (F(chain_init)(&ch, h, D)) {
+    j->err = 1;
+    return NULL;
+  }
+  T* buf = (T*)malloc(sizeof(T) * 6 * (size_t)h);
+  T *are = buf, *aim = buf + h, *bre = buf + 2 * h, *bim = buf + 3 * h, *wr = buf + 4 * h, *wi = buf + 5 * h;
+  for (int64_t k = 0; k < h; ++k) {
+    long double c, s;
+    expi_exact(k, n, D, &c, &s);
+    wr[k] = (T)c;
+    wi[k] = (T)s;
+  }
+  for (int64_t b = j->lo; b < j->hi; ++b) {
+    if (!j->inverse) {
+      const T* x = j->x + n * b;
+      T* y = j->y + 2 * (h + 1) * b;
+      for (int64_t i = 0; i < h; ++i) {
+        are[i] = x[2 * i];
+        aim[i] = x[2 * i + 1];
+      }
+      int which = F(chain_run)(&ch, are, aim, bre, bim, 1, D);
+      T* zr = which ? bre : are;
+      T* zi = which ? bim : aim;
+      y[0] = zr[0] + zi[0];
+      y[1] = 0;
+      y[2 * h] = zr[0] - zi[0];
+      y[2 * h + 1] = 0;
+      for (int64_t k = 1; k < h; ++k) {
+        T ar = zr[k], ai = zi[k], br = zr[h - k], bi = -zi[h - k];
+        T xer = (ar + br) * (T)0.5, xei = (ai + bi) * (T)0.5;
+        T tr = (ai - bi) * (T)0.5, ti = -(ar - br) * (T)0.5;
+        y[2 * k] = xer + (wr[k] * tr - wi[k] * ti);
+        y[2 * k + 1] = xei + (wr[k] * ti + wi[k] * tr);
+      }
+    } else {
+      const T* X = j->x + 2 * (h + 1) * b;
+      T* y = j->y + n * b;
+      const T pre = j->normalized ? (T)0.5 : (T)1;
+      for (int64_t k = 0; k < h; ++k) {
+        T ar = X[2 * k], ai = X[2 * k + 1], br = X[2 * (h - k)], bi = -X[2 * (h - k) + 1];
+        T xer = (ar + br) * pre, xei = (ai + bi) * pre;
+        T xdr = (ar - br) * pre, xdi = (ai - bi) * pre;
+        T tr = wr[k] * xdr - wi[k] * xdi, ti = wr[k] * xdi + wi[k] * xdr;
+        are[k] = xer - ti;
+        aim[k] = xei + tr;
+      }
+      int which = F(chain_run)(&ch, are, aim, bre, bim, 1, D);
+      T* zr = which ? bre : are;
+      T* zi = which ? bim : aim;
+      const T sc = j->normalized ? (T)(1.0L / h) : (T)1;
+      for (int64_t i = 0; i < h; ++i) {
+        y[2 * i] = zr[i] * sc;
+        y[2 * i + 1] = zi[i] * sc;
+      }
+    }
+  }
+  free(buf);
+  F(chain_free)(&ch);
+  return NULL;
+}
+
+int F(baseline_real_batch)(int64_t n, int64_t batch, int inverse, int normalized, const T* x, T* y, int nt) {
+  if (n < 2 || n % 2 || !is_smooth13(n / 2)) return -1;
+  if (nt < 1) nt = 1;
+  if (nt > 256) nt = 256;
+  pthread_t th[256];
+  F(RealJob) jobs[256];
+  int used = 0;
+  for (int i = 0; i < nt; ++i) {
+    int64_t lo, hi;
+    chunk_range(batch, nt, i, 1, &lo, &hi);
+    if (lo >= hi) continue;
+    F(RealJob) j = {n, lo, hi, inverse, normalized, 0, x, y};
+    jobs[used] = j;
+    pthread_create(&th[used], NULL, F(real_worker), &jobs[used]);
+    ++used;
+  }
+  int err = 0;
+  for (int i = 0; i < used; ++i) {
+    pthread_join(th[i], NULL);
+    err |= jobs[i].err;
+  }
+  return err ? -1 : 0;
+}
+
 #undef F
 #undef CAT
 #undef CAT2

@@ -215,6 +215,22 @@ def test_nd_real_planted_mistakes():
     assert rel_l2(O.irfft(z, 10), x) > 0.1
 
 
+def test_threaded_baselines_match_the_oracle():
+    # the CPU baseline's threaded N-d and real-row restatements give the
+    # serial oracle's numbers (same chains, pencils split over threads)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((6, 10, 16)) + 1j * rng.standard_normal((6, 10, 16))
+    ref = O.ndfft(x.ravel(), [16, 10, 6]).reshape(x.shape)
+    assert rel_l2(O.baseline_ndfft(x.ravel(), [16, 10, 6], prec="f64", nthreads=3).reshape(x.shape), ref) < 1e-15
+    xr = rng.standard_normal((5, 7, 32))
+    X = O.baseline_rfft_batch(xr, prec="f64", nthreads=4)
+    assert rel_l2(X, O.rfft(xr)) < 1e-15
+    assert rel_l2(O.baseline_irfft_batch(X, 32, prec="f64", nthreads=4), xr) < 1e-15
+    assert rel_l2(O.baseline_irfft_batch(X, 32, normalized=False, prec="f64", nthreads=2), 32 * xr) < 1e-15
+    Y = O.baseline_ndfft(X.ravel(), [17, 7, 5], prec="f64", nthreads=4, first=1).reshape(X.shape)
+    assert rel_l2(Y, O.ndrfft(xr)) < 1e-15
+
+
 # ---------------------------------------------------------------- RNG spec
 
 def test_splitmix64_kat(fj):
