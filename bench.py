@@ -178,8 +178,8 @@ def cpu_configs_leg(F, per_config_s=3.0):
                   min(8, threads), "1-D plan per transform across cores"))
     for n in (65537, 1000003):
         xp = c64(0, 4 * n).reshape(4, n)
-        cases.append((f"c2c_f32_{n}x4", (lambda xp=xp: rows(lambda r: O.resample(r, prec="f32"), xp)), 4,
-                      "ResamplePlan per transform across cores (the reference routes these primes to HvdhPlan)"))
+        cases.append((f"c2c_f32_{n}x4", (lambda xp=xp: rows(lambda r: O.hvdh(r, prec="f32"), xp)), 4,
+                      f"HvdhPlan (t = {O.hvdh_t(n)}) per transform across cores"))
     xr = r32(0, 4 << 22).reshape(4, 1 << 22)
     cases.append(("r2c_f32_2pow22x4", lambda: O.baseline_rfft_batch(xr, prec="f32", nthreads=4), 4,
                   "RfftPlan per transform across cores, inner 2^21 as one chain (the reference runs it "

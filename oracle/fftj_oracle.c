@@ -12,6 +12,8 @@
  *   chain_run / fft_chain      plan.jl:311-321, 359-396
  *   fourstep                   fourstep.jl:272-495 (pass A / pass B)
  *   resample                   resample.jl:12-21, 33-55, 202-276, 392-424
+ *   hvdh                       hvdh.jl:1-330, 560-688 (Gaussian resampling,
+ *                              banded chunk solves)
  *   oracle_rfft / irfft        real.jl:26-39, 80-211
  *   oracle_ndfft               ndplan.jl:57-118, 270-327
  *   chunk_range / baselines    threads.jl:15-38
@@ -68,6 +70,50 @@ static void expi_exact(int64_t e, int64_t L, int D, long double* c, long double*
   }
   *c = rc;
   *s = D < 0 ? -rs : rs;
+}
+
+static int64_t gcd64(int64_t a, int64_t b) {
+  while (b) {
+    int64_t r = a % b;
+    a = b;
+    b = r;
+  }
+  return a < 0 ? -a : a;
+}
+
+/* number of bits of x (Base.top_set_bit): 0 for x = 0 */
+static int top_bit(int64_t x) {
+  int k = 0;
+  while (x > 0) {
+    ++k;
+    x >>= 1;
+  }
+  return k;
+}
+
+/* HvdH's t (hvdh.jl:75-92): smooth, coprime to n, at least n(1 + 1/128);
+ * odd n: the smallest o * 2^k over SMOOTH_ODD (resample.jl:12) with
+ * gcd(o, n) = 1; even n: the smallest odd 13-smooth m; -1 if t - n >= n */
+static int is_smooth13(int64_t n);
+static int64_t hvdh_t(int64_t n) {
+  static const int64_t kSmoothOdd[7] = {1, 3, 5, 7, 9, 15, 21};
+  const int64_t N = (int64_t)ceill((long double)n * (1.0L + 1.0L / 128));
+  int64_t t = INT64_MAX;
+  if (n % 2) {
+    for (int i = 0; i < 7; ++i) {
+      const int64_t o = kSmoothOdd[i];
+      if (gcd64(o, n) != 1) continue;
+      const int64_t c = o << top_bit((N + o - 1) / o - 1);
+      if (c < t) t = c;
+    }
+  } else {
+    for (int64_t m = N | 1; m <= 2 * n - 1; m += 2)
+      if (is_smooth13(m) && gcd64(m, n) == 1) {
+        t = m;
+        break;
+      }
+  }
+  return (t != INT64_MAX && t - n < n) ? t : -1;
 }
 
 static int64_t isqrt64(int64_t n) {
@@ -261,6 +307,8 @@ static long double resample_tol_ld(void) { return 10.0L * 2.220446049250313e-16L
 int oracle_chain_radices(int64_t n, int W, int* rad) { return build_chain(n, W, rad); }
 
 int64_t oracle_nextsmooth(int64_t n) { return nextsmooth(n); }
+
+int64_t oracle_hvdh_t(int64_t n) { return hvdh_t(n); }
 
 void oracle_gauss_legendre(int nq, double* z, double* w) { gauss_legendre(nq, z, w); }
 

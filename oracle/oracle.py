@@ -38,6 +38,8 @@ def lib():
             getattr(L, f"oracle_fft_batch_{p}").argtypes = [i64, i64, ctypes.c_int, T, vp, vp]
             getattr(L, f"oracle_fourstep_{p}").argtypes = [i64, i64, i64, ctypes.c_int, T, vp, vp, ctypes.c_int]
             getattr(L, f"oracle_resample_{p}").argtypes = [i64, ctypes.c_int, T, vp, vp]
+            getattr(L, f"oracle_hvdh_{p}").argtypes = [i64, ctypes.c_int, T, vp, vp]
+            getattr(L, f"oracle_hvdh_halo_{p}").argtypes = [i64, ctypes.c_int, T, vp, vp, i64]
             getattr(L, f"oracle_rfft_{p}").argtypes = [i64, vp, vp]
             getattr(L, f"oracle_irfft_{p}").argtypes = [i64, vp, vp, ctypes.c_int]
             getattr(L, f"oracle_ndfft_{p}").argtypes = [ctypes.c_int, vp, ctypes.c_int, T, vp, vp]
@@ -50,6 +52,8 @@ def lib():
         L.oracle_chain_radices.argtypes = [i64, ctypes.c_int, vp]
         L.oracle_nextsmooth.argtypes = [i64]
         L.oracle_nextsmooth.restype = i64
+        L.oracle_hvdh_t.argtypes = [i64]
+        L.oracle_hvdh_t.restype = i64
         L.oracle_gauss_legendre.argtypes = [ctypes.c_int, vp, vp]
         L.oracle_chunk_range.argtypes = [i64, ctypes.c_int, ctypes.c_int, i64,
                                          ctypes.POINTER(i64), ctypes.POINTER(i64)]
@@ -96,6 +100,31 @@ def resample(x, sign=-1, scale=1.0, prec="f64"):
     y = np.empty_like(x)
     _check(getattr(lib(), f"oracle_resample_{prec}")(x.size, sign, scale, _ptr(x), _ptr(y)), "resample")
     return y
+
+
+def hvdh(x, sign=-1, scale=1.0, prec="f64", halo=None):
+    """1-D transform of each row by the HvdH Gaussian-resampling route (the
+    reference's route for odd non-smooth lengths, hvdh.jl).  halo overrides
+    the probed halo rows (tests plant a truncated one)."""
+    x = _cplx(x, prec)
+    n = x.shape[-1]
+    rows = x.reshape(-1, n)
+    y = np.empty_like(rows)
+    for i in range(rows.shape[0]):
+        r = np.ascontiguousarray(rows[i])
+        o = np.empty_like(r)
+        if halo is None:
+            rc = getattr(lib(), f"oracle_hvdh_{prec}")(n, sign, scale, _ptr(r), _ptr(o))
+        else:
+            rc = getattr(lib(), f"oracle_hvdh_halo_{prec}")(n, sign, scale, _ptr(r), _ptr(o), halo)
+        _check(rc, "hvdh")
+        y[i] = o
+    return y.reshape(x.shape)
+
+
+def hvdh_t(n):
+    """HvdH's smooth grid length t for n (-1: none)."""
+    return int(lib().oracle_hvdh_t(n))
 
 
 def rfft(x, prec="f64"):

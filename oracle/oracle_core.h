@@ -441,6 +441,298 @@ int F(oracle_resample)(int64_t n, int D, T scale, const T* x, T* y) {
   return F(resample)(n, D, scale, x, y, F(resample_tol)());
 }
 
+/* ----------------------------------------------------------------- HvdH */
+
+/* Length-n DFT by Gaussian resampling (hvdh.jl:1-19, after Harvey & van der
+ * Hoeven) onto the smooth t of hvdh_t, Delta = t - n, rho = t/n:
+ *   v[Delta*k mod t] = (S u)_k = (1/alpha) sum_j exp(-b (k n/t - j)^2) u_(j mod n),
+ *                      b = pi/alpha^2, 2h taps (hvdh.jl:360-470);
+ *   Y = F_t v, unnormalized, sign D;
+ *   (n/t) Y[q] = sum_l exp(-a (rho*l - q)^2) x_l,  a = pi*alpha^2,
+ *                x_l = X_(Delta*l mod n)     (Poisson summation of the spread)
+ * on the rows q = [rho*l] nearest each unknown; the other rows are
+ * identities.  Indexed by q the system is banded (entries exp(-a(d+beta)^2),
+ * |d| <= B) and is solved by LU without pivoting in chunks of L rows with H
+ * halo rows per side (hvdh.jl:474-686); the inverse decays, so each chunk's
+ * core rows are exact to rounding.  Parameters as hvdh.jl:154-190: alpha =
+ * 1.0 (F32) / 1.45 (F64), tol = eps/4, the AVX-512 chunk counts K = 16 / 8
+ * and the row target 256.  Weights by the reference's factorisations,
+ * exp(-b(delta-o)^2) = E G^o exp(-b o^2) and exp(-a(d+beta)^2) =
+ * A' g^d exp(-a d^2), two exponentials per output / per column. */
+
+typedef struct {
+  int64_t n, t, delta, H, L, G, K;
+  int B, h;
+  long double a, b, alpha, tol;
+} F(Hvdh);
+
+/* e = 2t*l' - 2n*q + n for the first unknown l' with rho*l' >= q - 1/2
+ * (hvdh.jl:147-151); row q holds unknown l' iff e < 2n */
+static int64_t F(hvdh_e)(int64_t n, int64_t t, int64_t q, int64_t* lp) {
+  const int64_t num = 2 * n * q - n, den = 2 * t;
+  int64_t l = num >= 0 ? (num + den - 1) / den : -((-num) / den);
+  *lp = l;
+  return 2 * t * l - 2 * n * q + n;
+}
+
+/* banded LU without pivoting, in place: A is m x m with half band w, stored
+ * dense row-major; solves A x = rhs */
+static int F(band_solve)(long double* A, long double* x, int64_t m, int64_t w) {
+  for (int64_t k = 0; k < m; ++k) {
+    const long double piv = A[k * m + k];
+    if (piv == 0) return -1;
+    const int64_t hi = k + w < m - 1 ? k + w : m - 1;
+    for (int64_t i = k + 1; i <= hi; ++i) {
+      const long double f = A[i * m + k] / piv;
+      if (f == 0) continue;
+      for (int64_t j = k; j <= hi; ++j) A[i * m + j] -= f * A[k * m + j];
+      x[i] -= f * x[k];
+    }
+  }
+  for (int64_t k = m - 1; k >= 0; --k) {
+    long double acc = x[k];
+    const int64_t hi = k + w < m - 1 ? k + w : m - 1;
+    for (int64_t j = k + 1; j <= hi; ++j) acc -= A[k * m + j] * x[j];
+    x[k] = acc / A[k * m + k];
+  }
+  return 0;
+}
+
+/* halo rows per side (hvdh.jl:118-145): rows of the inverse of local
+ * (2P+1)-windows around 18 probe unknowns decay below tol within H columns */
+static int64_t F(hvdh_halo)(int64_t n, int64_t t, long double a, int B, long double tol) {
+  const long double rho = (long double)t / n;
+  /* c = (t - n)^-1 mod n */
+  int64_t c = 1;
+  {
+    int64_t r0 = n, r1 = (t - n) % n, s0 = 0, s1 = 1;
+    while (r1) {
+      const int64_t qq = r0 / r1, r2 = r0 - qq * r1, s2 = s0 - qq * s1;
+      r0 = r1;
+      r1 = r2;
+      s0 = s1;
+      s1 = s2;
+    }
+    c = ((s0 % n) + n) % n;
+  }
+  int64_t probes[18];
+  int np = 0;
+  const int64_t cnt = n < 16 ? n : 16;
+  for (int64_t k = 0; k < cnt; ++k)
+    probes[np++] = cnt > 1 ? llrintl((long double)k * (n - 1) / (cnt - 1)) : 0;
+  probes[np++] = (int64_t)(((__int128)(n / 2) * c) % n);
+  probes[np++] = (int64_t)(((__int128)(n / 2 + 1) * c) % n);
+  for (int64_t P = 64; P <= 8192; P *= 2) {
+    const int64_t m = 2 * P + 1;
+    long double* M = (long double*)malloc(sizeof(long double) * (size_t)(m * m));
+    long double* r = (long double*)malloc(sizeof(long double) * (size_t)m);
+    int64_t H = 0;
+    for (int pi = 0; pi < np; ++pi) {
+      const int64_t l0 = probes[pi];
+      /* M^T, entry (j, i) = M[i, j] = exp(-a (rho l_j - fld(2t l_i + n, 2n))^2) */
+      memset(M, 0, sizeof(long double) * (size_t)(m * m));
+      for (int64_t i = 0; i < m; ++i) {
+        const int64_t li = l0 - P + i;
+        const int64_t num = 2 * t * li + n, den = 2 * n;
+        const int64_t qi = num >= 0 ? num / den : -((-num + den - 1) / den);
+        for (int64_t j = i - B - 1; j <= i + B + 1; ++j) {
+          if (j < 0 || j >= m) continue;
+          const long double d = rho * (long double)(l0 - P + j) - (long double)qi;
+          M[j * m + i] = expl(-a * d * d);
+        }
+        r[i] = 0;
+      }
+      r[P] = 1;
+      if (F(band_solve)(M, r, m, B + 1)) {
+        free(M);
+        free(r);
+        return -1;
+      }
+      long double mx = 0;
+      for (int64_t j = 0; j < m; ++j) mx = fabsl(r[j]) > mx ? fabsl(r[j]) : mx;
+      for (int64_t j = 0; j < m; ++j) {
+        const int64_t dj = j - P < 0 ? P - j : j - P;
+        if (fabsl(r[j]) > tol * mx && dj > H) H = dj;
+      }
+    }
+    free(M);
+    free(r);
+    if (H < P - 2 * B) return (int64_t)ceill(rho * H) + B;
+  }
+  return -1;
+}
+
+static int F(hvdh_init)(F(Hvdh) * p, int64_t n) {
+  p->n = n;
+  p->t = hvdh_t(n);
+  if (p->t < 0) return -1;
+  p->delta = p->t - n;
+  const int f64 = sizeof(T) >= 8;
+  p->alpha = f64 ? 1.45L : 1.0L;
+  p->a = (long double)M_PI * p->alpha * p->alpha;
+  p->b = (long double)M_PI / (p->alpha * p->alpha);
+  p->tol = f64 ? ldexpl(1.0L, -54) : ldexpl(1.0L, -25);
+  p->h = (int)ceill(p->alpha * sqrtl(logl(1 / p->tol) / (long double)M_PI));
+  p->B = (int)floorl(sqrtl(logl(1 / p->tol) / p->a) + 0.5L);
+  p->H = F(hvdh_halo)(n, p->t, p->a, p->B, p->tol);
+  if (p->H < 0) return -1;
+  p->K = f64 ? 8 : 16;
+  const int64_t Lt = 256;
+  p->G = (p->t + p->K * Lt - 1) / (p->K * Lt);
+  if (p->G < 1) p->G = 1;
+  p->L = (p->t + p->K * p->G - 1) / (p->K * p->G);
+  return 0;
+}
+
+/* halo < 0: the probed halo; halo >= 0 overrides it (tests plant a
+ * truncated halo there) */
+static int F(hvdh)(int64_t n, int D, T scale, const T* x, T* y, int64_t halo) {
+  F(Hvdh) p;
+  if (F(hvdh_init)(&p, n)) return -1;
+  if (halo >= 0) p.H = halo;
+  const int64_t t = p.t, dl = p.delta;
+  const int h = p.h, B = p.B;
+  /* spread, in transform order */
+  T* v = (T*)calloc(2 * (size_t)t, sizeof(T));
+  T* Y = (T*)malloc(sizeof(T) * 2 * (size_t)t);
+  T* kc = (T*)malloc(sizeof(T) * (size_t)(2 * h + 1));
+  for (int o = -h; o <= h; ++o) kc[o + h] = (T)expl(-p.b * o * o);
+  const T bT = (T)p.b, ia = (T)(1 / p.alpha);
+  for (int64_t k = 0; k < t; ++k) {
+    const int64_t kn_mod = (int64_t)(((__int128)k * n) % t);
+    int64_t s = (int64_t)(((__int128)k * n) / t);
+    T dlt;
+    if (kn_mod == 0) {
+      s -= 1;
+      dlt = (T)1;
+    } else {
+      dlt = (T)((long double)kn_mod / t);
+    }
+    const T E = (T)exp((double)(-bT * dlt * dlt)) * ia;
+    const T Gp = (T)exp((double)(2 * bT * dlt)), Gm = (T)1 / Gp;
+    T re = 0, im = 0;
+    T P = E;
+    for (int o = 1; o <= h; ++o) { /* taps right of s: weight E G^o exp(-b o^2) */
+      P *= Gp;
+      const T W = P * kc[o + h];
+      const int64_t j = ((s + o) % n + n) % n;
+      re += W * x[2 * j];
+      im += W * x[2 * j + 1];
+    }
+    P = E;
+    for (int o = 0; o >= 1 - h; --o) {
+      if (o < 0) P *= Gm;
+      const T W = P * kc[o + h];
+      const int64_t j = ((s + o) % n + n) % n;
+      re += W * x[2 * j];
+      im += W * x[2 * j + 1];
+    }
+    const int64_t q = (int64_t)(((__int128)dl * k) % t);
+    v[2 * q] = re;
+    v[2 * q + 1] = im;
+  }
+  free(kc);
+  if (F(fft_chain)(t, D, (T)1, v, Y)) {
+    free(v);
+    free(Y);
+    return -1;
+  }
+  free(v);
+  /* chunk solves: band storage, row i holds columns i-B..i+B; LU without
+   * pivoting in the plan's precision, one factorisation for the real and
+   * imaginary right-hand sides */
+  const int64_t R = p.L + 2 * p.H, W = 2 * B + 1;
+  T* A = (T*)malloc(sizeof(T) * (size_t)(R * W));
+  T* br = (T*)malloc(sizeof(T) * (size_t)R);
+  T* bi = (T*)malloc(sizeof(T) * (size_t)R);
+  T* beta = (T*)malloc(sizeof(T) * (size_t)R);
+  T* Ap = (T*)malloc(sizeof(T) * (size_t)R);
+  T* gp = (T*)malloc(sizeof(T) * (size_t)R);
+  T* ch = (T*)malloc(sizeof(T) * (size_t)(B + 1));
+  int64_t* out = (int64_t*)malloc(sizeof(int64_t) * (size_t)R);
+  for (int d = 0; d <= B; ++d) ch[d] = (T)expl(-p.a * d * d);
+  const T nt_ratio = (T)((long double)n / t), aT = (T)p.a;
+  int rc = 0;
+  for (int64_t c = 0; c < p.G * p.K && !rc; ++c) {
+    if (c * p.L >= t) break;
+    const int64_t q0 = c * p.L - p.H;
+    for (int64_t i = 0; i < R; ++i) {
+      int64_t lp;
+      const int64_t e = F(hvdh_e)(n, t, q0 + i, &lp);
+      out[i] = e < 2 * n ? (int64_t)((((__int128)dl * lp) % n + n) % n) : -1;
+      beta[i] = (T)((long double)(e - n) / (2.0L * n));
+      /* column state: A' = exp(-a beta^2), g = exp(-2 a beta) */
+      Ap[i] = out[i] < 0 ? (T)0 : (T)exp((double)(-aT * beta[i] * beta[i]));
+      gp[i] = (T)exp((double)(-2 * aT * beta[i]));
+    }
+    for (int64_t i = 0; i < R; ++i) {
+      T* row = A + i * W;
+      const int64_t qm = ((q0 + i) % t + t) % t;
+      for (int64_t m = 0; m < W; ++m) row[m] = 0;
+      if (out[i] < 0) {
+        row[B] = 1;
+        br[i] = bi[i] = 0;
+        continue;
+      }
+      for (int m = -B; m <= B; ++m) {
+        const int64_t j = i + m;
+        if (j < 0 || j >= R || out[j] < 0) continue;
+        /* exp(-a (m + beta_j)^2) = A'_j g_j^m exp(-a m^2) */
+        T gm = 1;
+        const T gb = m >= 0 ? gp[j] : (T)1 / gp[j];
+        for (int r = 0; r < (m >= 0 ? m : -m); ++r) gm *= gb;
+        row[m + B] = Ap[j] * gm * ch[m >= 0 ? m : -m];
+      }
+      br[i] = nt_ratio * Y[2 * qm];
+      bi[i] = nt_ratio * Y[2 * qm + 1];
+    }
+    for (int64_t k = 0; k < R && !rc; ++k) {
+      const T piv = A[k * W + B];
+      if (piv == 0) {
+        rc = -1;
+        break;
+      }
+      for (int64_t r = 1; r <= B && k + r < R; ++r) {
+        T* ri = A + (k + r) * W;
+        const T f = ri[B - r] / piv;
+        if (f == 0) continue;
+        for (int64_t j = 0; j <= B; ++j) ri[B - r + j] -= f * A[k * W + B + j];
+        br[k + r] -= f * br[k];
+        bi[k + r] -= f * bi[k];
+      }
+    }
+    for (int64_t k = R - 1; k >= 0 && !rc; --k) {
+      T sr = br[k], si = bi[k];
+      for (int64_t j = 1; j <= B && k + j < R; ++j) {
+        sr -= A[k * W + B + j] * br[k + j];
+        si -= A[k * W + B + j] * bi[k + j];
+      }
+      br[k] = sr / A[k * W + B];
+      bi[k] = si / A[k * W + B];
+    }
+    for (int64_t i = p.H; i < p.H + p.L && !rc; ++i) {
+      if (q0 + i >= t || out[i] < 0) continue;
+      y[2 * out[i]] = br[i] * scale;
+      y[2 * out[i] + 1] = bi[i] * scale;
+    }
+  }
+  free(A);
+  free(br);
+  free(bi);
+  free(beta);
+  free(Ap);
+  free(gp);
+  free(ch);
+  free(out);
+  free(Y);
+  return rc;
+}
+
+int F(oracle_hvdh)(int64_t n, int D, T scale, const T* x, T* y) { return F(hvdh)(n, D, scale, x, y, -1); }
+int F(oracle_hvdh_halo)(int64_t n, int D, T scale, const T* x, T* y, int64_t halo) {
+  return F(hvdh)(n, D, scale, x, y, halo);
+}
+
 /* ------------------------------------------------------------------ real */
 
 /* rfft (real.jl:80-142): even n packs to n/2 complex + twist, odd n full */

@@ -37,6 +37,8 @@ def test_config3_primes(n):
     rows = [0, 3]
     ref = O.fft(x[rows].astype(np.complex128))
     assert rel_l2(y[rows], ref) <= tol_for("f32", n)
+    # and against the reference's own route for these primes (HvdhPlan)
+    assert rel_l2(y[rows], O.hvdh(x[rows].astype(np.complex128))) <= tol_for("f32", n)
 
 
 # ------------------------------------------------------------------ real (A10)

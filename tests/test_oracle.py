@@ -167,6 +167,46 @@ def test_rfft_golden(golden, n):
     assert rel_l2(O.irfft(y, n, normalized=False), n * O.irfft(y, n)) < 1e-14  # brfft = n irfft
 
 
+# ------------------------------------------------------- HvdH (F1, hvdh.jl)
+
+def test_hvdh_grid_lengths():
+    # the reference's t for the configuration primes (t = 73,728 for 65,537
+    # and 1,048,576 for 1,000,003): smooth, coprime to n, >= n(1 + 1/128)
+    assert O.hvdh_t(65537) == 73728 and O.hvdh_t(1000003) == 1048576
+    for n in (17, 101, 1009, 10007, 20014, 255255):
+        t = O.hvdh_t(n)
+        assert t >= n * (1 + 1 / 128) and np.gcd(t, n) == 1 and t < 2 * n
+        assert O.nextsmooth(t) >= t
+    assert O.hvdh_t(20014) % 2 == 1   # even n: an odd t
+
+
+@pytest.mark.parametrize("n", [17, 101, 331, 1009, 2 * 10007, 65537])
+def test_hvdh_matches_the_dft(n):
+    # the Gaussian-resampling route against the DFT: brute force in long
+    # double for small n, numpy's FFT beyond; both directions and a scale
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = O.dft_naive(x, prec="ld").astype(np.complex128) if n <= 1009 else np.fft.fft(x)
+    assert rel_l2(O.hvdh(x), ref) < 1e-14
+    assert rel_l2(O.hvdh(x, sign=1, scale=1.0 / n), np.fft.ifft(x)) < 1e-14
+    assert rel_l2(O.hvdh(x.astype(np.complex64), prec="f32"), ref) < 1e-6
+
+
+def test_hvdh_planted_mistakes():
+    # chunks solved without their halo rows (a mistake planted in the
+    # oracle) fail the DFT check; outputs off by one index or permuted do too
+    n = 65537
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.fft(x)
+    y = O.hvdh(x)
+    assert rel_l2(y, ref) < 1e-14
+    assert rel_l2(O.hvdh(x, halo=0), ref) > 1e-10
+    assert rel_l2(O.hvdh(x, halo=2), ref) > 1e-12
+    assert rel_l2(np.roll(y, 1), ref) > 0.1
+    assert rel_l2(y[(np.arange(n) * 2) % n], ref) > 0.1
+
+
 # ------------------------------------------------------------------ N-d (A11)
 
 @pytest.mark.parametrize("shape", [(16, 11), (11, 16), (12, 10, 14), (33, 17), (8, 1), (24, 40, 3)])
