@@ -91,6 +91,9 @@ struct PassDesc {
   // w_k = exp(dir*2*pi*i*k/n), k = 0..N, in the plan's precision
   int32_t real_io;
   const void* rtw;
+  // generic (non-power-of-two) pass: first stage fused into the HBM load,
+  // last into the store (chains of two or more stages)
+  int32_t fuse_io;
 };
 
 // LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)

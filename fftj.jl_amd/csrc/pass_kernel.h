@@ -9,6 +9,100 @@
 
 namespace fftjh {
 
+// First stage fused with the HBM load (ns = 1, no twiddles): the thread's
+// butterflies j = jt + b*jstep read their inputs j + r*n/R straight from HBM
+// into registers and scatter the outputs j*R + r to LDS.  Saves the load
+// phase's LDS round trip and barrier.
+template <typename T, int DIR, int R>
+__device__ __noinline__ void first_stage(const int n, const int C, TwPtr<T> xb, const int64_t ist, const bool valid,
+                                         LdsPtr<T> lds, const int c, const int jt, const int jstep) {
+  constexpr int BPT = kElemsPerThread / R;
+  const int nr = n / R;
+  V2<T> v[BPT][R];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = jt + b * jstep;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      v[b][r] = (valid && j < nr) ? ldv<T>(xb, static_cast<int64_t>(j + r * nr) * ist) : mk<T>(T(0), T(0));
+  }
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = jt + b * jstep;
+    if (j < nr) {
+      dft<T, DIR, R>(v[b]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) stv(lds, lpad((j * R + r) * C + c), v[b][r]);
+    }
+  }
+  __syncthreads();
+}
+
+// Last stage fused with the HBM store (ns = n/R): butterfly j's outputs are
+// the transform's outputs j + r*ns, written from registers with the
+// multi-pass twiddle (started at j, stepped by ns) and the scale.  Saves the
+// store phase's LDS round trip.
+// The multi-pass twiddle of outputs j + r*ns, j = jt + b*jstep, comes in as
+// the caller's FP64 start w0 = w(jt) and the steps over r (ns) and over b
+// (jstep), each as separate arguments (in registers; a struct would go
+// through the stack): FP64 recurrences here, one rounding per output.
+template <typename T, int DIR, int R>
+__device__ __noinline__ void last_stage(const int n, const int C, const StageDesc s, LdsPtr<T> lds, TwPtr<T> tw,
+                                        GPtr<T> yb, const int64_t ost, const bool valid, const T scale,
+                                        const bool scaled, const bool twon, const double2 w0, const double2 Sr,
+                                        const double2 Sb, const int c, const int jt, const int jstep) {
+  constexpr int BPT = kElemsPerThread / R;
+  const int nr = n / R, ns = s.ns;
+  V2<T> v[BPT][R];
+  V2<T> w[BPT][R - 1];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int j = jt + b * jstep;
+    if (j < nr) {
+      TwPtr<T> twp = tw + (s.tw_off + j);
+#pragma unroll
+      for (int r = 1; r < R; ++r) w[b][r - 1] = ldv<T>(twp, (r - 1) * ns);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = ldv<T>(lds, lpad((j + r * nr) * C + c));
+    }
+  }
+  if (!valid) return;
+  if (twon) {
+    double2 wb = w0;
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = jt + b * jstep;
+      if (j < nr) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = cmul<T>(v[b][r], w[b][r - 1]);
+        dft<T, DIR, R>(v[b]);
+        double2 wr = wb;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          V2<T> val = cmul<T>(v[b][r], mk<T>(static_cast<T>(wr.x), static_cast<T>(wr.y)));
+          if (scaled) val = cscale<T>(val, scale);
+          stv(yb, static_cast<int64_t>(j + r * ns) * ost, val);
+          wr = cmul64(wr, Sr);
+        }
+      }
+      wb = cmul64(wb, Sb);
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int j = jt + b * jstep;
+      if (j < nr) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = cmul<T>(v[b][r], w[b][r - 1]);
+        dft<T, DIR, R>(v[b]);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          stv(yb, static_cast<int64_t>(j + r * ns) * ost, scaled ? cscale<T>(v[b][r], scale) : v[b][r]);
+      }
+    }
+  }
+}
+
 template <typename T> struct MaxThreads { static constexpr int value = kMaxThreads; };
 template <> struct MaxThreads<double> { static constexpr int value = kMaxThreads / 2; };
 
@@ -109,6 +203,125 @@ k_pass(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<T>* t
   }
 }
 
+
+// The same contract for chains of two or more VALU stages, with the first
+// stage fused into the HBM load and the last into the store (two LDS round
+// trips and barriers fewer per group).  A kernel of its own: the stage
+// calls then hold little live state, and the plain kernel keeps its
+// register allocation.
+template <typename T, int DIR>
+__global__ void __launch_bounds__(MaxThreads<T>::value)
+k_pass_fused(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<T>* tw) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  V2<T>* lds = reinterpret_cast<V2<T>*>(smem);
+  const PassDesc& d = *dp;
+  const int n = d.n, C = d.C;
+  const int tid = static_cast<int>(threadIdx.x);
+  const int c = tid & (C - 1);
+  const int jt = tid >> d.logC;
+  const int jstep = d.nthreads >> d.logC;
+  const int64_t ist = d.in.stride, ost = d.out.stride;
+  const GroupWalk walk(d);
+  const T scale = static_cast<T>(d.scale);
+  const bool scaled = d.scale != 1.0;
+  const int S = d.nstages;
+  for (int g = walk.start; g < walk.end; g += walk.step) {
+    const int64_t b = static_cast<int64_t>(g) * C + c;
+    const bool valid = b < d.batch;
+    {
+      const TwPtr<T> xb = as_tw<T>(x + BatchIndex(d, b).base(d.in));
+#define FFTJH_FIRST(R_) first_stage<T, DIR, R_>(n, C, xb, ist, valid, as_lds<T>(lds), c, jt, jstep)
+      switch (d.st[0].radix) {
+        case 2: FFTJH_FIRST(2); break;
+        case 3: FFTJH_FIRST(3); break;
+        case 4: FFTJH_FIRST(4); break;
+        case 5: FFTJH_FIRST(5); break;
+        case 7: FFTJH_FIRST(7); break;
+        case 8: FFTJH_FIRST(8); break;
+        case 11: FFTJH_FIRST(11); break;
+        case 13: FFTJH_FIRST(13); break;
+        default: FFTJH_FIRST(16); break;
+      }
+#undef FFTJH_FIRST
+    }
+    for (int s = 1; s < S - 1; ++s) {
+      const StageDesc& st = d.st[s];
+      switch (st.radix) {
+        case 2: run_stage<T, DIR, 2>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 3: run_stage<T, DIR, 3>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 4: run_stage<T, DIR, 4>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 5: run_stage<T, DIR, 5>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 7: run_stage<T, DIR, 7>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 8: run_stage<T, DIR, 8>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 11: run_stage<T, DIR, 11>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 13: run_stage<T, DIR, 13>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        default: run_stage<T, DIR, 16>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+      }
+    }
+    {
+      // (the batch index is recomputed here rather than kept live across
+      // the stage calls)
+      const BatchIndex ix(d, b);
+      const StageDesc& sl = d.st[S - 1];
+      const GPtr<T> yb = as_g<T>(y + ix.base(d.out));
+      const bool twon = d.tw_period != 0;
+      double2 w0 = make_double2(1.0, 0.0), Sr = w0, Sb = w0;
+      if (twon) {
+        PostTwiddle pt;
+        pt.init(d, ix, jt, sl.ns);
+        w0 = pt.w;
+        Sr = pt.S;
+        Sb = pt.step_for(d, jstep);
+      }
+#define FFTJH_LAST(R_)                                                                                         \
+  last_stage<T, DIR, R_>(n, C, sl, as_lds<T>(lds), as_tw<T>(tw), yb, ost, valid, scale, scaled, twon, w0, Sr, Sb, \
+                         c, jt, jstep)
+      switch (sl.radix) {
+        case 2: FFTJH_LAST(2); break;
+        case 3: FFTJH_LAST(3); break;
+        case 4: FFTJH_LAST(4); break;
+        case 5: FFTJH_LAST(5); break;
+        case 7: FFTJH_LAST(7); break;
+        case 8: FFTJH_LAST(8); break;
+        case 11: FFTJH_LAST(11); break;
+        case 13: FFTJH_LAST(13); break;
+        default: FFTJH_LAST(16); break;
+      }
+#undef FFTJH_LAST
+    }
+    __syncthreads();  // the last stage's LDS reads are done before the next group's first stage
+  }
+}
+
+template <typename T, int DIR>
+static hipError_t launch_pass_fused_t(const PassDesc& d, const PassDesc* dev_desc, const void* x, void* y,
+                                      const void* tw, hipStream_t stream) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass_fused<T, DIR>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+    attr_set = true;
+  }
+  const size_t lds = static_cast<size_t>(lds_elems(d.C * d.n)) * sizeof(V2<T>);
+  const int grid = d.grid > 0 ? d.grid : d.ngroups;
+  hipLaunchKernelGGL((k_pass_fused<T, DIR>), dim3(grid), dim3(d.nthreads), lds, stream, dev_desc,
+                     static_cast<const V2<T>*>(x), static_cast<V2<T>*>(y), static_cast<const V2<T>*>(tw));
+  return hipGetLastError();
+}
+
+template <typename T, int DIR>
+static int resident_groups_fused_t(int nthreads, size_t lds) {
+  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pass_fused<T, DIR>),
+                      hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_pass_fused<T, DIR>),
+                                                   nthreads, lds) != hipSuccess)
+    return 0;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  return per_cu * cus;
+}
 
 template <typename T, int DIR, int MAT>
 static hipError_t launch_pass_t(const PassDesc& d, const PassDesc* dev_desc, const void* x, void* y,

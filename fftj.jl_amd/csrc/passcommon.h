@@ -90,10 +90,10 @@ __device__ __forceinline__ double2 cmul64(double2 a, double2 b) {
 // each w once to the plan's precision.
 struct PostTwiddle {
   double2 w, S;
+  uint64_t q;  // the transform's multiplier q (reduced mod P)
   __device__ void init(const PassDesc& d, const BatchIndex& ix, int k0, int kstep) {
     const uint64_t P = static_cast<uint64_t>(d.tw_period);
     const double ip = d.tw_invp;
-    uint64_t q;
     if (d.fast_batch)
       q = fdiv(static_cast<uint32_t>(ix.bi), d.qd_m, d.qd_s);
     else
@@ -115,6 +115,16 @@ struct PostTwiddle {
   template <typename T>
   __device__ V2<T> value() const { return mk<T>(static_cast<T>(w.x), static_cast<T>(w.y)); }
   __device__ void advance() { w = cmul64(w, S); }
+  // the step of a second sequence over the same transform, w_P^(q*bcoef*kstep)
+  __device__ double2 step_for(const PassDesc& d, int kstep) const {
+    const uint64_t P = static_cast<uint64_t>(d.tw_period);
+    const double ip = d.tw_invp;
+    const uint64_t kr = static_cast<uint64_t>(kstep) < P ? static_cast<uint64_t>(kstep) : static_cast<uint64_t>(kstep) % P;
+    const uint64_t de = mulmod(q, mulmod(static_cast<uint64_t>(d.tw_br), kr, P, ip), P, ip);
+    const double2* t = static_cast<const double2*>(d.ptw);
+    const uint64_t mask = (uint64_t(1) << d.ptw_sh) - 1;
+    return cmul64(t[de >> d.ptw_sh], t[d.ptw_lo + static_cast<int64_t>(de & mask)]);
+  }
 };
 
 }  // namespace fftjh

@@ -472,6 +472,19 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
     d.xcd_remap = 0;
   }
   if (pt && pt->xcd_remap > 0) d.xcd_remap = pt->xcd_remap == 2 ? 1 : 0;
+  {
+    // generic chains: first stage fused into the load, last into the store.
+    // Measured (fused / unfused time): rows of 1000+ points and strided
+    // passes of 64+ gain (4000 x 16384 0.83, 6000 x 4096 0.81, 3^10 x 64
+    // 0.89, 10^6 x 8 0.97); short rows and short strided passes lose a
+    // little (100 x 2^20 1.06, 12 x 2^22 1.03, 49^3 x 256 1.03), so they keep
+    // the unfused kernel.  FFTJH_GENERIC_FUSE=0 turns it off.
+    static const bool fuse_env = [] {
+      const char* e = std::getenv("FFTJH_GENERIC_FUSE");
+      return !e || std::atoi(e) != 0;
+    }();
+    d.fuse_io = (fuse_env && !s.pow2 && !L.mat && (n >= 512 || (colmode && n >= 64))) ? 1 : 0;
+  }
   int ns = 1, off = 0;
   d.nstages = static_cast<int>(s.radices.size());
   for (size_t i = 0; i < s.radices.size(); ++i) {
@@ -1433,6 +1446,7 @@ static std::string describe(const Layout& L) {
       o << " n=" << s.d.n << " radices=";
       for (size_t i = 0; i < s.radices.size(); ++i) o << (i ? "," : "") << s.radices[i];
       if (s.pow2) o << " kernel=pow2";
+      if (s.d.fuse_io && s.radices.size() >= 2) o << " kernel=fused";
       if (s.mat) o << (s.mat == kMatF16 ? " kernel=mfma_f16" : s.mat == kMatBF16 ? " kernel=mfma_bf16" : " kernel=mfma_bf16x3");
       if (s.d.real_io) o << (s.d.real_io == 1 ? " real=r2c" : " real=c2r");
       if (s.d.rows_in) o << " rows_in";

@@ -22,8 +22,18 @@ namespace fftjh {
 
 int max_threads(int dtype) { return dtype == 0 ? MaxThreads<float>::value : MaxThreads<double>::value; }
 
+// chains of two or more stages take the kernel with fused first and last stages
+static bool fused(const PassDesc& d) { return d.fuse_io && d.nstages >= 2; }
+
 int pass_resident_groups(int dtype, int dir, const PassDesc& d) {
   const size_t lds = static_cast<size_t>(lds_elems(d.C * d.n)) * (dtype == 0 ? 8 : 16);
+  if (fused(d)) {
+    if (dtype == 0)
+      return dir < 0 ? resident_groups_fused_t<float, -1>(d.nthreads, lds)
+                     : resident_groups_fused_t<float, 1>(d.nthreads, lds);
+    return dir < 0 ? resident_groups_fused_t<double, -1>(d.nthreads, lds)
+                   : resident_groups_fused_t<double, 1>(d.nthreads, lds);
+  }
   if (dtype == 0)
     return dir < 0 ? resident_groups_t<float, -1, kMatNone>(d.nthreads, lds)
                    : resident_groups_t<float, 1, kMatNone>(d.nthreads, lds);
@@ -33,6 +43,13 @@ int pass_resident_groups(int dtype, int dir, const PassDesc& d) {
 
 hipError_t launch_pass(int dtype, int dir, const PassDesc& d, const PassDesc* dev_desc, const void* x,
                        void* y, const void* tw, hipStream_t stream) {
+  if (fused(d)) {
+    if (dtype == 0)
+      return dir < 0 ? launch_pass_fused_t<float, -1>(d, dev_desc, x, y, tw, stream)
+                     : launch_pass_fused_t<float, 1>(d, dev_desc, x, y, tw, stream);
+    return dir < 0 ? launch_pass_fused_t<double, -1>(d, dev_desc, x, y, tw, stream)
+                   : launch_pass_fused_t<double, 1>(d, dev_desc, x, y, tw, stream);
+  }
   if (dtype == 0)
     return dir < 0 ? launch_pass_t<float, -1, kMatNone>(d, dev_desc, x, y, tw, nullptr, stream)
                    : launch_pass_t<float, 1, kMatNone>(d, dev_desc, x, y, tw, nullptr, stream);
