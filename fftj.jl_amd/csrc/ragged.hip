@@ -47,6 +47,27 @@ __device__ void tile_stages(LdsPtr<T> lds, const RaggedSched& sc, int C, int log
   }
 }
 
+// Tile load: this thread's elements u < kRaggedTile/kRT of a tile, element u
+// at src[base + u*step], present while i0 + u*js < lim (all when `ok`),
+// into LDS slot tid + u*kRT.  Every load is issued before the first LDS
+// store (one load in flight per thread left the two-level items latency
+// bound); out of line, so its registers are its own.
+template <typename T, typename P>
+__device__ __noinline__ void tile_load(P src, const int64_t base, const int64_t step, const int i0, const int js,
+                                       const int lim, const bool ok, LdsPtr<T> lds) {
+  constexpr int kU = kRaggedTile / kRT;
+  const int tid = static_cast<int>(threadIdx.x);
+  V2<T> v[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    v[u] = mk<T>(T(0), T(0));
+    if (ok && i0 + u * js < lim) v[u] = ldv<T>(src, base + u * step);
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u)
+    if (i0 + u * js < lim) stv(lds, lpad(tid + u * kRT), v[u]);
+}
+
 // length-n transform src -> dst (both contiguous, global) as two tile phases
 // through `mid` (global, n elements); dst may equal src
 template <typename T, int DIR>
@@ -65,14 +86,10 @@ __device__ void two_level(LdsPtr<T> lds, const RaggedItem& it, const RaggedSched
   // phase A: columns j1 (stride n1) of length n2, twiddle w_n^{j1*k2}, mid[j1 + n1*k2]
   {
     const int C = it.cA, logC = it.logcA;
+    const int ca = tid & (C - 1), ia = tid >> logC, ja = kRT >> logC;
     for (int c0 = 0; c0 < n1; c0 += C) {
-      for (int f = tid; f < C * n2; f += kRT) {
-        const int c = f & (C - 1), i = f >> logC;
-        const int col = c0 + c;
-        V2<T> v = mk<T>(T(0), T(0));
-        if (col < n1) v = ldv<T>(src, col + static_cast<int64_t>(n1) * i);
-        stv(lds, lpad(i * C + c), v);
-      }
+      tile_load<T>(src, c0 + ca + static_cast<int64_t>(n1) * ia, static_cast<int64_t>(n1) * ja, ia, ja, n2,
+                   c0 + ca < n1, lds);
       __syncthreads();
       tile_stages<T, DIR>(lds, sa, C, logC, tw);
       for (int f = tid; f < C * n2; f += kRT) {
@@ -92,14 +109,9 @@ __device__ void two_level(LdsPtr<T> lds, const RaggedItem& it, const RaggedSched
   // phase B: rows k2 of mid (length n1), output dst[k2 + n2*k1]
   {
     const int C = it.cB, logC = it.logcB;
+    const int cb = tid & (C - 1), ib = tid >> logC, jb = kRT >> logC;
     for (int r0 = 0; r0 < n2; r0 += C) {
-      for (int f = tid; f < C * n1; f += kRT) {
-        const int c = f & (C - 1), i = f >> logC;
-        const int row = r0 + c;
-        V2<T> v = mk<T>(T(0), T(0));
-        if (row < n2) v = ldv<T>(mid, static_cast<int64_t>(row) * n1 + i);
-        stv(lds, lpad(i * C + c), v);
-      }
+      tile_load<T>(mid, static_cast<int64_t>(r0 + cb) * n1 + ib, jb, ib, jb, n1, r0 + cb < n2, lds);
       __syncthreads();
       tile_stages<T, DIR>(lds, sb, C, logC, tw);
       for (int f = tid; f < C * n1; f += kRT) {
@@ -165,7 +177,7 @@ k_ragged(const RaggedItem* __restrict__ items, const int32_t* __restrict__ wg_pt
     const int n = it.n;
     if (it.kind == kRaggedDirect) {
       const RaggedSched& sc = scheds[it.sA];
-      for (int i = tid; i < n; i += kRT) lds[lpad(i)] = xi[i];
+      tile_load<T>(as_tw<T>(xi), tid, kRT, tid, kRT, n, true, l3);
       __syncthreads();
       tile_stages<T, DIR>(l3, sc, 1, 0, t1);
       for (int k = tid; k < n; k += kRT) yi[k] = cscale<T>(lds[lpad(k)], scale);
