@@ -7,6 +7,8 @@
 //  * powers of two (2..64): Cooley-Tukey split R = R1*R2 down to radix 2/4,
 //    with the inner twiddles exp(DIR*2*pi*i*e/R) as compile-time constants
 //    (+-i and -1 become swaps/negations);
+//  * composites 6, 10, 12, 14, 15 (prime-factor split) and 9 (3 x 3 with
+//    inner twiddles);
 //  * odd primes 3..13: the conjugate-pair form also used by the reference's
 //    generic odd butterfly (kernels.jl:146-228): with t_j = x_j + x_{P-j},
 //    d_j = x_j - x_{P-j},  u_k = x_0 + sum_j cos(2 pi jk/P) t_j,
@@ -159,10 +161,85 @@ __device__ __forceinline__ void dft_odd(V2<T>* v) {
   for (int k = 0; k < P; ++k) v[k] = out[k];
 }
 
+// Composite lengths R = R1*R2 with gcd(R1, R2) = 1 (6, 10, 12, 14, 15): the
+// prime-factor (Good-Thomas) split, no inner twiddles.  Input n = (R2*n1 +
+// R1*n2) mod R, output k = (k1*R2*(R2^-1 mod R1) + k2*R1*(R1^-1 mod R2)) mod R:
+// then w_R^(nk) = w_R1^(n1 k1) * w_R2^(n2 k2), so R2 DFTs of length R1 and R1
+// of length R2 with the two index maps as compile-time permutations.
+template <typename T, int DIR, int R>
+__device__ __forceinline__ void dft_any(V2<T>* v);
+
+constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 1;
+}
+
+template <typename T, int DIR, int R1, int R2>
+__device__ __forceinline__ void dft_pfa(V2<T>* v) {
+  constexpr int R = R1 * R2;
+  constexpr int s1 = inv_mod(R2 % R1, R1), s2 = inv_mod(R1 % R2, R2);
+  V2<T> t[R];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    V2<T> u[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = v[(R2 * n1 + R1 * n2) % R];
+    dft_any<T, DIR, R1>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) t[k1 * R2 + n2] = u[k1];
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    V2<T> u[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) u[n2] = t[k1 * R2 + n2];
+    dft_any<T, DIR, R2>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[(k1 * R2 * s1 + k2 * R1 * s2) % R] = u[k2];
+  }
+}
+
+// R = 9 = 3 x 3 (Cooley-Tukey): input n = 3*n1 + n2, output k = k1 + 3*k2,
+// inner twiddles w_9^(n2 k1) from the 9th roots of unity
+template <typename T, int DIR>
+__device__ __forceinline__ void dft9(V2<T>* v) {
+  V2<T> t[9];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; ++n2) {
+    V2<T> u[3] = {v[n2], v[3 + n2], v[6 + n2]};
+    dft_odd<T, DIR, 3>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+      const int e = n2 * k1;
+      t[n2 * 3 + k1] = e == 0 ? u[k1]
+                              : cmul<T>(u[k1], mk<T>(static_cast<T>(kCos9[e]), static_cast<T>(DIR * kSin9[e])));
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    V2<T> u[3] = {t[k1], t[3 + k1], t[6 + k1]};
+    dft_odd<T, DIR, 3>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < 3; ++k2) v[k1 + 3 * k2] = u[k2];
+  }
+}
+
+template <typename T, int DIR, int R>
+__device__ __forceinline__ void dft_any(V2<T>* v) {
+  if constexpr ((R & (R - 1)) == 0) dft_pow2<T, DIR, R>(v);
+  else if constexpr (R == 6) dft_pfa<T, DIR, 2, 3>(v);
+  else if constexpr (R == 9) dft9<T, DIR>(v);
+  else if constexpr (R == 10) dft_pfa<T, DIR, 2, 5>(v);
+  else if constexpr (R == 12) dft_pfa<T, DIR, 4, 3>(v);
+  else if constexpr (R == 14) dft_pfa<T, DIR, 2, 7>(v);
+  else if constexpr (R == 15) dft_pfa<T, DIR, 3, 5>(v);
+  else dft_odd<T, DIR, R>(v);
+}
+
 template <typename T, int DIR, int R>
 __device__ __forceinline__ void dft(V2<T>* v) {
-  if constexpr ((R & (R - 1)) == 0) dft_pow2<T, DIR, R>(v);
-  else dft_odd<T, DIR, R>(v);
+  dft_any<T, DIR, R>(v);
 }
 
 }  // namespace fftjh

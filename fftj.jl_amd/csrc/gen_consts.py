@@ -4,7 +4,7 @@ The odd-prime butterflies use the conjugate-pair form of a length-P DFT
 (X_k = u_k + i v_k, X_{P-k} = u_k - i v_k with u_k = x_0 + sum_j cos(2 pi jk/P) t_j
 and v_k = sum_j sin(2 pi jk/P) d_j, the form of kernels.jl:146-228), so they need
 cos/sin(2 pi jk/P) for j, k = 1..(P-1)/2.  The power-of-two butterflies need the
-64th roots of unity.  Values are computed with mpmath at 40 digits and printed
+64th roots of unity, the radix-9 butterfly (3 x 3 with inner twiddles) the 9th.  Values are computed with mpmath at 40 digits and printed
 with 17 significant digits, which round-trips every double exactly.
 
 Run:  python gen_consts.py > dft_consts.h
@@ -38,6 +38,11 @@ def main():
         out.append(f"// cos/sin(2 pi j k / {p}) at [(k-1)*{h} + (j-1)], j,k = 1..{h}")
         out.append(f"__device__ constexpr double kCos{p}[{h*h}] = {{{', '.join(cs)}}};")
         out.append(f"__device__ constexpr double kSin{p}[{h*h}] = {{{', '.join(ss)}}};")
+    cs = [lit(mpmath.cos(2 * mpmath.pi * e / 9)) for e in range(9)]
+    ss = [lit(mpmath.sin(2 * mpmath.pi * e / 9)) for e in range(9)]
+    out.append("// cos/sin(2 pi e / 9), e = 0..8 (the radix-9 butterfly's inner twiddles)")
+    out.append(f"__device__ constexpr double kCos9[9] = {{{', '.join(cs)}}};")
+    out.append(f"__device__ constexpr double kSin9[9] = {{{', '.join(ss)}}};")
     cs = [lit(mpmath.cos(2 * mpmath.pi * e / 64)) for e in range(64)]
     ss = [lit(mpmath.sin(2 * mpmath.pi * e / 64)) for e in range(64)]
     out.append("// cos/sin(2 pi e / 64), e = 0..63")

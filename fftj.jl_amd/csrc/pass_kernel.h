@@ -171,6 +171,12 @@ k_pass(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<T>* t
         case 3: run_stage<T, DIR, 3>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
         case 4: run_stage<T, DIR, 4>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
         case 5: run_stage<T, DIR, 5>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 6: run_stage<T, DIR, 6>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 9: run_stage<T, DIR, 9>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 10: run_stage<T, DIR, 10>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 12: run_stage<T, DIR, 12>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 14: run_stage<T, DIR, 14>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 15: run_stage<T, DIR, 15>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
         case 7: run_stage<T, DIR, 7>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
         case 8: run_stage<T, DIR, 8>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
         case 11: run_stage<T, DIR, 11>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
@@ -236,6 +242,12 @@ k_pass_fused(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2
         case 3: FFTJH_FIRST(3); break;
         case 4: FFTJH_FIRST(4); break;
         case 5: FFTJH_FIRST(5); break;
+        case 6: FFTJH_FIRST(6); break;
+        case 9: FFTJH_FIRST(9); break;
+        case 10: FFTJH_FIRST(10); break;
+        case 12: FFTJH_FIRST(12); break;
+        case 14: FFTJH_FIRST(14); break;
+        case 15: FFTJH_FIRST(15); break;
         case 7: FFTJH_FIRST(7); break;
         case 8: FFTJH_FIRST(8); break;
         case 11: FFTJH_FIRST(11); break;
@@ -251,6 +263,12 @@ k_pass_fused(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2
         case 3: run_stage<T, DIR, 3>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
         case 4: run_stage<T, DIR, 4>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
         case 5: run_stage<T, DIR, 5>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 6: run_stage<T, DIR, 6>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 9: run_stage<T, DIR, 9>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 10: run_stage<T, DIR, 10>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 12: run_stage<T, DIR, 12>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 14: run_stage<T, DIR, 14>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
+        case 15: run_stage<T, DIR, 15>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
         case 7: run_stage<T, DIR, 7>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
         case 8: run_stage<T, DIR, 8>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
         case 11: run_stage<T, DIR, 11>(n, C, st, as_lds<T>(lds), as_tw<T>(tw), c, jt, jstep); break;
@@ -281,6 +299,12 @@ k_pass_fused(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2
         case 3: FFTJH_LAST(3); break;
         case 4: FFTJH_LAST(4); break;
         case 5: FFTJH_LAST(5); break;
+        case 6: FFTJH_LAST(6); break;
+        case 9: FFTJH_LAST(9); break;
+        case 10: FFTJH_LAST(10); break;
+        case 12: FFTJH_LAST(12); break;
+        case 14: FFTJH_LAST(14); break;
+        case 15: FFTJH_LAST(15); break;
         case 7: FFTJH_LAST(7); break;
         case 8: FFTJH_LAST(8); break;
         case 11: FFTJH_LAST(11); break;

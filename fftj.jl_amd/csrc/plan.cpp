@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -115,6 +116,61 @@ static std::vector<int> default_radices(int64_t n) {
   for (int i = 0; i < 5; ++i)
     for (int k = 0; k < cnt[i]; ++k) r.push_back(kOddRadices[i]);
   return r;
+}
+
+// threads one column of a chain needs: every stage's butterflies covered,
+// kElemsPerThread / R butterflies per thread
+static int64_t chain_threads(int64_t n, const std::vector<int>& radices) {
+  int64_t t = 1;
+  for (int R : radices) {
+    const int64_t bpt = kElemsPerThread / R;
+    t = std::max<int64_t>(t, (n / R + bpt - 1) / bpt);
+  }
+  return t;
+}
+
+// The fewest stages over the VALU radices, composites 6, 9, 10, 12, 14, 15
+// included (e.g. 100 = 10*10, 1000 = 10*10*10, 81 = 9*9 where the prime
+// chain takes 3, 4 and 4 stages); ties go to the chain with fewer threads
+// per column, then more power-of-two stages.  Power-of-two stages first
+// (smallest first), the rest ascending.  Empty when n is not 13-smooth.
+static std::vector<int> compact_radices(int64_t n) {
+  static const int kValu[] = {16, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+  int a, cnt[5];
+  if (n <= 1 || !factor13(n, a, cnt)) return {};
+  std::vector<int> best, cur;
+  std::function<void(int64_t, int)> rec = [&](int64_t m, int maxr) {
+    if (!best.empty() && cur.size() >= best.size() && m > 1) return;
+    if (m == 1) {
+      std::vector<int> c = cur;
+      std::sort(c.begin(), c.end(), [](int x, int y) {
+        const bool px = (x & (x - 1)) == 0, py = (y & (y - 1)) == 0;
+        if (px != py) return px;
+        return x < y;
+      });
+      if (best.empty() || c.size() < best.size()) {
+        best = c;
+        return;
+      }
+      if (c.size() > best.size()) return;
+      const int64_t tc = chain_threads(n, c), tb = chain_threads(n, best);
+      auto pow2s = [](const std::vector<int>& v) {
+        int k = 0;
+        for (int R : v) k += (R & (R - 1)) == 0;
+        return k;
+      };
+      if (tc < tb || (tc == tb && pow2s(c) > pow2s(best))) best = c;
+      return;
+    }
+    for (int R : kValu) {
+      if (R > maxr || m % R) continue;
+      cur.push_back(R);
+      rec(m / R, R);
+      cur.pop_back();
+    }
+  };
+  rec(n, 16);
+  return best;
 }
 
 static int esize(int dtype) { return dtype == FFTJH_F32 ? 8 : 16; }
@@ -354,7 +410,7 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
     std::vector<int> r(pt->radices, pt->radices + pt->nradix);
     int64_t prod = 1;
     for (int R : r) {
-      const bool valu = R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 11 || R == 13 || R == 16;
+      const bool valu = R >= 2 && R <= 16;  // primes to 13, powers of two, composites 6..15
       if (!valu && !(L.mat && mat_radix(R)))
         fail(FFTJH_ERR_ARGUMENT, "tuning radix " + std::to_string(R) + " has no kernel");
       prod *= R;
@@ -376,6 +432,35 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   // transforms per workgroup: strided passes take adjacent columns for
   // contiguous HBM runs; row passes group short transforms to fill a workgroup
   const bool colmode = in.stride > 1 || out.stride > 1;
+  const int row_cap = L.dtype == FFTJH_F32 ? 4 : 8;
+  // the generic kernel's group size for a chain (0: none fits)
+  auto generic_C = [&](const std::vector<int>& rad, int* tout) {
+    for (int C = colmode ? 16 : 64; C >= 1; C >>= 1) {
+      int t;
+      if (!pass_fits(L.dtype, n, C, rad, &t)) continue;
+      if (C > 1 && C > batch) continue;
+      if (!colmode && t > 256 && C > 1) continue;
+      if (!colmode && C > row_cap && t >= 64 * (C / row_cap) && pass_fits_rows_ok(L.dtype, n, C, row_cap)) continue;
+      if (tout) *tout = t;
+      return C;
+    }
+    return 0;
+  };
+  // composite radices: fewer stages, taken where the chain still fits as
+  // many transforms per workgroup (FFTJH_COMPOSITE_RADICES=0 turns it off)
+  {
+    static const bool comp_env = [] {
+      const char* e = std::getenv("FFTJH_COMPOSITE_RADICES");
+      return !e || std::atoi(e) != 0;
+    }();
+    if (comp_env && !s.pow2 && !L.mat && !(pt && pt->nradix > 0) && n > 1) {
+      std::vector<int> rc = compact_radices(n);
+      if (!rc.empty() && rc.size() < s.radices.size() && static_cast<int>(rc.size()) <= kMaxStages) {
+        const int c_old = generic_C(s.radices, nullptr), c_new = generic_C(rc, nullptr);
+        if (c_new > 0 && c_new >= c_old) s.radices = rc;
+      }
+    }
+  }
   auto fits = [&](int C, int* t) {
     if (L.mat) return pass_fits_mat(n, C, s.radices, t);
     return s.pow2 ? pow2_fits(L.dtype, n, C, t) : pass_fits(L.dtype, n, C, s.radices, t);
@@ -395,7 +480,6 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   } else {
     // row passes: a wave covers C rows x 64/C consecutive elements, so keep
     // C <= 128 bytes / 8*esize ... i.e. at least 128-byte runs
-    const int row_cap = L.dtype == FFTJH_F32 ? 4 : 8;
     for (int attempt = 0; attempt < 2 && !bestC; ++attempt) {
     if (attempt == 1) {
       if (!s.pow2) break;

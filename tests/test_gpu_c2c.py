@@ -131,3 +131,33 @@ def test_config2c_1e6x8():
     y = to_host(F.plan_fft(torch.float32, n, batch=batch)(to_dev(x)))
     ref = O.fft(x[[0, 7]].astype(np.complex128))
     assert rel_l2(y.reshape(batch, n)[[0, 7]], ref) <= tol_for("f32", n)
+
+
+# composite-radix butterflies (prime-factor 6, 10, 12, 14, 15 and 3 x 3 = 9):
+# each forced as a stage of an explicit chain, first, middle and last
+# position, both directions (the fused generic kernel and the plain one)
+COMPOSITE_CHAINS = [[6, 16], [16, 9], [10, 16], [16, 12], [14, 16], [16, 15], [6, 6, 6], [9, 9, 9],
+                    [10, 10, 10], [12, 5, 12], [14, 14], [15, 15], [2, 15, 3], [9, 7, 10]]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("chain", COMPOSITE_CHAINS, ids=lambda c: "x".join(map(str, c)))
+def test_composite_radix_chains(prec, chain):
+    L = F._lib
+    n = int(np.prod(chain))
+    batch = 3
+    x = inputs_c(n, n * batch, prec).reshape(batch, n)
+    code = L.F32 if prec == "f32" else L.F64
+    for sign in (-1, 1):
+        p = F.FFTPlan(code, (n,), sign, 1.0, batch, None, L.make_tuning((), [{"radices": chain}]))
+        assert f"radices={','.join(map(str, chain))} " in p.describe()
+        y = to_host(p(to_dev(x)))
+        ref = O.fft(x.astype(np.complex128), sign=sign)
+        assert rel_l2(y, ref) <= tol_for(prec, n)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [90, 196, 225, 1000, 4000, 6000, 3 ** 10, 10 ** 6])
+def test_default_composite_plans(prec, n):
+    # lengths whose default chains take composite radices
+    _check_c2c(prec, n, batch=2)

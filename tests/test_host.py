@@ -166,6 +166,25 @@ def test_route_flags():
         F.plan_fft("f32", n, route="bluestein")
 
 
+def test_composite_radix_schedules():
+    """Fewer stages with composite radices where the group size holds."""
+    L = _lib
+    def radices(n, batch):
+        d = L.describe_request(L.KIND_C2C, L.F32, [n], batch=batch)
+        return [ln.split("radices=")[1].split()[0] for ln in d.splitlines()[1:] if "radices=" in ln]
+    assert radices(1000, 131072) == ["10,10,10"]
+    assert radices(10 ** 6, 8) == ["10,10"] * 3
+    assert radices(196, 100) == ["14,14"] and radices(225, 100) == ["15,15"] and radices(90, 1000) == ["6,15"]
+    assert radices(12, 1 << 22) == ["12"]
+    # powers of two keep the fused power-of-two chain
+    assert radices(4096, 4) == ["16,16,16"]
+    # tuned chains may name composite radices; radix 17 has no kernel
+    d = L.describe_tuned(L.F32, (160,), 2, -1, L.make_tuning((), [{"radices": [10, 16]}]))
+    assert "radices=10,16 " in d
+    with pytest.raises(L.ArgumentError):
+        L.describe_tuned(L.F32, (17 * 16,), 1, -1, L.make_tuning((), [{"radices": [17, 16]}]))
+
+
 def test_region_check():
     import torch
     x = torch.zeros(4, 4, dtype=torch.complex64)
