@@ -53,19 +53,76 @@ __device__ void tile_stages(LdsPtr<T> lds, const RaggedSched& sc, int C, int log
 // store (one load in flight per thread left the two-level items latency
 // bound); out of line, so its registers are its own.
 template <typename T, typename P>
-__device__ __noinline__ void tile_load(P src, const int64_t base, const int64_t step, const int i0, const int js,
-                                       const int lim, const bool ok, LdsPtr<T> lds) {
+__device__ __noinline__ void tile_load(P src_, const int64_t base, const int64_t step_, const int i0, const int js_,
+                                       const int lim_, const bool ok, LdsPtr<T> lds_) {
   constexpr int kU = kRaggedTile / kRT;
   const int tid = static_cast<int>(threadIdx.x);
+  // (src, step, js, lim and lds are uniform: scalar registers)
+  const P src = uni_ptr(src_);
+  const int64_t step = static_cast<int64_t>(
+      (static_cast<uint64_t>(static_cast<unsigned>(uni(static_cast<int>(static_cast<uint64_t>(step_) >> 32)))) << 32) |
+      static_cast<unsigned>(uni(static_cast<int>(step_ & 0xffffffff))));
+  const int js = uni(js_), lim = uni(lim_);
+  const LdsPtr<T> lds = uni_lds<T>(lds_) + lpad(tid);
+  // rows present: u < cnt
+  const int cnt = i0 < lim ? (lim - i0 + js - 1) / js : 0;
+  const int got = ok ? cnt : 0;
   V2<T> v[kU];
+  P p = src + base;
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     v[u] = mk<T>(T(0), T(0));
-    if (ok && i0 + u * js < lim) v[u] = ldv<T>(src, base + u * step);
+    if (u < got) v[u] = ldv<T>(p, 0);
+    p += step;
   }
+  // lpad(tid + u*kRT) = lpad(tid) + u*(kRT + kRT/16)
 #pragma unroll
   for (int u = 0; u < kU; ++u)
-    if (i0 + u * js < lim) stv(lds, lpad(tid + u * kRT), v[u]);
+    if (u < cnt) stv(lds, u * (kRT + kRT / 16), v[u]);
+}
+
+// Phase-A epilogue of a column tile: element f = tid + u*kRT of the tile
+// (column c0 + (f & (C-1)), row k2 = f >> logC) times w_n^{col*k2} into
+// mid[col + n1*k2].  All twiddle-table loads are issued before the first
+// use (a load and its wait per element left the loop latency bound).
+template <typename T>
+__device__ __noinline__ void tile_store_tw(LdsPtr<T> lds_, GPtr<T> mid_, TwPtr<T> thi_, TwPtr<T> tlo_,
+                                           const int lo_bits_, const int n1_, const int total_, const int C_,
+                                           const int logC_, const int c0_) {
+  constexpr int kU = kRaggedTile / kRT;
+  const int tid = static_cast<int>(threadIdx.x);
+  const int lo_bits = uni(lo_bits_), n1 = uni(n1_), total = uni(total_), C = uni(C_), logC = uni(logC_);
+  const int c0 = uni(c0_);
+  const unsigned lo_mask = (1u << lo_bits) - 1u;
+  using TwBytes = const __attribute__((address_space(1))) char*;
+  const TwBytes thi = reinterpret_cast<TwBytes>(uni_ptr(thi_));
+  const TwBytes tlo = reinterpret_cast<TwBytes>(uni_ptr(tlo_));
+  const LdsPtr<T> lds = uni_lds<T>(lds_) + lpad(tid);
+  const int col = c0 + (tid & (C - 1));
+  const int k20 = tid >> logC, ks = kRT >> logC;
+  // rows of this thread present in the tile (the column itself must exist)
+  const int cnt = (col < n1 && tid < total) ? (total - tid + kRT - 1) / kRT : 0;
+  const unsigned e0 = static_cast<unsigned>(col) * static_cast<unsigned>(k20);
+  const unsigned de = static_cast<unsigned>(col) * static_cast<unsigned>(ks);
+  NV2<T> a[kU], b[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    if (u < cnt) {
+      const unsigned e = e0 + static_cast<unsigned>(u) * de;
+      a[u] = *reinterpret_cast<TwPtr<T>>(thi + (e >> lo_bits) * static_cast<unsigned>(sizeof(NV2<T>)));
+      b[u] = *reinterpret_cast<TwPtr<T>>(tlo + (e & lo_mask) * static_cast<unsigned>(sizeof(NV2<T>)));
+    }
+  }
+  GPtr<T> mp = uni_ptr(mid_) + (col + static_cast<int64_t>(n1) * k20);
+  const int64_t mstep = static_cast<int64_t>(n1) * ks;
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    if (u < cnt) {
+      const V2<T> w = cmul<T>(mk<T>(a[u].x, a[u].y), mk<T>(b[u].x, b[u].y));
+      stv(mp, 0, cmul<T>(ldv<T>(lds, u * (kRT + kRT / 16)), w));
+    }
+    mp += mstep;
+  }
 }
 
 // length-n transform src -> dst (both contiguous, global) as two tile phases
@@ -82,7 +139,6 @@ __device__ void two_level(LdsPtr<T> lds, const RaggedItem& it, const RaggedSched
   const RaggedSched& sb = scheds[it.sB];
   TwPtr<T> thi = tw + it.tw_hi;
   TwPtr<T> tlo = tw + it.tw_lo;
-  const uint32_t lo_mask = (1u << it.lo_bits) - 1u;
   // phase A: columns j1 (stride n1) of length n2, twiddle w_n^{j1*k2}, mid[j1 + n1*k2]
   {
     const int C = it.cA, logC = it.logcA;
@@ -92,15 +148,7 @@ __device__ void two_level(LdsPtr<T> lds, const RaggedItem& it, const RaggedSched
                    c0 + ca < n1, lds);
       __syncthreads();
       tile_stages<T, DIR>(lds, sa, C, logC, tw);
-      for (int f = tid; f < C * n2; f += kRT) {
-        const int c = f & (C - 1), k2 = f >> logC;
-        const int col = c0 + c;
-        if (col < n1) {
-          const uint32_t e = static_cast<uint32_t>(col) * static_cast<uint32_t>(k2);
-          const V2<T> w = cmul<T>(ldv<T>(thi, e >> it.lo_bits), ldv<T>(tlo, e & lo_mask));
-          stv(mid, col + static_cast<int64_t>(n1) * k2, cmul<T>(ldv<T>(lds, lpad(k2 * C + c)), w));
-        }
-      }
+      tile_store_tw<T>(lds, mid, thi, tlo, it.lo_bits, n1, C * n2, C, logC, c0);
       __syncthreads();
     }
   }

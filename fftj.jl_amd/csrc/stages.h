@@ -45,18 +45,50 @@ template <typename T> __device__ __forceinline__ GPtr<T> as_g(V2<T>* p) { return
 // passes for a 64-lane access.
 __device__ __forceinline__ int lpad(int f) { return f + (f >> 4); }
 
+// Values that are the same in every lane of a wave but reach an out-of-line
+// function in vector registers (its arguments): reading lane 0 puts them in
+// scalar registers, so the index arithmetic on them runs on the scalar unit
+// and the memory instructions take a scalar base.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+template <typename T> __device__ __forceinline__ LdsPtr<T> uni_lds(LdsPtr<T> p) {
+  const unsigned a = static_cast<unsigned>(reinterpret_cast<uintptr_t>(p));
+  return (LdsPtr<T>)(static_cast<uintptr_t>(static_cast<unsigned>(uni(static_cast<int>(a)))));
+}
+template <typename P> __device__ __forceinline__ P uni_ptr(P p) {
+  const uint64_t a = static_cast<uint64_t>(reinterpret_cast<uintptr_t>(p));
+  const uint64_t lo = static_cast<unsigned>(uni(static_cast<int>(a & 0xffffffffu)));
+  const uint64_t hi = static_cast<unsigned>(uni(static_cast<int>(a >> 32)));
+  return (P)(static_cast<uintptr_t>(lo | (hi << 32)));
+}
+
 // One radix stage, LDS -> registers -> LDS.  Kept out of line: each radix
 // then gets its own register allocation (inlined into the stage switch the
 // union of all radices' live ranges spills; out of line none does).  The
 // twiddle loads are issued first so their latency overlaps the LDS reads and
-// the barrier.
+// the barrier.  Every argument but c and jt is uniform over the workgroup.
+// A butterfly reads flat slots f + r*dr and writes g + r*dw (dr = C*n/R,
+// dw = C*ns); when a step is a multiple of 16, lpad is affine along it,
+// lpad(f + r*d) = lpad(f) + r*(d + d/16), and each access costs one add.
 template <typename T, int DIR, int R>
-__device__ __noinline__ void run_stage(const int n, const int C, const StageDesc s, LdsPtr<T> lds,
-                                       TwPtr<T> tw, const int c, const int jt,
-                                       const int jstep) {
+__device__ __noinline__ void run_stage(const int n_, const int C_, const StageDesc s, LdsPtr<T> lds_,
+                                       TwPtr<T> tw_, const int c, const int jt,
+                                       const int jstep_) {
   constexpr int BPT = kElemsPerThread / R;
+  const int n = uni(n_), C = uni(C_), ns = uni(s.ns), jstep = uni(jstep_);
+  const unsigned magic = static_cast<unsigned>(uni(static_cast<int>(s.ns_magic)));
+  const LdsPtr<T> lds = uni_lds<T>(lds_);
+  using TwBytes = const __attribute__((address_space(1))) char*;
+  const TwBytes twb = reinterpret_cast<TwBytes>(uni_ptr(tw_) + uni(s.tw_off));
   const int nr = n / R;
-  const int ns = s.ns;
+  const int dr = nr * C, dw = ns * C;
+  const bool lin_r = (dr & 15) == 0, lin_w = (dw & 15) == 0;
+  // per-r offsets of the affine case, formed on the scalar unit
+  int orr[R], orw[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    orr[r] = uni(r * (dr + (dr >> 4)));
+    orw[r] = uni(r * (dw + (dw >> 4)));
+  }
   V2<T> v[BPT][R];
   V2<T> w[BPT][R > 1 ? R - 1 : 1];
   int kk[BPT];
@@ -65,20 +97,31 @@ __device__ __noinline__ void run_stage(const int n, const int C, const StageDesc
     const int j = jt + b * jstep;
     kk[b] = 0;
     if (j < nr && ns > 1) {
-      const int q = static_cast<int>(__umulhi(static_cast<unsigned>(j), s.ns_magic));
-      const int k = j - q * ns;
+      const int q = static_cast<int>(__umulhi(static_cast<unsigned>(j), magic));
+      const int k = j - static_cast<int>(__umul24(static_cast<unsigned>(q), static_cast<unsigned>(ns)));
       kk[b] = k;
-      TwPtr<T> twp = tw + (s.tw_off + k);
 #pragma unroll
-      for (int r = 1; r < R; ++r) w[b][r - 1] = ldv<T>(twp, (r - 1) * ns);
+      for (int r = 1; r < R; ++r) {
+        // a 32-bit byte offset from the scalar base (the saddr form)
+        const unsigned off = static_cast<unsigned>(k + (r - 1) * ns) * static_cast<unsigned>(sizeof(NV2<T>));
+        const NV2<T> e = *reinterpret_cast<TwPtr<T>>(twb + off);
+        w[b][r - 1] = mk<T>(e.x, e.y);
+      }
     }
   }
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = jt + b * jstep;
     if (j < nr) {
+      const int f = static_cast<int>(__umul24(static_cast<unsigned>(j), static_cast<unsigned>(C))) + c;
+      if (lin_r) {
+        const LdsPtr<T> p = lds + lpad(f);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = ldv<T>(lds, lpad((j + r * nr) * C + c));
+        for (int r = 0; r < R; ++r) v[b][r] = ldv<T>(p, orr[r]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = ldv<T>(lds, lpad(f + r * dr));
+      }
     }
   }
   __syncthreads();
@@ -92,8 +135,15 @@ __device__ __noinline__ void run_stage(const int n, const int C, const StageDesc
       }
       dft<T, DIR, R>(v[b]);
       const int base = (j - kk[b]) * R + kk[b];
+      const int f = static_cast<int>(__umul24(static_cast<unsigned>(base), static_cast<unsigned>(C))) + c;
+      if (lin_w) {
+        const LdsPtr<T> p = lds + lpad(f);
 #pragma unroll
-      for (int r = 0; r < R; ++r) stv(lds, lpad((base + r * ns) * C + c), v[b][r]);
+        for (int r = 0; r < R; ++r) stv(p, orw[r], v[b][r]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) stv(lds, lpad(f + r * dw), v[b][r]);
+      }
     }
   }
   __syncthreads();
