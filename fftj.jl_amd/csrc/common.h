@@ -91,6 +91,11 @@ struct PassDesc {
   // w_k = exp(dir*2*pi*i*k/n), k = 0..N, in the plan's precision
   int32_t real_io;
   const void* rtw;
+  // real_io 3 / 4: the last (R2C) or first (C2R) pass of a multi-pass
+  // half-length transform of length real_h, with pair_M transforms per row;
+  // a group holds 16 transforms T and their 16 mirrors pair_M - T
+  int32_t pair_M;
+  int64_t real_h;
   // generic (non-power-of-two) pass: first stage fused into the HBM load,
   // last into the store (chains of two or more stages)
   int32_t fuse_io;

@@ -1075,6 +1075,44 @@ static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch
   return L;
 }
 
+// The edges of an even-length real transform fused into the last (R2C,
+// real_io 3) or first (C2R, real_io 4) pass of its multi-pass half-length
+// transform of h points, `rows` rows: that pass must be a power-of-two pass
+// whose transform b = T + M*row (M = h/n per row, a multiple of 32) sits at
+// T + row*h on the edge side with element stride M, and 32 of them must fit
+// one workgroup.  The edge side's row stride becomes h + 1.  On success the
+// step is rewritten; FFTJH_REAL_PAIRED=0 keeps the edges as launches of
+// their own.
+static bool pair_real_pass(Layout& L, Step& s, int64_t h, int64_t rows, int real_io, double scale) {
+  static const bool off = [] {
+    const char* e = std::getenv("FFTJH_REAL_PAIRED");
+    return e && std::atoi(e) == 0;
+  }();
+  if (off || s.kind != ST_PASS || !s.pow2 || s.mat) return false;
+  PassDesc& d = s.d;
+  if (h % d.n) return false;
+  const int64_t M = h / d.n;
+  if (M % 32 || M > (int64_t(1) << 30) || d.batch != M * rows || (real_io == 3 && d.tw_period != 0)) return false;
+  Geometry& g = real_io == 3 ? d.out : d.in;
+  const bool rowmap = g.stride == M && g.s_out == h &&
+                      ((d.Bi == 1 && d.B2 == M && g.s_mid == 1) ||
+                       (d.Bi * d.B2 == M && g.s_in == 1 && (d.B2 == 1 || g.s_mid == d.Bi)));
+  if (!rowmap) return false;
+  int t;
+  if (!pow2_fits(L.dtype, d.n, 32, &t)) return false;
+  g.s_out = h + 1;
+  d.C = 32;
+  d.logC = 5;
+  d.nthreads = t;
+  d.ngroups = static_cast<int>(d.batch / 32);
+  d.rows_in = d.rows_out = 0;
+  d.real_io = real_io;
+  d.pair_M = static_cast<int32_t>(M);
+  d.real_h = h;
+  d.scale = scale;
+  return true;
+}
+
 // Real-to-complex rows: `batch` rows of n reals at src (row stride n) ->
 // rows of n/2+1 complex at dst (row stride n/2+1), through the scratch roles
 // in `free` (front first; at least four).  Even n: the half-length complex
@@ -1088,6 +1126,15 @@ static void add_r2c_rows(Layout& L, int64_t n, int64_t batch, int src, int dst, 
     add_pass(L, -1, static_cast<int>(h), batch, 1, geo(1, 0, h), geo(1, 0, h + 1), 1.0, src, dst, PassTw(), 1, 1);
   } else if (n % 2 == 0) {
     const int64_t h = n / 2;
+    {
+      // the twist at the store of the half-length transform's last pass
+      Layout trial = L;
+      build_c2c_rows(trial, h, batch, -1, 1.0, src, dst, free);
+      if (pair_real_pass(trial, trial.steps.back(), h, batch, 3, 1.0)) {
+        L = std::move(trial);
+        return;
+      }
+    }
     need_scratch(L, free[0], h * batch);
     build_c2c_rows(L, h, batch, -1, 1.0, src, free[0], std::vector<int>(free.begin() + 1, free.end()));
     Step t;
@@ -1138,6 +1185,18 @@ static void add_c2r_rows(Layout& L, int64_t n, int64_t batch, bool normalized, i
              normalized ? 1.0 / static_cast<double>(n) : 1.0, src, dst, PassTw(), 1, 2);
   } else if (n % 2 == 0) {
     const int64_t h = n / 2;
+    {
+      // the pre-pass at the load of the half-length transform's first pass
+      // (its store carries the pre-factor, the last pass the 1/h)
+      Layout trial = L;
+      const size_t first = trial.steps.size();
+      build_c2c_rows(trial, h, batch, 1, normalized ? 1.0 / static_cast<double>(h) : 1.0, src, dst, free);
+      if (trial.steps.size() > first + 1 &&
+          pair_real_pass(trial, trial.steps[first], h, batch, 4, normalized ? 0.5 : 1.0)) {
+        L = std::move(trial);
+        return;
+      }
+    }
     need_scratch(L, free[0], h * batch);
     Step p;
     p.kind = ST_C2R_PRE;
@@ -1532,7 +1591,10 @@ static std::string describe(const Layout& L) {
       if (s.pow2) o << " kernel=pow2";
       if (s.d.fuse_io && s.radices.size() >= 2) o << " kernel=fused";
       if (s.mat) o << (s.mat == kMatF16 ? " kernel=mfma_f16" : s.mat == kMatBF16 ? " kernel=mfma_bf16" : " kernel=mfma_bf16x3");
-      if (s.d.real_io) o << (s.d.real_io == 1 ? " real=r2c" : " real=c2r");
+      if (s.d.real_io) {
+        static const char* kReal[] = {"", " real=r2c", " real=c2r", " real=r2c-paired", " real=c2r-paired"};
+        o << kReal[s.d.real_io];
+      }
       if (s.d.rows_in) o << " rows_in";
       if (s.d.rows_out) o << " rows_out";
       o << " batch=" << s.d.batch << " C=" << s.d.C << " threads=" << s.d.nthreads << " groups=" << s.d.ngroups
@@ -1638,7 +1700,7 @@ static void materialize(fftjh_plan_s& P) {
       }
       if (s.d.real_io) {
         // w_k = exp(dir*2*pi*i*k/n), k = 0..h, n = 2h (real.jl:26-39)
-        const int64_t h = s.d.n, n = 2 * h;
+        const int64_t h = s.d.real_io >= 3 ? s.d.real_h : s.d.n, n = 2 * h;
         s.aux = cached(key_of("real", {dev, double(dt), double(dir), double(n)}), (h + 1) * es,
                        [&](void* p) { hip_check(launch_tw_pow(dt, p, h + 1, 1, n, dir, 0), "twiddle kernel"); });
         s.d.rtw = s.aux->p;

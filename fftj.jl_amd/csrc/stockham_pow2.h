@@ -129,7 +129,9 @@ __device__ __forceinline__ void middle_stages(V2<T> (&v)[kE], V2<T>* lds, const 
 // IO: 0 = registers on both HBM sides, 1 = rows staged through LDS on the
 // way in, 2 = on the way out (separate instantiations keep the plain
 // kernel's register allocation small); 3 = R2C rows (register I/O, the
-// real twist at the store), 4 = C2R rows (the real pre-pass at the load)
+// real twist at the store), 4 = C2R rows (the real pre-pass at the load),
+// 5 / 6 = the same edges on the last / first pass of a multi-pass
+// half-length transform, partners exchanged between paired columns in LDS
 #ifndef FFTJH_P2_WPE
 #define FFTJH_P2_WPE 0
 #endif
@@ -155,8 +157,29 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
 
   V2<T> v[kE], pf[kE];
   const int64_t istep = static_cast<int64_t>(J) * ist, ostep = static_cast<int64_t>(J) * ost;
+  // transform of column cc in group gg.  Paired real edges (IO 5, 6): with
+  // M = pair_M transforms per row and G1 = M/32 groups per row, group
+  // bo*G1 + a holds T = 16a + i (columns i < 16) and their mirrors M - T
+  // (columns 16 + i); the slot of the mirror of T = 0 holds T = M/2, the
+  // other self-mirrored transform.
+  const int pM = d.pair_M;
+  auto bidx = [&](int gg, int cc) -> int64_t {
+    if constexpr (IO >= 5) {
+      const int G1 = pM >> 5;
+      const int bo = gg / G1, a = gg - bo * G1, i = cc & 15;
+      const int Tm = cc < 16 ? 16 * a + i : ((a == 0 && i == 0) ? (pM >> 1) : pM - 16 * a - i);
+      return static_cast<int64_t>(bo) * pM + Tm;
+    } else {
+      return static_cast<int64_t>(gg) * C + cc;
+    }
+  };
+  // column holding the mirror of this thread's transform in group gg
+  auto mirror_col = [&](int gg) {
+    const int a = gg % (pM >> 5);
+    return (a == 0 && (c & 15) == 0) ? c : (c ^ 16);
+  };
   auto load = [&](int g, V2<T> (&dst)[kE]) {
-    const int64_t b = static_cast<int64_t>(g) * C + c;
+    const int64_t b = bidx(g, c);
     if (b < d.batch) {
       const V2<T>* p = x + (BatchIndex(d, b).base(d.in) + jt * ist);
 #pragma unroll
@@ -231,6 +254,40 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     }
   };
 
+  // C2R edge of a multi-pass first pass (IO 6): column P = T of the
+  // half-length input j = P + M*j3 (j3 = jt + u*J here); the partner bin
+  // X_{h-j} is element N-1-j3 of column M-P (element (N-j3) mod N of column
+  // 0 for P = 0, and X_h itself for j = 0), exchanged through LDS; then the
+  // same pre-pass as IO 4
+  auto load_pair = [&](int gg, V2<T> (&dst)[kE]) {
+    load(gg, dst);
+#pragma unroll
+    for (int u = 0; u < kE; ++u) lds[lpad2((jt + u * J) * C + c)] = dst[u];
+    __syncthreads();
+    const int64_t b = bidx(gg, c);
+    const int P = static_cast<int>(b % pM);
+    const int pc = mirror_col(gg);
+    const V2<T>* w = static_cast<const V2<T>*>(d.rtw);
+    const V2<T>* xr = x + BatchIndex(d, b).base(d.in);
+#pragma unroll
+    for (int u = 0; u < kE; ++u) {
+      const int j3 = jt + u * J;
+      V2<T> xc;
+      if (P == 0 && j3 == 0) {
+        xc = xr[d.real_h];
+      } else {
+        const int jp = P == 0 ? ((N - j3) & (N - 1)) : (N - 1 - j3);
+        xc = lds[lpad2(jp * C + pc)];
+      }
+      const V2<T> a = dst[u];
+      const V2<T> cx = mk<T>(xc.x, -xc.y);
+      const V2<T> xe = cadd<T>(a, cx);
+      const V2<T> t = cmul<T>(csub<T>(a, cx), w[P + static_cast<int64_t>(pM) * j3]);
+      dst[u] = mk<T>(xe.x - t.y, xe.y + t.x);
+    }
+    __syncthreads();
+  };
+
   int g = walk.start;
   if (g >= walk.end) return;
   if constexpr (IO == 1) {
@@ -240,6 +297,8 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     __syncthreads();
   } else if constexpr (IO == 4) {
     load_pre(g, v);
+  } else if constexpr (IO == 6) {
+    load_pair(g, v);
   } else {
     load(g, v);
   }
@@ -258,11 +317,11 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     {
       constexpr int S = NST - 1;
       constexpr int R = Ch::get(S), NS = Ch::ns(S), OFF = Ch::off(S);
-      const int64_t b = static_cast<int64_t>(g) * C + c;
+      const int64_t b = bidx(g, c);
       const BatchIndex ix(d, b);
       // the multi-pass twiddle's base and step, formed while v is dead
       PostTwiddle pt;
-      if (IO <= 1 && d.tw_period) pt.init(d, ix, jt, J);
+      if ((IO <= 1 || IO == 6) && d.tw_period) pt.init(d, ix, jt, J);
       stage_gather<T, N, R>(v, lds, C, c, jt, J);
       __syncthreads();
       stage_twiddle<T, R, NS>(v, tw + OFF, jt, J);
@@ -302,6 +361,36 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
           }
         }
         __syncthreads();
+      } else if constexpr (IO == 5) {
+        // R2C edge of a multi-pass last pass: v[u] = Z_k, k = T + M*k1,
+        // k1 = jt + u*J; Z_{h-k} is element N-1-k1 of transform M-T
+        // ((N-k1) mod N of transform 0 for T = 0), from the mirror column
+        // through LDS; the twist as IO 3, y_k at T + M*k1 of the h+1 bins
+#pragma unroll
+        for (int u = 0; u < kE; ++u) lds[lpad2((jt + u * J) * C + c)] = v[u];
+        __syncthreads();
+        const int Tm = static_cast<int>(b % pM);
+        const int pc = mirror_col(g);
+        V2<T>* yr = y + ix.base(d.out);
+        const V2<T>* w = static_cast<const V2<T>*>(d.rtw);
+#pragma unroll
+        for (int u = 0; u < kE; ++u) {
+          const int k1 = jt + u * J;
+          const V2<T> z = v[u];
+          if (Tm == 0 && k1 == 0) {
+            yr[0] = mk<T>(z.x + z.y, T(0));
+            yr[d.real_h] = mk<T>(z.x - z.y, T(0));
+          } else {
+            const int kp = Tm == 0 ? ((N - k1) & (N - 1)) : (N - 1 - k1);
+            const V2<T> zc = lds[lpad2(kp * C + pc)];
+            const V2<T> cz = mk<T>(zc.x, -zc.y);
+            const T half = T(0.5);
+            const V2<T> xe = cscale<T>(cadd<T>(z, cz), half);
+            const V2<T> xo = cscale<T>(csub<T>(z, cz), half);
+            yr[k1 * ost] = cadd<T>(xe, cmul<T>(mk<T>(xo.y, -xo.x), w[Tm + static_cast<int64_t>(pM) * k1]));
+          }
+        }
+        __syncthreads();
       } else {
        if (b < d.batch) {
         V2<T>* yb = y + (ix.base(d.out) + jt * ost);
@@ -335,6 +424,8 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
       __syncthreads();
     } else if constexpr (IO == 4) {
       load_pre(gn, v);
+    } else if constexpr (IO == 6) {
+      load_pair(gn, v);
     } else {
       load(gn, v);
     }
@@ -364,11 +455,17 @@ template <typename T, int DIR, int N, int R1, int R2, int R3, int R4>
 hipError_t launch_p2(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
                      hipStream_t st) {
   if (d.real_io) {
-    // the real edges exist in one direction each: R2C forward, C2R inverse
+    // the real edges exist in one direction each: R2C forward, C2R inverse;
+    // the paired ones need 32 columns of N points in one workgroup
+    constexpr bool paired = 2 * N <= MaxThreadsP2<T>::value;
     if constexpr (DIR == -1) {
       if (d.real_io == 1) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 3>(d, dd, x, y, tw, st);
+      if constexpr (paired)
+        if (d.real_io == 3) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 5>(d, dd, x, y, tw, st);
     } else {
       if (d.real_io == 2) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 4>(d, dd, x, y, tw, st);
+      if constexpr (paired)
+        if (d.real_io == 4) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 6>(d, dd, x, y, tw, st);
     }
     return hipErrorInvalidValue;
   }
@@ -394,10 +491,15 @@ int resident_p2_io(const PassDesc& d) {
 template <typename T, int DIR, int N, int R1, int R2, int R3, int R4>
 int resident_p2(const PassDesc& d) {
   if (d.real_io) {
+    constexpr bool paired = 2 * N <= MaxThreadsP2<T>::value;
     if constexpr (DIR == -1) {
       if (d.real_io == 1) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 3>(d);
+      if constexpr (paired)
+        if (d.real_io == 3) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 5>(d);
     } else {
       if (d.real_io == 2) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 4>(d);
+      if constexpr (paired)
+        if (d.real_io == 4) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 6>(d);
     }
     return 0;
   }

@@ -89,6 +89,41 @@ def test_real_rows_batched(prec, n, batch):
         assert p.describe().count("\n  ") == 1
 
 
+# Multi-pass half lengths: the real edges ride on the last (R2C) or first
+# (C2R) pass of the half-length transform, a workgroup holding 16 of its
+# transforms T and their mirrors M - T (T = 0 and T = M/2 mirror themselves
+# and share group 0).  Every row checked, the inverse on an arbitrary
+# spectrum (imaginary DC and Nyquist parts) too.
+PAIRED = [("f32", 2**16, 2), ("f32", 2**17, 3), ("f32", 2**18, 1), ("f32", 3 * 2**17, 2), ("f32", 2**20, 2),
+          ("f32", 5 * 2**15, 2), ("f64", 2**16, 2), ("f64", 2**19, 1), ("f64", 2**21, 2)]
+
+
+@pytest.mark.parametrize("prec,n,batch", PAIRED)
+def test_real_multipass_paired(prec, n, batch):
+    T = torch_dtype(prec)
+    h = n // 2
+    x = F.rng.normal(n + 7, n * batch).astype(real_np(prec)).reshape(batch, n)
+    p = F.plan_rfft(T, n, batch=batch)
+    pi = F.plan_irfft(T, n, batch=batch)
+    pb = F.plan_brfft(T, n, batch=batch)
+    assert "paired" in p.describe() or "paired" in pi.describe()
+    Xd = p(to_dev(x))
+    X = to_host(Xd).reshape(batch, h + 1)
+    ref = O.rfft(x.astype(np.float64))
+    for b in range(batch):
+        assert rel_l2(X[b], ref[b]) <= tol_for(prec, n), b
+    assert rel_l2(to_host(pi(Xd)).reshape(batch, n), x) <= 2 * tol_for(prec, n)
+    Z = F.rng.uniform_complex(n + 9, (h + 1) * batch).astype(cplx_np(prec)).reshape(batch, h + 1)
+    Zd = to_dev(Z)
+    yi = to_host(pi(Zd)).reshape(batch, n)
+    yb = to_host(pb(Zd)).reshape(batch, n)
+    for b in range(batch):
+        zr = Z[b].astype(np.complex128)
+        assert rel_l2(yi[b], O.irfft(zr, n)) <= tol_for(prec, n), b
+        assert rel_l2(yb[b], O.irfft(zr, n, normalized=False)) <= tol_for(prec, n), b
+    assert np.array_equal(to_host(Zd), Z)   # the inverse leaves its input intact
+
+
 def test_config3_rfft_2pow22_x4():
     n, batch = 1 << 22, 4
     x = F.rng.normal(0, n * batch).astype(np.float32).reshape(batch, n)
@@ -96,8 +131,8 @@ def test_config3_rfft_2pow22_x4():
     yd = p(to_dev(x))
     y = to_host(yd)
     assert y.shape == (batch, n // 2 + 1)
-    ref = O.rfft(x[0].astype(np.float64))
-    assert rel_l2(y[0], ref) <= tol_for("f32", n)
+    for b in (0, batch - 1):
+        assert rel_l2(y[b], O.rfft(x[b].astype(np.float64))) <= tol_for("f32", n)
     back = to_host(F.plan_irfft(torch.float32, n, batch=batch)(yd))
     assert rel_l2(back, x) <= 2 * tol_for("f32", n)
 

@@ -84,10 +84,15 @@ def test_schedule_config3():
     assert s[0].startswith("rs_deconv") and "m=147456" in s[0] and s[-1].startswith("rs_interp")
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1000003], batch=4))
     assert "m=2097152" in s[0]
+    # the real edges ride on the half-length transform's last / first pass
+    # (paired columns), three launches each
     s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F32, [1 << 22], batch=4))
-    assert s[-1].startswith("r2c_twist")
+    assert len(s) == 3 and "real=r2c-paired" in s[-1] and "C=32 " in s[-1] and "->y " in s[-1]
     s = _passes(_lib.describe_request(_lib.KIND_C2R, _lib.F32, [1 << 22], batch=4))
-    assert s[0].startswith("c2r_pre")
+    assert len(s) == 3 and "real=c2r-paired" in s[0] and "x->" in s[0] and "scale=0.5" in s[0]
+    # a generic last pass keeps the twist as its own launch
+    s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F32, [3 * 2**17], batch=2))
+    assert s[-1].startswith("r2c_twist") and not any("paired" in p for p in s)
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [256, 256, 256]))
     assert len(s) == 3 and "in_stride=1 " in s[0] and "in_stride=256 " in s[1] and "in_stride=65536 " in s[2]
 
