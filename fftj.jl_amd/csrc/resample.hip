@@ -113,10 +113,32 @@ __global__ void k_rs_interp(const V2<T>* G, V2<T>* y, const T* __restrict__ coef
                             int64_t n, int64_t m, int w_rt, int64_t batch, int64_t Bi, Geometry gy, int run) {
   extern __shared__ __align__(16) unsigned char smem[];
   const T* coef = coef_g;
+  // degree 3 staged in LDS: each tap's Chebyshev coefficients a0..a3 become
+  // the monomial ones of the same cubic, a0 - a2 + (a1 - 3a3) t + 2a2 t^2 +
+  // 4a3 t^3 (coefficient growth at most 4), evaluated by Horner (3 FMAs a tap)
+  constexpr bool kHorner = LDS_COEF && DEG == 3;
+  // In LDS a piece's W*(DEG+1) values sit at a stride padded by 16 bytes:
+  // lanes read the pieces of their own remainders, effectively at random,
+  // and an unpadded power-of-two stride put every lane's 16-byte read on the
+  // same few banks.
+  constexpr int kPS = LDS_COEF ? W * (DEG + 1) + 16 / static_cast<int>(sizeof(T)) : W * (DEG + 1);
   if constexpr (LDS_COEF) {
     T* c = reinterpret_cast<T*>(smem);
-    const int cnt = pieces * W * (DEG + 1);
-    for (int e = static_cast<int>(threadIdx.x); e < cnt; e += static_cast<int>(blockDim.x)) c[e] = coef_g[e];
+    constexpr int kPer = W * (DEG + 1);
+    const int cnt = pieces * kPer;
+    if constexpr (kHorner) {
+      for (int e = static_cast<int>(threadIdx.x); e < cnt / 4; e += static_cast<int>(blockDim.x)) {
+        const T a0 = coef_g[4 * e], a1 = coef_g[4 * e + 1], a2 = coef_g[4 * e + 2], a3 = coef_g[4 * e + 3];
+        T* d = c + (4 * e / kPer) * kPS + (4 * e) % kPer;
+        d[0] = a0 - a2;
+        d[1] = a1 - T(3) * a3;
+        d[2] = T(2) * a2;
+        d[3] = T(4) * a3;
+      }
+    } else {
+      for (int e = static_cast<int>(threadIdx.x); e < cnt; e += static_cast<int>(blockDim.x))
+        c[(e / kPer) * kPS + e % kPer] = coef_g[e];
+    }
     __syncthreads();
     coef = c;
   }
@@ -131,7 +153,12 @@ __global__ void k_rs_interp(const V2<T>* G, V2<T>* y, const T* __restrict__ coef
   const int64_t wave = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) >> 6;
   const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * blockDim.x) >> 6;
   for (int64_t sp = wave; sp < batch * runs; sp += nwaves) {
-    const int64_t b = sp / runs;
+    // (sp, hence b, is the same in every lane of the wave)
+    const int64_t bl = sp / runs;
+    const int64_t b = static_cast<int64_t>(
+        (static_cast<uint64_t>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(bl >> 32))))
+         << 32) |
+        static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(bl & 0xffffffff))));
     int64_t k = (sp - b * runs) * span + lane;
     if (k >= n) continue;
     const int64_t km = k * m;
@@ -143,7 +170,7 @@ __global__ void k_rs_interp(const V2<T>* G, V2<T>* y, const T* __restrict__ coef
       int p = static_cast<int>(sd);
       if (p >= pieces) p = pieces - 1;
       const T t = static_cast<T>(2.0 * (sd - p) - 1.0);
-      const T* cp = coef + static_cast<int64_t>(p) * w * N;
+      const T* cp = coef + static_cast<int64_t>(p) * (W > 0 && DEG > 0 ? kPS : w * N);
       int64_t l = q - h + 1;
       if (l < 0) l += m;
       V2<T> acc = mk<T>(T(0), T(0));
@@ -151,14 +178,29 @@ __global__ void k_rs_interp(const V2<T>* G, V2<T>* y, const T* __restrict__ coef
         T a[W * (DEG + 1)];
 #pragma unroll
         for (int e = 0; e < W * (DEG + 1); ++e) a[e] = cp[e];
+        T wt[W];
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-          const T wt = clenshaw<T, DEG>(a + i * (DEG + 1), t, DEG);
-          int64_t li = l + i;
-          if (li >= m) li -= m;
-          const V2<T> gv = gb[li];
-          acc.x = fma(wt, gv.x, acc.x);
-          acc.y = fma(wt, gv.y, acc.y);
+          if constexpr (kHorner) {
+            const T* ai = a + i * 4;
+            wt[i] = fma(fma(fma(ai[3], t, ai[2]), t, ai[1]), t, ai[0]);
+          } else {
+            wt[i] = clenshaw<T, DEG>(a + i * (DEG + 1), t, DEG);
+          }
+        }
+        // taps l + i mod m in 32 bits (m < 2^31) from the row's base
+        V2<T> gv[W];
+        const unsigned l32 = static_cast<unsigned>(l), m32 = static_cast<unsigned>(m);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+          unsigned li = l32 + static_cast<unsigned>(i);
+          li = li >= m32 ? li - m32 : li;
+          gv[i] = gb[li];
+        }
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+          acc.x = fma(wt[i], gv[i].x, acc.x);
+          acc.y = fma(wt[i], gv[i].y, acc.y);
         }
       } else {
         for (int i = 0; i < w; ++i) {
@@ -230,7 +272,8 @@ static hipError_t launch_interp_t(const void* G, void* y, const void* coef, int 
   // the planner's usual fits: FP32 w = 8 degree 3 (table staged in LDS),
   // FP64 w = 16 degree 5
   if (w == 8 && degree == 3) {
-    const size_t lds = static_cast<size_t>(pieces) * 8 * 4 * sizeof(T);
+    // (the LDS table's piece stride: 32 values plus 16 bytes)
+    const size_t lds = static_cast<size_t>(pieces) * (8 * 4 * sizeof(T) + 16);
     if (lds <= 48 * 1024)
       hipLaunchKernelGGL((k_rs_interp<T, 8, 3, true>), gr, dim3(256), lds, st, g, yy, c, pieces, degree, n, m, w, batch,
                          Bi, gy, run);
