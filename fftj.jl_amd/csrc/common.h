@@ -96,6 +96,13 @@ struct PassDesc {
   // a group holds 16 transforms T and their 16 mirrors pair_M - T
   int32_t pair_M;
   int64_t real_h;
+  // rs_io = 1: the first pass of a resampled transform reads its m-grid
+  // straight from contiguous rows of rs_n inputs: g[j] = x[j]*corr[j] for
+  // j < hp = (rs_n+1)/2, x[j-(rs_m-rs_n)]*corr[j-(rs_m-rs_n)] for
+  // j >= rs_m - rs_n + hp, 0 between (the deconvolution, fused)
+  int32_t rs_io;
+  int64_t rs_n, rs_m;
+  const void* rs_corr;
   // generic (non-power-of-two) pass: first stage fused into the HBM load,
   // last into the store (chains of two or more stages)
   int32_t fuse_io;

@@ -966,6 +966,49 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
   const RsParams rp = rs_params(L.dtype, n);
   const int g = free[0];
   need_scratch(L, g, rp.m * batch);
+  {
+    // the deconvolution at the load of the m-point transform's first pass,
+    // when that is a power-of-two pass over the grid rows and x is rows
+    // (FFTJH_RS_FUSE=0: its own launch)
+    static const bool rs_fuse = [] {
+      const char* e = std::getenv("FFTJH_RS_FUSE");
+      return !e || std::atoi(e) != 0;
+    }();
+    if (rs_fuse && is_rows(gin, n, Bi) && gin.stride == 1 && rp.m < (int64_t(1) << 31)) {
+      Layout trial = L;
+      const size_t first = trial.steps.size();
+      build_c2c(trial, rp.m, batch, 1, geo(1, 0, rp.m), geo(1, 0, rp.m), dir, 1.0, g, g,
+                std::vector<int>(free.begin() + 1, free.end()));
+      Step& f = trial.steps[first];
+      if (f.kind == ST_PASS && f.pow2 && !f.mat && f.src == g && !f.d.rows_in && !f.d.real_io &&
+          f.d.in.s_out == rp.m && f.d.batch == batch * (rp.m / f.d.n)) {
+        f.src = src;
+        f.d.rs_io = 1;
+        f.d.rs_n = n;
+        f.d.rs_m = rp.m;
+        f.n = n;
+        f.m = rp.m;
+        f.w = rp.w;
+        f.beta = rp.beta;
+        f.scale = scale;
+        Step ip;
+        ip.kind = ST_RS_INTERP;
+        ip.src = g;
+        ip.dst = dst;
+        ip.dir = dir;
+        ip.n = n;
+        ip.m = rp.m;
+        ip.batch = batch;
+        ip.w = rp.w;
+        ip.beta = rp.beta;
+        ip.Bi = Bi;
+        ip.gb = gout;
+        trial.steps.push_back(ip);
+        L = std::move(trial);
+        return;
+      }
+    }
+  }
   Step dc;
   dc.kind = ST_RS_DECONV;
   dc.src = src;
@@ -1601,6 +1644,7 @@ static std::string describe(const Layout& L) {
         << " in_stride=" << s.d.in.stride << " out_stride=" << s.d.out.stride;
       if (s.d.tw_period) o << " twiddle=w_" << s.d.tw_period;
       if (s.d.scale != 1.0) o << " scale=" << s.d.scale;
+      if (s.d.rs_io) o << " deconv n=" << s.d.rs_n << " m=" << s.d.rs_m << " w=" << s.w;
     } else if (s.kind == ST_RAGGED) {
       const RaggedData& R = *s.rg;
       o << " transforms=" << R.items.size() << " direct=" << R.counts[0] << " two_level=" << R.counts[1]
@@ -1643,6 +1687,28 @@ static void materialize(fftjh_plan_s& P) {
   const int dt = L.dtype, es = esize(dt);
   for (Step& s : L.steps) {
     const double dev = P.device;
+    // corr_j = scale / (w * Phi(j'/m)) of a resampled length (deconvolution
+    // step, or the first pass that does it at its load)
+    auto rs_corr = [&](const Step& st) {
+      const int64_t n = st.n, m = st.m;
+      const int w = st.w;
+      const double beta = st.beta, scale = st.scale;
+      return cached(key_of("rs_corr", {dev, double(dt), double(n), double(m), double(w), beta, scale}), n * (es / 2),
+                    [&](void* p) {
+                      std::vector<double> zq, wq;
+                      gauss_legendre(2 * w + 20, zq, wq);
+                      DevBuf nodes(zq.size() * 2 * sizeof(double));
+                      hip_check(hipMemcpy(nodes.p, zq.data(), zq.size() * sizeof(double), hipMemcpyHostToDevice),
+                                "node upload");
+                      double* wqd = static_cast<double*>(nodes.p) + zq.size();
+                      hip_check(hipMemcpy(wqd, wq.data(), wq.size() * sizeof(double), hipMemcpyHostToDevice),
+                                "node upload");
+                      hip_check(launch_rs_corr(dt, p, n, m, w, beta, static_cast<const double*>(nodes.p), wqd,
+                                               static_cast<int>(zq.size()), scale, 0),
+                                "corr kernel");
+                      hip_check(hipDeviceSynchronize(), "corr kernel");
+                    });
+    };
     if (s.kind == ST_PASS) {
       // stage twiddle table, shared by every pass with the same chain
       std::ostringstream k;
@@ -1697,6 +1763,10 @@ static void materialize(fftjh_plan_s& P) {
         const int mode = s.mat;
         s.mat_tab = cached(mk.str(), mat_table_bytes(mode),
                            [&](void* p) { hip_check(launch_mat_table(mode, dir, p, 0), "operand table kernel"); });
+      }
+      if (s.d.rs_io) {
+        s.aux = rs_corr(s);
+        s.d.rs_corr = s.aux->p;
       }
       if (s.d.real_io) {
         // w_k = exp(dir*2*pi*i*k/n), k = 0..h, n = 2h (real.jl:26-39)
@@ -1781,24 +1851,7 @@ static void materialize(fftjh_plan_s& P) {
                        }
                      });
     } else if (s.kind == ST_RS_DECONV) {
-      const int64_t n = s.n, m = s.m;
-      const int w = s.w;
-      const double beta = s.beta, scale = s.scale;
-      s.aux = cached(key_of("rs_corr", {dev, double(dt), double(n), double(m), double(w), beta, scale}),
-                     n * (es / 2), [&](void* p) {
-                       std::vector<double> zq, wq;
-                       gauss_legendre(2 * w + 20, zq, wq);
-                       DevBuf nodes(zq.size() * 2 * sizeof(double));
-                       hip_check(hipMemcpy(nodes.p, zq.data(), zq.size() * sizeof(double), hipMemcpyHostToDevice),
-                                 "node upload");
-                       double* wqd = static_cast<double*>(nodes.p) + zq.size();
-                       hip_check(hipMemcpy(wqd, wq.data(), wq.size() * sizeof(double), hipMemcpyHostToDevice),
-                                 "node upload");
-                       hip_check(launch_rs_corr(dt, p, n, m, w, beta, static_cast<const double*>(nodes.p), wqd,
-                                                static_cast<int>(zq.size()), scale, 0),
-                                 "corr kernel");
-                       hip_check(hipDeviceSynchronize(), "corr kernel");
-                     });
+      s.aux = rs_corr(s);
     }
   }
   for (int i = 0; i < 4; ++i)

@@ -131,7 +131,8 @@ __device__ __forceinline__ void middle_stages(V2<T> (&v)[kE], V2<T>* lds, const 
 // kernel's register allocation small); 3 = R2C rows (register I/O, the
 // real twist at the store), 4 = C2R rows (the real pre-pass at the load),
 // 5 / 6 = the same edges on the last / first pass of a multi-pass
-// half-length transform, partners exchanged between paired columns in LDS
+// half-length transform, partners exchanged between paired columns in LDS;
+// 7 = the resampling deconvolution at the load of a first pass
 #ifndef FFTJH_P2_WPE
 #define FFTJH_P2_WPE 0
 #endif
@@ -288,6 +289,35 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     __syncthreads();
   };
 
+  // resampling deconvolution at the load (IO 7): element j of this
+  // transform's m-grid row, from x and the correction factors
+  auto load_rs = [&](int gg, V2<T> (&dst)[kE]) {
+    const int64_t b = bidx(gg, c);
+    if (b < d.batch) {
+      const BatchIndex ix(d, b);
+      // grid indices in 32 bits (the planner keeps m below 2^31)
+      const int j0 = static_cast<int>(ix.base(d.in) - ix.bo * d.in.s_out);
+      const int js = static_cast<int>(ist);
+      const V2<T>* xr = x + ix.bo * d.rs_n;
+      const T* cr = static_cast<const T*>(d.rs_corr);
+      const int hp = static_cast<int>((d.rs_n + 1) / 2), shift = static_cast<int>(d.rs_m - d.rs_n);
+      const int hi0 = shift + hp;
+#pragma unroll
+      for (int u = 0; u < kE; ++u) {
+        const int j = j0 + (jt + u * J) * js;
+        dst[u] = mk<T>(T(0), T(0));
+        if (j < hp) {
+          dst[u] = cscale<T>(xr[j], cr[j]);
+        } else if (j >= hi0) {
+          dst[u] = cscale<T>(xr[j - shift], cr[j - shift]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kE; ++u) dst[u] = mk<T>(T(0), T(0));
+    }
+  };
+
   int g = walk.start;
   if (g >= walk.end) return;
   if constexpr (IO == 1) {
@@ -299,6 +329,8 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     load_pre(g, v);
   } else if constexpr (IO == 6) {
     load_pair(g, v);
+  } else if constexpr (IO == 7) {
+    load_rs(g, v);
   } else {
     load(g, v);
   }
@@ -321,7 +353,7 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
       const BatchIndex ix(d, b);
       // the multi-pass twiddle's base and step, formed while v is dead
       PostTwiddle pt;
-      if ((IO <= 1 || IO == 6) && d.tw_period) pt.init(d, ix, jt, J);
+      if ((IO <= 1 || IO >= 6) && d.tw_period) pt.init(d, ix, jt, J);
       stage_gather<T, N, R>(v, lds, C, c, jt, J);
       __syncthreads();
       stage_twiddle<T, R, NS>(v, tw + OFF, jt, J);
@@ -426,6 +458,8 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
       load_pre(gn, v);
     } else if constexpr (IO == 6) {
       load_pair(gn, v);
+    } else if constexpr (IO == 7) {
+      load_rs(gn, v);
     } else {
       load(gn, v);
     }
@@ -469,6 +503,7 @@ hipError_t launch_p2(const PassDesc& d, const PassDesc* dd, const void* x, void*
     }
     return hipErrorInvalidValue;
   }
+  if (d.rs_io) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 7>(d, dd, x, y, tw, st);
   if (d.rows_in) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 1>(d, dd, x, y, tw, st);
   if (d.rows_out) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 2>(d, dd, x, y, tw, st);
   return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 0>(d, dd, x, y, tw, st);
@@ -503,6 +538,7 @@ int resident_p2(const PassDesc& d) {
     }
     return 0;
   }
+  if (d.rs_io) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 7>(d);
   if (d.rows_in) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 1>(d);
   if (d.rows_out) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 2>(d);
   return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 0>(d);

@@ -80,10 +80,16 @@ def test_schedule_config2():
 
 
 def test_schedule_config3():
+    # the deconvolution rides on the first (power-of-two) pass of the m-point
+    # transform; the interpolation is its own launch
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [65537], batch=4))
-    assert s[0].startswith("rs_deconv") and "m=147456" in s[0] and s[-1].startswith("rs_interp")
+    assert len(s) == 4 and s[0].startswith("pass x->") and "deconv n=65537 m=147456" in s[0]
+    assert s[-1].startswith("rs_interp")
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1000003], batch=4))
-    assert "m=2097152" in s[0]
+    assert "deconv n=1000003 m=2097152" in s[0]
+    # a generic first pass keeps the deconvolution as a launch of its own
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F64, [10007], batch=3))
+    assert s[0].startswith("rs_deconv") and "deconv" not in s[1]
     # the real edges ride on the half-length transform's last / first pass
     # (paired columns), three launches each
     s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F32, [1 << 22], batch=4))
@@ -154,17 +160,22 @@ def test_route_flags():
     s = _passes(L.describe_tuned(L.F32, (n,), 1, -1, L.make_tuning((), (), L.FLAG_ROUTE_COMPOSITE)))
     assert s[0].startswith("pass x->y n=64 ") and "in_stride=10007 " in s[0] and f"twiddle=w_{n}" in s[0]
     assert s[1].startswith("rs_deconv y->") and "n=10007 " in s[1] and s[-1].startswith("rs_interp") and "->y" in s[-1]
+    def resampled_from_x(step, n):
+        # the deconvolution as its own launch, or at the load of the first pass
+        return (step.startswith("rs_deconv x->") and f"n={n} " in step) or \
+            (step.startswith("pass x->") and f" deconv n={n} " in step)
+
     s = _passes(L.describe_tuned(L.F32, (n,), 1, -1, L.make_tuning((), (), L.FLAG_ROUTE_RESAMPLE)))
-    assert s[0].startswith("rs_deconv x->") and f"n={n} " in s[0]
+    assert resampled_from_x(s[0], n)
     # the planner's own choice: composite where the resampled length-p
     # transforms fit one pass and the whole length's would not
     s = _passes(L.describe_request(L.KIND_C2C, L.F32, [100 * 1009]))
     assert s[0].startswith("pass x->y n=100 ") and s[1].startswith("rs_deconv")
     s = _passes(L.describe_request(L.KIND_C2C, L.F32, [2 * 10007]))
-    assert s[0].startswith("rs_deconv x->")
+    assert resampled_from_x(s[0], 2 * 10007)
     # a prime has no smooth part: both routes resample it
     s = _passes(L.describe_tuned(L.F32, (65537,), 1, -1, L.make_tuning((), (), L.FLAG_ROUTE_COMPOSITE)))
-    assert s[0].startswith("rs_deconv")
+    assert resampled_from_x(s[0], 65537)
     with pytest.raises(L.ArgumentError):
         L.describe_tuned(L.F32, (n,), 1, -1, L.make_tuning((), (), L.FLAG_ROUTE_COMPOSITE | L.FLAG_ROUTE_RESAMPLE))
     with pytest.raises(F.ArgumentError):
