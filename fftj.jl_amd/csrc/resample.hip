@@ -14,6 +14,7 @@
 // at plan time (the planner, fit_rs_weights), evaluated in the plan's
 // precision from the exact integer remainder r = k*m mod n.
 #include <algorithm>
+#include <type_traits>
 
 #include "butterflies.h"
 #include "common.h"
@@ -269,17 +270,24 @@ static hipError_t launch_interp_t(const void* G, void* y, const void* coef, int 
   const auto* g = static_cast<const V2<T>*>(G);
   auto* yy = static_cast<V2<T>*>(y);
   const auto* c = static_cast<const T*>(coef);
-  // the planner's usual fits: FP32 w = 8 degree 3 (table staged in LDS),
-  // FP64 w = 16 degree 5
-  if (w == 8 && degree == 3) {
-    // (the LDS table's piece stride: 32 values plus 16 bytes)
-    const size_t lds = static_cast<size_t>(pieces) * (8 * 4 * sizeof(T) + 16);
+  // the planner's usual fits: FP32 w = 8 degree 3 to 5 (table staged in
+  // LDS, its piece stride W*(DEG+1) values plus 16 bytes), FP64 w = 16 degree 5
+  auto w8 = [&](auto deg) {
+    constexpr int D = decltype(deg)::value;
+    const size_t lds = static_cast<size_t>(pieces) * (8 * (D + 1) * sizeof(T) + 16);
     if (lds <= 48 * 1024)
-      hipLaunchKernelGGL((k_rs_interp<T, 8, 3, true>), gr, dim3(256), lds, st, g, yy, c, pieces, degree, n, m, w, batch,
+      hipLaunchKernelGGL((k_rs_interp<T, 8, D, true>), gr, dim3(256), lds, st, g, yy, c, pieces, degree, n, m, w, batch,
                          Bi, gy, run);
     else
-      hipLaunchKernelGGL((k_rs_interp<T, 8, 3, false>), gr, dim3(256), 0, st, g, yy, c, pieces, degree, n, m, w, batch,
+      hipLaunchKernelGGL((k_rs_interp<T, 8, D, false>), gr, dim3(256), 0, st, g, yy, c, pieces, degree, n, m, w, batch,
                          Bi, gy, run);
+  };
+  if (w == 8 && degree == 3) {
+    w8(std::integral_constant<int, 3>());
+  } else if (w == 8 && degree == 4) {
+    w8(std::integral_constant<int, 4>());
+  } else if (w == 8 && degree == 5) {
+    w8(std::integral_constant<int, 5>());
   } else if (w == 16 && degree == 5) {
     hipLaunchKernelGGL((k_rs_interp<T, 16, 5, false>), gr, dim3(256), 0, st, g, yy, c, pieces, degree, n, m, w, batch,
                        Bi, gy, run);
