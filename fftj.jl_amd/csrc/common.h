@@ -93,9 +93,13 @@ struct PassDesc {
   const void* rtw;
   // real_io 3 / 4: the last (R2C) or first (C2R) pass of a multi-pass
   // half-length transform of length real_h, with pair_M transforms per row;
-  // a group holds 16 transforms T and their 16 mirrors pair_M - T
+  // a group holds 16 transforms T and their 16 mirrors pair_M - T.  rtw is
+  // then an FP64 two-level table of w_{2h}^e: w = rtw[e >> rtw_sh] *
+  // rtw[rtw_hi + (e & (2^rtw_sh - 1))]
   int32_t pair_M;
   int64_t real_h;
+  int32_t rtw_sh;
+  int64_t rtw_hi;
   // rs_io = 1: the first pass of a resampled transform reads its m-grid
   // straight from contiguous rows of rs_n inputs: g[j] = x[j]*corr[j] for
   // j < hp = (rs_n+1)/2, x[j-(rs_m-rs_n)]*corr[j-(rs_m-rs_n)] for

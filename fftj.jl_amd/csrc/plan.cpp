@@ -1153,6 +1153,10 @@ static bool pair_real_pass(Layout& L, Step& s, int64_t h, int64_t rows, int real
   d.pair_M = static_cast<int32_t>(M);
   d.real_h = h;
   d.scale = scale;
+  int bits = 0;
+  while ((int64_t(1) << bits) < 2 * h) ++bits;
+  d.rtw_sh = (bits + 1) / 2;
+  d.rtw_hi = ((2 * h - 1) >> d.rtw_sh) + 1;  // entries of the high table
   return true;
 }
 
@@ -1768,9 +1772,20 @@ static void materialize(fftjh_plan_s& P) {
         s.aux = rs_corr(s);
         s.d.rs_corr = s.aux->p;
       }
-      if (s.d.real_io) {
+      if (s.d.real_io >= 3) {
+        // paired edges: w_{2h}^e = exp(dir*2*pi*i*e/(2h)) from an FP64
+        // two-level table (w^(i*2^sh) for i < rtw_hi, then w^i for i < 2^sh)
+        const int64_t P = 2 * s.d.real_h, hi = s.d.rtw_hi, lo = int64_t(1) << s.d.rtw_sh;
+        std::ostringstream rk;
+        rk << "ptw:" << dev << ":" << dir << ":" << P;
+        s.aux = cached(rk.str(), (hi + lo) * 16, [&](void* p) {
+          hip_check(launch_tw_pow(FFTJH_F64, p, hi, lo, P, dir, 0), "twiddle kernel");
+          hip_check(launch_tw_pow(FFTJH_F64, static_cast<char*>(p) + hi * 16, lo, 1, P, dir, 0), "twiddle kernel");
+        });
+        s.d.rtw = s.aux->p;
+      } else if (s.d.real_io) {
         // w_k = exp(dir*2*pi*i*k/n), k = 0..h, n = 2h (real.jl:26-39)
-        const int64_t h = s.d.real_io >= 3 ? s.d.real_h : s.d.n, n = 2 * h;
+        const int64_t h = s.d.n, n = 2 * h;
         s.aux = cached(key_of("real", {dev, double(dt), double(dir), double(n)}), (h + 1) * es,
                        [&](void* p) { hip_check(launch_tw_pow(dt, p, h + 1, 1, n, dir, 0), "twiddle kernel"); });
         s.d.rtw = s.aux->p;

@@ -255,6 +255,12 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     }
   };
 
+  // the paired real edges' twiddle w_{2h}^e from the FP64 two-level table
+  auto rtw_at = [&](uint64_t e) -> double2 {
+    const double2* t = static_cast<const double2*>(d.rtw);
+    const uint64_t mask = (uint64_t(1) << d.rtw_sh) - 1;
+    return cmul64(t[e >> d.rtw_sh], t[d.rtw_hi + static_cast<int64_t>(e & mask)]);
+  };
   // C2R edge of a multi-pass first pass (IO 6): column P = T of the
   // half-length input j = P + M*j3 (j3 = jt + u*J here); the partner bin
   // X_{h-j} is element N-1-j3 of column M-P (element (N-j3) mod N of column
@@ -268,11 +274,15 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     const int64_t b = bidx(gg, c);
     const int P = static_cast<int>(b % pM);
     const int pc = mirror_col(gg);
-    const V2<T>* w = static_cast<const V2<T>*>(d.rtw);
+    // w_j, j = P + M*(jt + u*J): FP64 start and step, stepped per element
+    double2 wv = rtw_at(static_cast<uint64_t>(P) + static_cast<uint64_t>(pM) * jt);
+    const double2 ws = rtw_at(static_cast<uint64_t>(pM) * J);
     const V2<T>* xr = x + BatchIndex(d, b).base(d.in);
 #pragma unroll
     for (int u = 0; u < kE; ++u) {
       const int j3 = jt + u * J;
+      const V2<T> wj = mk<T>(static_cast<T>(wv.x), static_cast<T>(wv.y));
+      wv = cmul64(wv, ws);
       V2<T> xc;
       if (P == 0 && j3 == 0) {
         xc = xr[d.real_h];
@@ -283,7 +293,7 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
       const V2<T> a = dst[u];
       const V2<T> cx = mk<T>(xc.x, -xc.y);
       const V2<T> xe = cadd<T>(a, cx);
-      const V2<T> t = cmul<T>(csub<T>(a, cx), w[P + static_cast<int64_t>(pM) * j3]);
+      const V2<T> t = cmul<T>(csub<T>(a, cx), wj);
       dst[u] = mk<T>(xe.x - t.y, xe.y + t.x);
     }
     __syncthreads();
@@ -404,11 +414,14 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
         const int Tm = static_cast<int>(b % pM);
         const int pc = mirror_col(g);
         V2<T>* yr = y + ix.base(d.out);
-        const V2<T>* w = static_cast<const V2<T>*>(d.rtw);
+        double2 wv = rtw_at(static_cast<uint64_t>(Tm) + static_cast<uint64_t>(pM) * jt);
+        const double2 ws = rtw_at(static_cast<uint64_t>(pM) * J);
 #pragma unroll
         for (int u = 0; u < kE; ++u) {
           const int k1 = jt + u * J;
           const V2<T> z = v[u];
+          const V2<T> wk = mk<T>(static_cast<T>(wv.x), static_cast<T>(wv.y));
+          wv = cmul64(wv, ws);
           if (Tm == 0 && k1 == 0) {
             yr[0] = mk<T>(z.x + z.y, T(0));
             yr[d.real_h] = mk<T>(z.x - z.y, T(0));
@@ -419,7 +432,7 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
             const T half = T(0.5);
             const V2<T> xe = cscale<T>(cadd<T>(z, cz), half);
             const V2<T> xo = cscale<T>(csub<T>(z, cz), half);
-            yr[k1 * ost] = cadd<T>(xe, cmul<T>(mk<T>(xo.y, -xo.x), w[Tm + static_cast<int64_t>(pM) * k1]));
+            yr[k1 * ost] = cadd<T>(xe, cmul<T>(mk<T>(xo.y, -xo.x), wk));
           }
         }
         __syncthreads();
