@@ -177,20 +177,26 @@ __device__ void two_level(LdsPtr<T> lds, const RaggedItem& it, const RaggedSched
 __device__ __forceinline__ double es_weight(double z, double beta) {
   return fabs(z) >= 1.0 ? 0.0 : exp(beta * (sqrt(1.0 - z * z) - 1.0));
 }
+// the same weight in single precision, for FP32 plans (relative error a few
+// ulp of FP32, below the plans' tolerance)
+__device__ __forceinline__ float es_weight(float z, float beta) {
+  return fabsf(z) >= 1.0f ? 0.0f : expf(beta * (sqrtf(1.0f - z * z) - 1.0f));
+}
 
 // y[k] = sum_{i<w} psi((delta + h - 1 - i)/h) * G[(q - h + 1 + i) mod m],  k*m = q*n + r
 template <typename T, bool FROM_LDS>
 __device__ void interp(const RaggedItem& it, const V2<T>* G, V2<T>* y) {
   const int n = it.n, m = it.m, w = it.w, h = it.w / 2;
+  const T beta = static_cast<T>(it.beta), ih = T(1) / static_cast<T>(h);
   for (int k = static_cast<int>(threadIdx.x); k < n; k += kRT) {
     const int64_t km = static_cast<int64_t>(k) * m;
     const int64_t q = km / n, r = km - q * n;
-    const double delta = static_cast<double>(r) / n;
+    const T delta = static_cast<T>(static_cast<double>(r) / n);
     int l = static_cast<int>(q) - h + 1;
     if (l < 0) l += m;
     V2<T> acc = mk<T>(T(0), T(0));
     for (int i = 0; i < w; ++i) {
-      const T wt = static_cast<T>(es_weight((delta + (h - 1 - i)) / h, it.beta));
+      const T wt = es_weight((delta + static_cast<T>(h - 1 - i)) * ih, beta);
       int li = l + i;
       if (li >= m) li -= m;
       const V2<T> g = FROM_LDS ? G[lpad(li)] : G[li];
