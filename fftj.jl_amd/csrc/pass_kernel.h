@@ -14,25 +14,45 @@ namespace fftjh {
 // into registers and scatter the outputs j*R + r to LDS.  Saves the load
 // phase's LDS round trip and barrier.
 template <typename T, int DIR, int R>
-__device__ __noinline__ void first_stage(const int n, const int C, TwPtr<T> xb, const int64_t ist, const bool valid,
-                                         LdsPtr<T> lds, const int c, const int jt, const int jstep) {
+__device__ __noinline__ void first_stage(const int n_, const int C_, TwPtr<T> xb, const int64_t ist_, const bool valid,
+                                         LdsPtr<T> lds_, const int c, const int jt, const int jstep_) {
   constexpr int BPT = kElemsPerThread / R;
+  // (n, C, ist, lds and jstep are uniform: scalar registers)
+  const int n = uni(n_), C = uni(C_), jstep = uni(jstep_);
+  const int64_t ist = uni64(ist_);
+  const LdsPtr<T> lds = uni_lds<T>(lds_);
   const int nr = n / R;
+  const int64_t rstep = static_cast<int64_t>(nr) * ist;  // between a butterfly's inputs
   V2<T> v[BPT][R];
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = jt + b * jstep;
+    const bool on = valid && j < nr;
+    TwPtr<T> p = xb + static_cast<int64_t>(j) * ist;
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-      v[b][r] = (valid && j < nr) ? ldv<T>(xb, static_cast<int64_t>(j + r * nr) * ist) : mk<T>(T(0), T(0));
+    for (int r = 0; r < R; ++r) {
+      v[b][r] = on ? ldv<T>(p, 0) : mk<T>(T(0), T(0));
+      p += rstep;
+    }
   }
+  // outputs j*R + r of column c: flat slots (j*R)*C + c + r*C, affine in r
+  // when C is a multiple of 16
+  const bool lin = (C & 15) == 0;
+  const int pc = uni(C + (C >> 4));
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = jt + b * jstep;
     if (j < nr) {
       dft<T, DIR, R>(v[b]);
+      const int f = static_cast<int>(__umul24(static_cast<unsigned>(j * R), static_cast<unsigned>(C))) + c;
+      if (lin) {
+        const LdsPtr<T> q = lds + lpad(f);
 #pragma unroll
-      for (int r = 0; r < R; ++r) stv(lds, lpad((j * R + r) * C + c), v[b][r]);
+        for (int r = 0; r < R; ++r) stv(q, r * pc, v[b][r]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) stv(lds, lpad(f + r * C), v[b][r]);
+      }
     }
   }
   __syncthreads();
@@ -47,23 +67,43 @@ __device__ __noinline__ void first_stage(const int n, const int C, TwPtr<T> xb, 
 // (jstep), each as separate arguments (in registers; a struct would go
 // through the stack): FP64 recurrences here, one rounding per output.
 template <typename T, int DIR, int R>
-__device__ __noinline__ void last_stage(const int n, const int C, const StageDesc s, LdsPtr<T> lds, TwPtr<T> tw,
-                                        GPtr<T> yb, const int64_t ost, const bool valid, const T scale,
+__device__ __noinline__ void last_stage(const int n_, const int C_, const StageDesc s, LdsPtr<T> lds_, TwPtr<T> tw_,
+                                        GPtr<T> yb, const int64_t ost_, const bool valid, const T scale,
                                         const bool scaled, const bool twon, const double2 w0, const double2 Sr,
-                                        const double2 Sb, const int c, const int jt, const int jstep) {
+                                        const double2 Sb, const int c, const int jt, const int jstep_) {
   constexpr int BPT = kElemsPerThread / R;
-  const int nr = n / R, ns = s.ns;
+  const int n = uni(n_), C = uni(C_), jstep = uni(jstep_), ns = uni(s.ns);
+  const int64_t ost = uni64(ost_);
+  const LdsPtr<T> lds = uni_lds<T>(lds_);
+  using TwBytes = const __attribute__((address_space(1))) char*;
+  const TwBytes twb = reinterpret_cast<TwBytes>(uni_ptr(tw_) + uni(s.tw_off));
+  const int nr = n / R;
+  const int64_t ostep = static_cast<int64_t>(ns) * ost;  // between a butterfly's outputs
+  // inputs j + r*nr of column c: flat slots j*C + c + r*(nr*C)
+  const int dr = nr * C;
+  const bool lin = (dr & 15) == 0;
+  const int pr = uni(dr + (dr >> 4));
   V2<T> v[BPT][R];
   V2<T> w[BPT][R - 1];
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = jt + b * jstep;
     if (j < nr) {
-      TwPtr<T> twp = tw + (s.tw_off + j);
 #pragma unroll
-      for (int r = 1; r < R; ++r) w[b][r - 1] = ldv<T>(twp, (r - 1) * ns);
+      for (int r = 1; r < R; ++r) {
+        const unsigned off = static_cast<unsigned>(j + (r - 1) * ns) * static_cast<unsigned>(sizeof(NV2<T>));
+        const NV2<T> e = *reinterpret_cast<TwPtr<T>>(twb + off);
+        w[b][r - 1] = mk<T>(e.x, e.y);
+      }
+      const int f = static_cast<int>(__umul24(static_cast<unsigned>(j), static_cast<unsigned>(C))) + c;
+      if (lin) {
+        const LdsPtr<T> q = lds + lpad(f);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = ldv<T>(lds, lpad((j + r * nr) * C + c));
+        for (int r = 0; r < R; ++r) v[b][r] = ldv<T>(q, r * pr);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = ldv<T>(lds, lpad(f + r * dr));
+      }
     }
   }
   if (!valid) return;
@@ -77,11 +117,13 @@ __device__ __noinline__ void last_stage(const int n, const int C, const StageDes
         for (int r = 1; r < R; ++r) v[b][r] = cmul<T>(v[b][r], w[b][r - 1]);
         dft<T, DIR, R>(v[b]);
         double2 wr = wb;
+        GPtr<T> p = yb + static_cast<int64_t>(j) * ost;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           V2<T> val = cmul<T>(v[b][r], mk<T>(static_cast<T>(wr.x), static_cast<T>(wr.y)));
           if (scaled) val = cscale<T>(val, scale);
-          stv(yb, static_cast<int64_t>(j + r * ns) * ost, val);
+          stv(p, 0, val);
+          p += ostep;
           wr = cmul64(wr, Sr);
         }
       }
@@ -95,9 +137,12 @@ __device__ __noinline__ void last_stage(const int n, const int C, const StageDes
 #pragma unroll
         for (int r = 1; r < R; ++r) v[b][r] = cmul<T>(v[b][r], w[b][r - 1]);
         dft<T, DIR, R>(v[b]);
+        GPtr<T> p = yb + static_cast<int64_t>(j) * ost;
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-          stv(yb, static_cast<int64_t>(j + r * ns) * ost, scaled ? cscale<T>(v[b][r], scale) : v[b][r]);
+        for (int r = 0; r < R; ++r) {
+          stv(p, 0, scaled ? cscale<T>(v[b][r], scale) : v[b][r]);
+          p += ostep;
+        }
       }
     }
   }

@@ -50,6 +50,12 @@ __device__ __forceinline__ int lpad(int f) { return f + (f >> 4); }
 // scalar registers, so the index arithmetic on them runs on the scalar unit
 // and the memory instructions take a scalar base.
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uni64(int64_t x) {
+  const uint64_t u = static_cast<uint64_t>(x);
+  const uint64_t lo = static_cast<unsigned>(uni(static_cast<int>(u & 0xffffffffu)));
+  const uint64_t hi = static_cast<unsigned>(uni(static_cast<int>(u >> 32)));
+  return static_cast<int64_t>(lo | (hi << 32));
+}
 template <typename T> __device__ __forceinline__ LdsPtr<T> uni_lds(LdsPtr<T> p) {
   const unsigned a = static_cast<unsigned>(reinterpret_cast<uintptr_t>(p));
   return (LdsPtr<T>)(static_cast<uintptr_t>(static_cast<unsigned>(uni(static_cast<int>(a)))));
