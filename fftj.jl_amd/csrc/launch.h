@@ -15,8 +15,11 @@ int max_threads(int dtype);
 // workgroups of this pass that fit the device at once (0 if unknown)
 int pass_resident_groups(int dtype, int dir, const PassDesc& d);
 
-// compile-time-chain kernels for power-of-two lengths 256..16384
+// compile-time-chain kernels for power-of-two lengths 32..16384
 bool pow2_chain(int n, int* radices, int* count);
+// ... and for the listed other lengths: the chain and its elements per
+// thread (16 for the powers of two)
+bool fixed_chain(int n, int* radices, int* count, int* elems);
 hipError_t launch_pass_pow2(int dtype, int dir, const PassDesc& d, const PassDesc* dev_desc, const void* x,
                             void* y, const void* tw, hipStream_t stream);
 int pass_pow2_resident_groups(int dtype, int dir, const PassDesc& d);

@@ -246,10 +246,19 @@ static bool pass_fits_rows_ok(int dtype, int n, int C, int cap) {
   return static_cast<int64_t>(cap) * (n / 16) >= 64;
 }
 
-// the power-of-two kernel: 16 elements per thread, C*n/16 threads
+// elements per thread of the compile-time-chain kernel for length n (16 for
+// the powers of two), 0 when n has none
+static int chain_elems(int n) {
+  int pr[8], pc = 0, e = 0;
+  return fixed_chain(n, pr, &pc, &e) ? e : 0;
+}
+
+// the compile-time-chain kernel: E elements per thread, C*n/E threads
 static bool pow2_fits(int dtype, int n, int C, int* threads) {
   if (static_cast<int64_t>(lds_elems(C * n)) * esize(dtype) > kLdsBytes) return false;
-  const int t = C * (n / 16);
+  const int E = chain_elems(n);
+  if (!E) return false;
+  const int t = C * (n / E);
   if (t > pow2_max_threads(dtype) || t % 64) return false;
   if (threads) *threads = t;
   return true;
@@ -420,15 +429,23 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   }
   if (n > 1 && s.radices.empty()) fail(FFTJH_ERR_UNSUPPORTED, "pass length has a prime factor above 13");
   if (static_cast<int>(s.radices.size()) > kMaxStages) fail(FFTJH_ERR_UNSUPPORTED, "too many stages");
-  // power-of-two lengths with a compile-time chain take the fused kernel
+  // lengths with a compile-time chain (the powers of two, and the listed
+  // others: FFTJH_FIXED_CHAINS=0 leaves those to the runtime chain) take the
+  // fused kernel
   {
-    int pr[8], pc = 0;
+    int pr[8], pc = 0, pe = 0;
     static const bool no_pow2 = std::getenv("FFTJH_NO_POW2") != nullptr;
-    if (!no_pow2 && !L.mat && !(pt && pt->nradix > 0) && pow2_chain(n, pr, &pc)) {
+    static const bool fixed_env = [] {
+      const char* e = std::getenv("FFTJH_FIXED_CHAINS");
+      return !e || std::atoi(e) != 0;
+    }();
+    const bool p2 = pow2_chain(n, pr, &pc);
+    if (!no_pow2 && !L.mat && !(pt && pt->nradix > 0) && (p2 || (fixed_env && fixed_chain(n, pr, &pc, &pe)))) {
       s.pow2 = true;
       s.radices.assign(pr, pr + pc);
     }
   }
+  const int chainE = s.pow2 ? chain_elems(n) : 16;
   // transforms per workgroup: strided passes take adjacent columns for
   // contiguous HBM runs; row passes group short transforms to fill a workgroup
   const bool colmode = in.stride > 1 || out.stride > 1;
@@ -500,11 +517,12 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
       }
       if (bestC) break;
     }
-    for (int C = colmode ? 16 : 64; C >= 1; C >>= 1) {
+    // (a fixed chain of E != 16 elements may need 32 columns to fill waves)
+    for (int C = colmode ? (chainE != 16 ? 64 : 16) : 64; C >= 1; C >>= 1) {
       int t;
       if (!fits(C, &t)) continue;
       // more columns than transforms only to fill one wave (idle columns)
-      if (C > 1 && C > batch && !(s.pow2 && (C / 2) * (n / 16) < 64)) continue;
+      if (C > 1 && C > batch && !(s.pow2 && (C / 2) * (n / chainE) < 64)) continue;
       if (!colmode && t > 256 && C > 1) continue;
       if (!colmode && C > row_cap && t >= 64 * (C / row_cap) && pass_fits_rows_ok(L.dtype, n, C, row_cap)) continue;
       bestC = C;
@@ -527,7 +545,7 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   // (measured slower than register I/O at the variant's register cost: off
   // unless FFTJH_ROWS_IO=1)
   static const bool rows_io = std::getenv("FFTJH_ROWS_IO") != nullptr;
-  if (rows_io && s.pow2 && d.C > (L.dtype == FFTJH_F32 ? 4 : 8)) {
+  if (rows_io && s.pow2 && (n & (n - 1)) == 0 && d.C > (L.dtype == FFTJH_F32 ? 4 : 8)) {
     // the C transforms of a group must sit at a constant offset step
     int64_t cin = -1, cout = -1;
     if (Bi > 1 && Bi % d.C == 0) {
@@ -980,7 +998,8 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
       build_c2c(trial, rp.m, batch, 1, geo(1, 0, rp.m), geo(1, 0, rp.m), dir, 1.0, g, g,
                 std::vector<int>(free.begin() + 1, free.end()));
       Step& f = trial.steps[first];
-      if (f.kind == ST_PASS && f.pow2 && !f.mat && f.src == g && !f.d.rows_in && !f.d.real_io &&
+      if (f.kind == ST_PASS && f.pow2 && (f.d.n & (f.d.n - 1)) == 0 && !f.mat && f.src == g && !f.d.rows_in &&
+          !f.d.real_io &&
           f.d.in.s_out == rp.m && f.d.batch == batch * (rp.m / f.d.n)) {
         f.src = src;
         f.d.rs_io = 1;
@@ -1131,7 +1150,7 @@ static bool pair_real_pass(Layout& L, Step& s, int64_t h, int64_t rows, int real
     const char* e = std::getenv("FFTJH_REAL_PAIRED");
     return e && std::atoi(e) == 0;
   }();
-  if (off || s.kind != ST_PASS || !s.pow2 || s.mat) return false;
+  if (off || s.kind != ST_PASS || !s.pow2 || s.mat || (s.d.n & (s.d.n - 1))) return false;
   PassDesc& d = s.d;
   if (h % d.n) return false;
   const int64_t M = h / d.n;
@@ -1635,7 +1654,7 @@ static std::string describe(const Layout& L) {
     if (s.kind == ST_PASS) {
       o << " n=" << s.d.n << " radices=";
       for (size_t i = 0; i < s.radices.size(); ++i) o << (i ? "," : "") << s.radices[i];
-      if (s.pow2) o << " kernel=pow2";
+      if (s.pow2) o << ((s.d.n & (s.d.n - 1)) == 0 ? " kernel=pow2" : " kernel=chain");
       if (s.d.fuse_io && s.radices.size() >= 2) o << " kernel=fused";
       if (s.mat) o << (s.mat == kMatF16 ? " kernel=mfma_f16" : s.mat == kMatBF16 ? " kernel=mfma_bf16" : " kernel=mfma_bf16x3");
       if (s.d.real_io) {

@@ -26,7 +26,6 @@ namespace fftjh {
 
 namespace {
 
-constexpr int kE = 16;
 
 #ifndef FFTJH_P2_MAXT
 #define FFTJH_P2_MAXT 512
@@ -51,9 +50,9 @@ struct Radices {
 
 // butterflies of one stage on the thread's registers: b < 16/R, rows
 // j = jt + b*J, inputs v[b + r*16/R]; twiddles w[b][r-1] already applied
-template <typename T, int DIR, int R>
-__device__ __forceinline__ void stage_regs(V2<T> (&v)[kE]) {
-  constexpr int BPT = kE / R;
+template <typename T, int DIR, int R, int E>
+__device__ __forceinline__ void stage_regs(V2<T> (&v)[E]) {
+  constexpr int BPT = E / R;
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     V2<T> a[R];
@@ -66,14 +65,14 @@ __device__ __forceinline__ void stage_regs(V2<T> (&v)[kE]) {
 }
 
 // twiddles of stage (R, NS) for this thread's butterflies, from the table
-template <typename T, int R, int NS>
-__device__ __forceinline__ void stage_twiddle(V2<T> (&v)[kE], const V2<T>* tw, int jt, int J) {
+template <typename T, int R, int NS, int E>
+__device__ __forceinline__ void stage_twiddle(V2<T> (&v)[E], const V2<T>* tw, int jt, int J) {
   if constexpr (NS > 1) {
-    constexpr int BPT = kE / R;
+    constexpr int BPT = E / R;
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
       const int j = jt + b * J;
-      const int k = j & (NS - 1);
+      const int k = j % NS;
       const V2<T>* twp = tw + k;
 #pragma unroll
       for (int r = 1; r < R; ++r) v[b + r * BPT] = cmul<T>(v[b + r * BPT], twp[(r - 1) * NS]);
@@ -83,13 +82,13 @@ __device__ __forceinline__ void stage_twiddle(V2<T> (&v)[kE], const V2<T>* tw, i
 
 // scatter a stage's outputs into LDS: output r of butterfly j at
 // (j - k)*R + k + r*NS, k = j mod NS
-template <typename T, int R, int NS>
-__device__ __forceinline__ void stage_scatter(const V2<T> (&v)[kE], V2<T>* lds, int C, int c, int jt, int J) {
-  constexpr int BPT = kE / R;
+template <typename T, int R, int NS, int E>
+__device__ __forceinline__ void stage_scatter(const V2<T> (&v)[E], V2<T>* lds, int C, int c, int jt, int J) {
+  constexpr int BPT = E / R;
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = jt + b * J;
-    const int k = j & (NS - 1);
+    const int k = j % NS;
     const int base = (j - k) * R + k;
 #pragma unroll
     for (int r = 0; r < R; ++r) lds[lpad2((base + r * NS) * C + c)] = v[b + r * BPT];
@@ -97,9 +96,9 @@ __device__ __forceinline__ void stage_scatter(const V2<T> (&v)[kE], V2<T>* lds, 
 }
 
 // gather a stage's inputs from LDS: input r of butterfly j at j + r*N/R
-template <typename T, int N, int R>
-__device__ __forceinline__ void stage_gather(V2<T> (&v)[kE], const V2<T>* lds, int C, int c, int jt, int J) {
-  constexpr int BPT = kE / R;
+template <typename T, int N, int R, int E>
+__device__ __forceinline__ void stage_gather(V2<T> (&v)[E], const V2<T>* lds, int C, int c, int jt, int J) {
+  constexpr int BPT = E / R;
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int j = jt + b * J;
@@ -108,19 +107,19 @@ __device__ __forceinline__ void stage_gather(V2<T> (&v)[kE], const V2<T>* lds, i
   }
 }
 
-template <typename T, int DIR, int N, typename Ch, int S>
-__device__ __forceinline__ void middle_stages(V2<T> (&v)[kE], V2<T>* lds, const V2<T>* tw, int C,
+template <typename T, int DIR, int N, typename Ch, int S, int E>
+__device__ __forceinline__ void middle_stages(V2<T> (&v)[E], V2<T>* lds, const V2<T>* tw, int C,
                                               int c, int jt) {
   if constexpr (S < Ch::count - 1) {
     constexpr int R = Ch::get(S), NS = Ch::ns(S), OFF = Ch::off(S);
-    constexpr int J = N / kE;
-    stage_gather<T, N, R>(v, lds, C, c, jt, J);
+    constexpr int J = N / E;
+    stage_gather<T, N, R, E>(v, lds, C, c, jt, J);
     __syncthreads();
-    stage_twiddle<T, R, NS>(v, tw + OFF, jt, J);
-    stage_regs<T, DIR, R>(v);
-    stage_scatter<T, R, NS>(v, lds, C, c, jt, J);
+    stage_twiddle<T, R, NS, E>(v, tw + OFF, jt, J);
+    stage_regs<T, DIR, R, E>(v);
+    stage_scatter<T, R, NS, E>(v, lds, C, c, jt, J);
     __syncthreads();
-    middle_stages<T, DIR, N, Ch, S + 1>(v, lds, tw, C, c, jt);
+    middle_stages<T, DIR, N, Ch, S + 1, E>(v, lds, tw, C, c, jt);
   }
 }
 
@@ -136,7 +135,7 @@ __device__ __forceinline__ void middle_stages(V2<T> (&v)[kE], V2<T>* lds, const 
 #ifndef FFTJH_P2_WPE
 #define FFTJH_P2_WPE 0
 #endif
-template <typename T, int DIR, int N, int R1, int R2, int R3, int R4, int IO>
+template <typename T, int DIR, int N, int R1, int R2, int R3, int R4, int IO, int kE = 16>
 __global__ void __launch_bounds__(MaxThreadsP2<T>::value)
 #if FFTJH_P2_WPE > 0
 __attribute__((amdgpu_waves_per_eu(FFTJH_P2_WPE)))
@@ -333,7 +332,7 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
   if constexpr (IO == 1) {
     load_rows(g);
     __syncthreads();
-    stage_gather<T, N, R1>(v, lds, C, c, jt, J);
+    stage_gather<T, N, R1, kE>(v, lds, C, c, jt, J);
     __syncthreads();
   } else if constexpr (IO == 4) {
     load_pre(g, v);
@@ -348,13 +347,13 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
   const bool scaled = d.scale != 1.0;
   while (true) {
     // stage 1 on the loaded registers (ns = 1: no twiddles)
-    stage_regs<T, DIR, R1>(v);
-    stage_scatter<T, R1, 1>(v, lds, C, c, jt, J);
+    stage_regs<T, DIR, R1, kE>(v);
+    stage_scatter<T, R1, 1, kE>(v, lds, C, c, jt, J);
     __syncthreads();
     const int gn = g + walk.step;
     const bool more = gn < walk.end;
     if (FFTJH_P2_PREFETCH && more) load(gn, pf);
-    middle_stages<T, DIR, N, Ch, 1>(v, lds, tw, C, c, jt);
+    middle_stages<T, DIR, N, Ch, 1, kE>(v, lds, tw, C, c, jt);
     // last stage: LDS -> registers -> HBM
     {
       constexpr int S = NST - 1;
@@ -364,13 +363,13 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
       // the multi-pass twiddle's base and step, formed while v is dead
       PostTwiddle pt;
       if ((IO <= 1 || IO >= 6) && d.tw_period) pt.init(d, ix, jt, J);
-      stage_gather<T, N, R>(v, lds, C, c, jt, J);
+      stage_gather<T, N, R, kE>(v, lds, C, c, jt, J);
       __syncthreads();
-      stage_twiddle<T, R, NS>(v, tw + OFF, jt, J);
-      stage_regs<T, DIR, R>(v);
+      stage_twiddle<T, R, NS, kE>(v, tw + OFF, jt, J);
+      stage_regs<T, DIR, R, kE>(v);
       if constexpr (IO == 2) {
         // last stage: k == j, so output r of butterfly j goes to row j + r*N/R
-        stage_scatter<T, R, N / R>(v, lds, C, c, jt, J);
+        stage_scatter<T, R, N / R, kE>(v, lds, C, c, jt, J);
         __syncthreads();
         store_rows(g);
         __syncthreads();
@@ -465,7 +464,7 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     } else if constexpr (IO == 1) {
       load_rows(gn);
       __syncthreads();
-      stage_gather<T, N, R1>(v, lds, C, c, jt, J);
+      stage_gather<T, N, R1, kE>(v, lds, C, c, jt, J);
       __syncthreads();
     } else if constexpr (IO == 4) {
       load_pre(gn, v);
@@ -482,11 +481,11 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
 
 namespace {
 
-template <typename T, int DIR, int N, int R1, int R2, int R3, int R4, int IO>
+template <typename T, int DIR, int N, int R1, int R2, int R3, int R4, int IO, int E = 16>
 hipError_t launch_p2_io(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
                         hipStream_t st) {
   static bool attr = false;
-  auto* fn = &k_pass_pow2<T, DIR, N, R1, R2, R3, R4, IO>;
+  auto* fn = &k_pass_pow2<T, DIR, N, R1, R2, R3, R4, IO, E>;
   if (!attr) {
     hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
     attr = true;
@@ -522,9 +521,9 @@ hipError_t launch_p2(const PassDesc& d, const PassDesc* dd, const void* x, void*
   return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 0>(d, dd, x, y, tw, st);
 }
 
-template <typename T, int DIR, int N, int R1, int R2, int R3, int R4, int IO>
+template <typename T, int DIR, int N, int R1, int R2, int R3, int R4, int IO, int E = 16>
 int resident_p2_io(const PassDesc& d) {
-  auto* fn = &k_pass_pow2<T, DIR, N, R1, R2, R3, R4, IO>;
+  auto* fn = &k_pass_pow2<T, DIR, N, R1, R2, R3, R4, IO, E>;
   hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
   const size_t lds = static_cast<size_t>(lds_elems(d.C * N)) * sizeof(V2<T>);
   int per_cu = 0, dev = 0, cus = 0;
@@ -557,6 +556,19 @@ int resident_p2(const PassDesc& d) {
   return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 0>(d);
 }
 
+// fixed chains of other lengths (register I/O only): E elements per thread,
+// every radix dividing E, N/E threads per transform
+template <typename T, int DIR, int N, int R1, int R2, int R3, int R4, int E>
+hipError_t launch_fixed(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
+                        hipStream_t st) {
+  if (d.real_io || d.rs_io || d.rows_in || d.rows_out) return hipErrorInvalidValue;
+  return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 0, E>(d, dd, x, y, tw, st);
+}
+template <typename T, int DIR, int N, int R1, int R2, int R3, int R4, int E>
+int resident_fixed(const PassDesc& d) {
+  return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 0, E>(d);
+}
+
 // the compile-time chains: n -> radices (16s first, the remainder last)
 #define FFTJH_POW2_CHAINS(X)  \
   X(32, 16, 2, 1, 1)          \
@@ -569,6 +581,10 @@ int resident_p2(const PassDesc& d) {
   X(4096, 16, 16, 16, 1)      \
   X(8192, 16, 16, 16, 2)      \
   X(16384, 16, 16, 16, 4)
+// n -> radices, elements per thread (non-power-of-two lengths of the
+// multi-pass plans: 10^6 = 100^3)
+#define FFTJH_FIXED_CHAINS(X) \
+  X(100, 10, 10, 1, 1, 10)
 
 template <typename T, int DIR>
 hipError_t dispatch_launch(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
@@ -578,6 +594,11 @@ hipError_t dispatch_launch(const PassDesc& d, const PassDesc* dd, const void* x,
   case N:                  \
     return launch_p2<T, DIR, N, A, B, C_, D_>(d, dd, x, y, tw, st);
     FFTJH_POW2_CHAINS(X)
+#undef X
+#define X(N, A, B, C_, D_, E) \
+  case N:                     \
+    return launch_fixed<T, DIR, N, A, B, C_, D_, E>(d, dd, x, y, tw, st);
+    FFTJH_FIXED_CHAINS(X)
 #undef X
   }
   return hipErrorInvalidValue;
@@ -590,6 +611,11 @@ int dispatch_resident(const PassDesc& d) {
   case N:                  \
     return resident_p2<T, DIR, N, A, B, C_, D_>(d);
     FFTJH_POW2_CHAINS(X)
+#undef X
+#define X(N, A, B, C_, D_, E) \
+  case N:                     \
+    return resident_fixed<T, DIR, N, A, B, C_, D_, E>(d);
+    FFTJH_FIXED_CHAINS(X)
 #undef X
   }
   return 0;

@@ -35,6 +35,28 @@ bool pow2_chain(int n, int* radices, int* count) {
   return false;
 }
 
+bool fixed_chain(int n, int* radices, int* count, int* elems) {
+  if (pow2_chain(n, radices, count)) {
+    *elems = 16;
+    return true;
+  }
+  switch (n) {
+#define X(N, A, B, C_, D_, E)               \
+  case N: {                                 \
+    const int r[4] = {A, B, C_, D_};        \
+    int k = 0;                              \
+    for (int i = 0; i < 4; ++i)             \
+      if (r[i] > 1) radices[k++] = r[i];    \
+    *count = k;                             \
+    *elems = E;                             \
+    return true;                            \
+  }
+    FFTJH_FIXED_CHAINS(X)
+#undef X
+  }
+  return false;
+}
+
 hipError_t launch_pass_pow2(int dtype, int dir, const PassDesc& d, const PassDesc* dd, const void* x, void* y,
                             const void* tw, hipStream_t st) {
   if (dtype == 0)

@@ -38,7 +38,7 @@ def test_pow2(prec, logn):
 MIXED = [3, 5, 6, 7, 9, 10, 12, 14, 15, 20, 21, 24, 25, 27, 35, 45, 48, 49, 60, 63, 105, 120, 315, 27 * 32,
          3 * 2**8, 9 * 2**8, 5 * 2**8, 7 * 2**8, 15 * 2**7, 105 * 2**4, 5**3 * 2**6, 7**2 * 2**6,
          3**4 * 5 * 7 * 2**5, 11, 13, 22, 26, 429, 352, 416, 1001, 11**2 * 2**4, 13**2 * 2**4,
-         2**4 * 3**2 * 11 * 13, 3 * 5 * 7 * 11 * 13 * 2**4]
+         2**4 * 3**2 * 11 * 13, 3 * 5 * 7 * 11 * 13 * 2**4, 100]
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
@@ -48,7 +48,7 @@ def test_mixed_radix(prec, n):
 
 
 FOURSTEP = [2**17, 2**18, 2**20, 3 * 2**16, 5 * 2**16, 7 * 2**16, 9 * 2**15, 15 * 2**15,
-            3**8 * 4, 5**6 * 2, 7**5 * 3, 3**4 * 11 * 13 * 8, 13**4, 2 * 3**9, 3**10 * 2]
+            3**8 * 4, 5**6 * 2, 7**5 * 3, 3**4 * 11 * 13 * 8, 13**4, 2 * 3**9, 3**10 * 2, 10**5]
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
@@ -58,7 +58,8 @@ def test_fourstep_sizes(prec, n):
 
 
 @pytest.mark.parametrize("prec,n,batch", [("f32", 4096, 64), ("f64", 1024, 33), ("f32", 60, 1000),
-                                          ("f32", 1000, 8), ("f32", 2**17, 3), ("f64", 3 * 2**16, 2)])
+                                          ("f32", 1000, 8), ("f32", 2**17, 3), ("f64", 3 * 2**16, 2),
+                                          ("f32", 100, 37), ("f32", 10**4, 5)])
 def test_batched(prec, n, batch):
     _check_c2c(prec, n, batch=batch, seed=batch)
 
