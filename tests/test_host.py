@@ -103,6 +103,16 @@ def test_schedule_config3():
     assert len(s) == 3 and "in_stride=1 " in s[0] and "in_stride=256 " in s[1] and "in_stride=65536 " in s[2]
 
 
+def test_fixed_chain_schedule():
+    # 10^6 = 100^3: FP32 passes of 100 = 10 x 10 on the compile-time chain
+    # kernel (10 elements per thread, 32 transforms per workgroup); FP64's
+    # 320 threads would exceed its limit, so it keeps the runtime chain
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [10**6], batch=8))
+    assert len(s) == 3 and all("n=100 radices=10,10 kernel=chain" in p and "C=32 threads=320" in p for p in s)
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F64, [10**6], batch=2))
+    assert len(s) == 3 and not any("kernel=chain" in p for p in s)
+
+
 def test_nd_long_and_nonsmooth_axes():
     # a long strided axis runs the multi-pass schedule on its column block
     # directly (offsets scaled by the column count), no gather / scatter
