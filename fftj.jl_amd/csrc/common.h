@@ -88,14 +88,14 @@ struct PassDesc {
   // real transform of n = 2*N reals (N the pass length): 1 = R2C (twist of
   // the half-length spectrum into N+1 bins at the store), 2 = C2R (pre-pass
   // of N+1 bins into the half-length input at the load); rtw holds
-  // w_k = exp(dir*2*pi*i*k/n), k = 0..N, in the plan's precision
+  // w_k = exp(dir*2*pi*i*k/n) as an FP64 two-level table:
+  // w = rtw[e >> rtw_sh] * rtw[rtw_hi + (e & (2^rtw_sh - 1))]
   int32_t real_io;
   const void* rtw;
   // real_io 3 / 4: the last (R2C) or first (C2R) pass of a multi-pass
-  // half-length transform of length real_h, with pair_M transforms per row;
-  // a group holds 16 transforms T and their 16 mirrors pair_M - T.  rtw is
-  // then an FP64 two-level table of w_{2h}^e: w = rtw[e >> rtw_sh] *
-  // rtw[rtw_hi + (e & (2^rtw_sh - 1))]
+  // half-length transform of length real_h (n = 2*real_h), with pair_M
+  // transforms per row; a group holds 16 transforms T and their 16 mirrors
+  // pair_M - T
   int32_t pair_M;
   int64_t real_h;
   int32_t rtw_sh;

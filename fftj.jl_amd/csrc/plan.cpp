@@ -568,6 +568,11 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   if (real_io) {
     // row r of a group at base(g*C) + r*row stride on both sides
     d.real_io = real_io;
+    d.real_h = n;
+    int bits = 0;
+    while ((int64_t(1) << bits) < 2 * static_cast<int64_t>(n)) ++bits;
+    d.rtw_sh = (bits + 1) / 2;
+    d.rtw_hi = ((2 * static_cast<int64_t>(n) - 1) >> d.rtw_sh) + 1;
     d.rows_in = d.rows_out = 0;
     d.in_cdelta = in.s_out;
     d.out_cdelta = out.s_out;
@@ -1791,8 +1796,8 @@ static void materialize(fftjh_plan_s& P) {
         s.aux = rs_corr(s);
         s.d.rs_corr = s.aux->p;
       }
-      if (s.d.real_io >= 3) {
-        // paired edges: w_{2h}^e = exp(dir*2*pi*i*e/(2h)) from an FP64
+      if (s.d.real_io) {
+        // w_{2h}^e = exp(dir*2*pi*i*e/(2h)) (real.jl:26-39) from an FP64
         // two-level table (w^(i*2^sh) for i < rtw_hi, then w^i for i < 2^sh)
         const int64_t P = 2 * s.d.real_h, hi = s.d.rtw_hi, lo = int64_t(1) << s.d.rtw_sh;
         std::ostringstream rk;
@@ -1801,12 +1806,6 @@ static void materialize(fftjh_plan_s& P) {
           hip_check(launch_tw_pow(FFTJH_F64, p, hi, lo, P, dir, 0), "twiddle kernel");
           hip_check(launch_tw_pow(FFTJH_F64, static_cast<char*>(p) + hi * 16, lo, 1, P, dir, 0), "twiddle kernel");
         });
-        s.d.rtw = s.aux->p;
-      } else if (s.d.real_io) {
-        // w_k = exp(dir*2*pi*i*k/n), k = 0..h, n = 2h (real.jl:26-39)
-        const int64_t h = s.d.n, n = 2 * h;
-        s.aux = cached(key_of("real", {dev, double(dt), double(dir), double(n)}), (h + 1) * es,
-                       [&](void* p) { hip_check(launch_tw_pow(dt, p, h + 1, 1, n, dir, 0), "twiddle kernel"); });
         s.d.rtw = s.aux->p;
       }
       s.d.grid = resident > 0 ? std::min(resident, s.d.ngroups) : s.d.ngroups;

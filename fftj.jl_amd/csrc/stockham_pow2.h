@@ -178,6 +178,13 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     const int a = gg % (pM >> 5);
     return (a == 0 && (c & 15) == 0) ? c : (c ^ 16);
   };
+  // the real edges' twiddle w_{2h}^e (h the half length: N for the row
+  // kernels, real_h for the paired ones) from the FP64 two-level table
+  auto rtw_at = [&](uint64_t e) -> double2 {
+    const double2* t = static_cast<const double2*>(d.rtw);
+    const uint64_t mask = (uint64_t(1) << d.rtw_sh) - 1;
+    return cmul64(t[e >> d.rtw_sh], t[d.rtw_hi + static_cast<int64_t>(e & mask)]);
+  };
   auto load = [&](int g, V2<T> (&dst)[kE]) {
     const int64_t b = bidx(g, c);
     if (b < d.batch) {
@@ -237,7 +244,9 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     const int64_t b = static_cast<int64_t>(g) * C + c;
     if (b < d.batch) {
       const V2<T>* xr = x + BatchIndex(d, b).base(d.in);
-      const V2<T>* w = static_cast<const V2<T>*>(d.rtw);
+      // w_j, j = jt + u*J: FP64 start and step
+      double2 wv = rtw_at(static_cast<uint64_t>(jt));
+      const double2 ws = rtw_at(static_cast<uint64_t>(J));
 #pragma unroll
       for (int u = 0; u < kE; ++u) {
         const int j = jt + u * J;
@@ -245,7 +254,8 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
         const V2<T> xc = xr[N - j];
         const V2<T> cx = mk<T>(xc.x, -xc.y);
         const V2<T> xe = cadd<T>(a, cx);
-        const V2<T> t = cmul<T>(csub<T>(a, cx), w[j]);
+        const V2<T> t = cmul<T>(csub<T>(a, cx), mk<T>(static_cast<T>(wv.x), static_cast<T>(wv.y)));
+        wv = cmul64(wv, ws);
         dst[u] = mk<T>(xe.x - t.y, xe.y + t.x);
       }
     } else {
@@ -254,12 +264,6 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     }
   };
 
-  // the paired real edges' twiddle w_{2h}^e from the FP64 two-level table
-  auto rtw_at = [&](uint64_t e) -> double2 {
-    const double2* t = static_cast<const double2*>(d.rtw);
-    const uint64_t mask = (uint64_t(1) << d.rtw_sh) - 1;
-    return cmul64(t[e >> d.rtw_sh], t[d.rtw_hi + static_cast<int64_t>(e & mask)]);
-  };
   // C2R edge of a multi-pass first pass (IO 6): column P = T of the
   // half-length input j = P + M*j3 (j3 = jt + u*J here); the partner bin
   // X_{h-j} is element N-1-j3 of column M-P (element (N-j3) mod N of column
@@ -383,11 +387,14 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
         __syncthreads();
         if (b < d.batch) {
           V2<T>* yr = y + ix.base(d.out);
-          const V2<T>* w = static_cast<const V2<T>*>(d.rtw);
+          double2 wv = rtw_at(static_cast<uint64_t>(jt));
+          const double2 ws = rtw_at(static_cast<uint64_t>(J));
 #pragma unroll
           for (int u = 0; u < kE; ++u) {
             const int k = jt + u * J;
             const V2<T> z = v[u];
+            const V2<T> wk = mk<T>(static_cast<T>(wv.x), static_cast<T>(wv.y));
+            wv = cmul64(wv, ws);
             if (u == 0 && jt == 0) {
               yr[0] = mk<T>(z.x + z.y, T(0));
               yr[N] = mk<T>(z.x - z.y, T(0));
@@ -397,7 +404,7 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
               const T half = T(0.5);
               const V2<T> xe = cscale<T>(cadd<T>(z, cz), half);
               const V2<T> xo = cscale<T>(csub<T>(z, cz), half);
-              yr[k] = cadd<T>(xe, cmul<T>(mk<T>(xo.y, -xo.x), w[k]));
+              yr[k] = cadd<T>(xe, cmul<T>(mk<T>(xo.y, -xo.x), wk));
             }
           }
         }
