@@ -6,9 +6,11 @@ A step is one forward and one inverse transform of the whole batch
 (synthetic inputs uniform in [-1, 1), seed 0, already resident in HBM).
 Prints one JSON line: value = GFLOP/s by the 5 N log2 N convention (both
 directions counted), the roofline of the dominant kernel (algorithmic bytes
-per launch over its HIP-event-timed mean duration on the plan's stream),
-and the CPU baseline (the oracle's FP32 port of FFTJ's threaded four-step,
-timed on this host on a bounded sample).
+per launch over its mean launch duration in the HIP-event-timed region: every
+launch of a step is one pass of the same kernel over the whole array, and the
+launches run back to back on the plan's stream), and the CPU baseline (the
+oracle's FP32 port of FFTJ's threaded four-step, timed on this host on a
+bounded sample).
 """
 import argparse
 import json
@@ -39,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch leg")
+    ap.add_argument("--launch-breakdown", action="store_true",
+                    help="also time each launch of each plan run alone (HIP events between launches)")
     ap.add_argument("--no-extras", action="store_true", help="skip the split-precision and learned-planner legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="measure the device's streaming copy rate at the 2^24 footprint and exit")
@@ -443,14 +447,26 @@ def main():
     # oracle by the tests; here only a cheap property)
     rt_err = (torch.linalg.vector_norm(z - x) / torch.linalg.vector_norm(x)).item()
 
-    # dominant kernel: per-launch mean duration from HIP events on the plan's stream
-    launches_f = pf.profile_launches(y, x, iters=20)
-    launches_i = pi.profile_launches(z, y, iters=20)
-    launches = launches_f + launches_i
-    k = max(range(len(launches)), key=lambda i: launches[i])
-    t_launch = launches[k] * 1e-3
+    # dominant kernel: every launch of a step is one pass of the pass kernel
+    # (one template, instantiated per direction) over the whole array
+    # (16*N*batch algorithmic bytes), issued back to back on the plan's stream
+    # (a kernel trace of this command shows no gaps between them), so the
+    # timed region's HIP events give its mean launch duration as
+    # ms_per_step / launches per step (events between launches would add
+    # gaps to what they time)
+    import re as _re
+    desc_f, desc_i = pf.describe(), pi.describe()
+    nl_f = int(_re.search(r"launches=(\d+)", desc_f).group(1))
+    nl_i = int(_re.search(r"launches=(\d+)", desc_i).group(1))
+    kernel = _re.search(r"n=(\d+) radices=\S+ kernel=(\w+)", desc_f)
+    t_launch = ms_step * 1e-3 / (nl_f + nl_i)
     bytes_per_launch = 16.0 * n * batch  # one read + one write of the complex FP32 array
     achieved = bytes_per_launch / t_launch / 1e9
+    # per-launch durations of each plan run alone (events between launches:
+    # each includes the gap before it), on request
+    launches = None
+    if args.launch_breakdown:
+        launches = pf.profile_launches(y, x, iters=20) + pi.profile_launches(z, y, iters=20)
     # HBM bytes per pass launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
     traffic = None
     tp = os.path.join(ROOT, "profiles", f"pmc_traffic_{args.workload}.md")
@@ -475,16 +491,18 @@ def main():
         "dtype": "f32",
         "data": "synthetic: uniform [-1,1) complex, splitmix64 seed 0, resident in HBM",
         "config": {"workload": args.workload, "description": desc, "n": n, "batch": batch,
-                   "launches_per_direction": len(launches_f), "parallelism": "single GPU"},
+                   "launches_per_direction": nl_f, "parallelism": "single GPU"},
         # whole transform: 16*N*batch algorithmic bytes per direction, two directions per step
         "hbm_frac_transform": round(2 * bytes_per_launch / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roundtrip_rel_err": rt_err,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"launch {k} ({'forward' if k < len(launches_f) else 'inverse'} pass "
-                               f"{k % len(launches_f)})",
-                     "launch_ms": [round(v, 5) for v in launches]},
+                     "kernel": (f"{kernel.group(2)} pass kernel, {kernel.group(1)}-point transforms, both "
+                                f"directions: the {nl_f + nl_i} launches of a step" if kernel else "pass kernel"),
+                     "launch_us_mean": round(t_launch * 1e6, 2)},
     }
+    if launches is not None:
+        result["roofline"]["launch_ms_isolated"] = [round(v, 5) for v in launches]
     if not args.no_ragged:
         result["ragged"] = ragged_leg(F)
     if not args.no_extras:
