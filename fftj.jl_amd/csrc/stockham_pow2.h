@@ -164,7 +164,7 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
   // other self-mirrored transform.
   const int pM = d.pair_M;
   auto bidx = [&](int gg, int cc) -> int64_t {
-    if constexpr (IO >= 5) {
+    if constexpr (IO == 5 || IO == 6) {
       const int G1 = pM >> 5;
       const int bo = gg / G1, a = gg - bo * G1, i = cc & 15;
       const int Tm = cc < 16 ? 16 * a + i : ((a == 0 && i == 0) ? (pM >> 1) : pM - 16 * a - i);
