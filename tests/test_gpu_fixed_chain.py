@@ -1,0 +1,39 @@
+"""The compile-time chain kernel on a non-power-of-two length (100 = 10 x 10,
+10 elements per thread, 32 transforms per workgroup): rows with partial last
+groups, strided N-d axes, and the multi-pass lengths built from it, against
+the oracle.  Each case asserts that the chain kernel runs."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from conftest import F, inputs_c, rel_l2, to_dev, to_host, tol_for
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,batch", [(100, 1), (100, 7), (100, 33), (100, 1000), (10**6, 1), (10**6, 3)])
+def test_chain_rows_and_multipass(n, batch):
+    import torch
+    p = F.plan_fft(torch.float32, n, batch=batch)
+    assert "kernel=chain" in p.describe()
+    x = inputs_c(n + batch, n * batch, "f32").reshape(batch, n)
+    y = to_host(p(to_dev(x))).reshape(batch, n)
+    ref = O.fft(x.astype(np.complex128))
+    for b in range(batch):
+        assert rel_l2(y[b], ref[b]) <= tol_for("f32", n), b
+    back = to_host(F.plan_ifft(torch.float32, n, batch=batch)(to_dev(y))).reshape(batch, n)
+    assert rel_l2(back, x) <= 2 * tol_for("f32", n)
+
+
+@pytest.mark.parametrize("dims", [(100, 100), (40, 100), (100, 3, 7), (7, 9, 100), (100, 100, 100)])
+def test_chain_nd_axes(dims):
+    n = int(np.prod(dims))
+    x = inputs_c(n + 3, n, "f32").reshape(tuple(reversed(dims)))
+    xd = to_dev(x)
+    p = F.plan_fft(xd)
+    assert "kernel=chain" in p.describe()
+    y = to_host(p * xd)
+    ref = O.ndfft(x.astype(np.complex128).ravel(), list(dims)).reshape(x.shape)
+    assert rel_l2(y, ref) <= 200 * np.finfo(np.float32).eps   # runtests.jl:332 tier
+    back = to_host(F.plan_ifft(xd) * to_dev(y))
+    assert rel_l2(back, x) <= 400 * np.finfo(np.float32).eps
