@@ -342,6 +342,7 @@ struct Layout {
   int npass = 0;        // pass launches laid out so far (index into tune.pass)
   int mat = kMatNone;   // split-precision MFMA mode of every pass (FP32 C2C)
   unsigned route = 0;   // FFTJH_FLAG_ROUTE_* for non-smooth lengths (0 = planner)
+  bool two_pass512 = true;  // the smaller 1-D lengths may take two passes of up to 512 points
   int dtype = 0, kind = FFTJH_KIND_C2C, sign = -1, rank = 1;
   int64_t dims[8] = {0};
   int64_t batch = 1;
@@ -890,7 +891,19 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
       }
       if (prod != n) fail(FFTJH_ERR_ARGUMENT, "tuning factors do not multiply to " + std::to_string(n));
     } else {
-      F = choose_factors(L.dtype, n, default_max_factor(L.dtype));
+      // two passes of up to 512 points beat three of up to 256 for the
+      // smaller 1-D lengths (measured: 2^17 x 64 F32 0.094 -> 0.081 ms, 10^5 x 64
+      // 0.096 -> 0.060 ms, F64 2^17 x 64 0.165 -> 0.130 ms); from 2^17 on (F32)
+      // the 512-point passes lose (196,608 x 32 0.064 -> 0.066 ms, 2^18 x 32
+      // 0.082 -> 0.094 ms).  FFTJH_MAX_FACTOR overrides both.
+      static const bool mf_env = std::getenv("FFTJH_MAX_FACTOR") != nullptr;
+      const int64_t two_pass_max = L.dtype == FFTJH_F32 ? (int64_t(1) << 17) : (int64_t(1) << 18);
+      if (!mf_env && L.two_pass512 && !colblock && n <= two_pass_max) {
+        std::vector<int64_t> cur;
+        double score = 1e300;
+        factor_search(L.dtype, n, 2, 512, cur, F, score);
+      }
+      if (F.empty()) F = choose_factors(L.dtype, n, default_max_factor(L.dtype));
     }
     if (F.size() < 2) fail(FFTJH_ERR_UNSUPPORTED, "no multi-pass split of length " + std::to_string(n));
     if (free.empty()) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for a multi-pass plan");
@@ -997,8 +1010,12 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
       const char* e = std::getenv("FFTJH_RS_FUSE");
       return !e || std::atoi(e) != 0;
     }();
-    if (rs_fuse && is_rows(gin, n, Bi) && gin.stride == 1 && rp.m < (int64_t(1) << 31)) {
+    // (the default split first, then passes of at most 256 points)
+    for (int attempt = 0; attempt < 2 && rs_fuse && is_rows(gin, n, Bi) && gin.stride == 1 &&
+                          rp.m < (int64_t(1) << 31);
+         ++attempt) {
       Layout trial = L;
+      trial.two_pass512 = attempt == 0;
       const size_t first = trial.steps.size();
       build_c2c(trial, rp.m, batch, 1, geo(1, 0, rp.m), geo(1, 0, rp.m), dir, 1.0, g, g,
                 std::vector<int>(free.begin() + 1, free.end()));
@@ -1006,6 +1023,7 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
       if (f.kind == ST_PASS && f.pow2 && (f.d.n & (f.d.n - 1)) == 0 && !f.mat && f.src == g && !f.d.rows_in &&
           !f.d.real_io &&
           f.d.in.s_out == rp.m && f.d.batch == batch * (rp.m / f.d.n)) {
+        trial.two_pass512 = L.two_pass512;
         f.src = src;
         f.d.rs_io = 1;
         f.d.rs_n = n;
@@ -1197,11 +1215,15 @@ static void add_r2c_rows(Layout& L, int64_t n, int64_t batch, int src, int dst, 
     add_pass(L, -1, static_cast<int>(h), batch, 1, geo(1, 0, h), geo(1, 0, h + 1), 1.0, src, dst, PassTw(), 1, 1);
   } else if (n % 2 == 0) {
     const int64_t h = n / 2;
-    {
-      // the twist at the store of the half-length transform's last pass
+    // the twist at the store of the half-length transform's last pass (with
+    // the default split, else with passes of at most 256 points, which the
+    // paired kernel takes in either precision)
+    for (int attempt = 0; attempt < 2; ++attempt) {
       Layout trial = L;
+      trial.two_pass512 = attempt == 0;
       build_c2c_rows(trial, h, batch, -1, 1.0, src, dst, free);
       if (pair_real_pass(trial, trial.steps.back(), h, batch, 3, 1.0)) {
+        trial.two_pass512 = L.two_pass512;
         L = std::move(trial);
         return;
       }
@@ -1256,14 +1278,17 @@ static void add_c2r_rows(Layout& L, int64_t n, int64_t batch, bool normalized, i
              normalized ? 1.0 / static_cast<double>(n) : 1.0, src, dst, PassTw(), 1, 2);
   } else if (n % 2 == 0) {
     const int64_t h = n / 2;
-    {
-      // the pre-pass at the load of the half-length transform's first pass
-      // (its store carries the pre-factor, the last pass the 1/h)
+    // the pre-pass at the load of the half-length transform's first pass
+    // (its store carries the pre-factor, the last pass the 1/h); default
+    // split first, then passes of at most 256 points
+    for (int attempt = 0; attempt < 2; ++attempt) {
       Layout trial = L;
+      trial.two_pass512 = attempt == 0;
       const size_t first = trial.steps.size();
       build_c2c_rows(trial, h, batch, 1, normalized ? 1.0 / static_cast<double>(h) : 1.0, src, dst, free);
       if (trial.steps.size() > first + 1 &&
           pair_real_pass(trial, trial.steps[first], h, batch, 4, normalized ? 0.5 : 1.0)) {
+        trial.two_pass512 = L.two_pass512;
         L = std::move(trial);
         return;
       }
