@@ -110,6 +110,10 @@ struct PassDesc {
   // generic (non-power-of-two) pass: first stage fused into the HBM load,
   // last into the store (chains of two or more stages)
   int32_t fuse_io;
+  // streaming hints for data the plan does not touch again: the pass's HBM
+  // loads (its input is read once) / stores (its output is the plan's result)
+  int32_t nt_load;
+  int32_t nt_store;
 };
 
 // LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)

@@ -38,6 +38,20 @@ template <> struct MaxThreadsP2<double> { static constexpr int value = FFTJH_P2_
 
 __device__ __forceinline__ int lpad2(int f) { return f + (f >> 4); }
 
+// non-temporal (streaming) element load / store
+template <typename T> __device__ __forceinline__ V2<T> ld_nt(const V2<T>* p) {
+  using NV = T __attribute__((ext_vector_type(2)));
+  const NV v = __builtin_nontemporal_load(reinterpret_cast<const NV*>(p));
+  return mk<T>(v.x, v.y);
+}
+template <typename T> __device__ __forceinline__ void st_nt(V2<T>* p, const V2<T>& v) {
+  using NV = T __attribute__((ext_vector_type(2)));
+  NV w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<NV*>(p));
+}
+
 // the chain's radices, and per stage its ns and twiddle-table offset
 // (entries (r-1)*ns + k per stage)
 template <int R1, int R2, int R3, int R4>
@@ -131,7 +145,8 @@ __device__ __forceinline__ void middle_stages(V2<T> (&v)[E], V2<T>* lds, const V
 // real twist at the store), 4 = C2R rows (the real pre-pass at the load),
 // 5 / 6 = the same edges on the last / first pass of a multi-pass
 // half-length transform, partners exchanged between paired columns in LDS;
-// 7 = the resampling deconvolution at the load of a first pass
+// 7 = the resampling deconvolution at the load of a first pass; 8 / 9 / 10
+// = register I/O with non-temporal loads / stores / both
 #ifndef FFTJH_P2_WPE
 #define FFTJH_P2_WPE 0
 #endif
@@ -143,6 +158,8 @@ __attribute__((amdgpu_waves_per_eu(FFTJH_P2_WPE)))
 k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<T>* tw) {
   using Ch = Radices<R1, R2, R3, R4>;
   constexpr int NST = Ch::count;
+  // IO 8 / 9 / 10: register I/O (as 0) with streamed loads / stores / both
+  constexpr bool kNtLoad = IO == 8 || IO == 10, kNtStore = IO == 9 || IO == 10;
   constexpr int J = N / kE;
   static_assert(NST >= 2, "fused first and last stages need two stages");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -189,10 +206,18 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
     const int64_t b = bidx(g, c);
     if (b < d.batch) {
       const V2<T>* p = x + (BatchIndex(d, b).base(d.in) + jt * ist);
+      if constexpr (kNtLoad) {
 #pragma unroll
-      for (int u = 0; u < kE; ++u) {
-        dst[u] = *p;
-        p += istep;
+        for (int u = 0; u < kE; ++u) {
+          dst[u] = ld_nt<T>(p);
+          p += istep;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < kE; ++u) {
+          dst[u] = *p;
+          p += istep;
+        }
       }
     } else {
 #pragma unroll
@@ -454,6 +479,12 @@ k_pass_pow2(const PassDesc* __restrict__ dp, const V2<T>* x, V2<T>* y, const V2<
             yb += ostep;
             pt.advance();
           }
+        } else if constexpr (kNtStore) {
+#pragma unroll
+          for (int u = 0; u < kE; ++u) {
+            st_nt<T>(yb, scaled ? cscale<T>(v[u], scale) : v[u]);
+            yb += ostep;
+          }
         } else {
 #pragma unroll
           for (int u = 0; u < kE; ++u) {
@@ -525,6 +556,9 @@ hipError_t launch_p2(const PassDesc& d, const PassDesc* dd, const void* x, void*
   if (d.rs_io) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 7>(d, dd, x, y, tw, st);
   if (d.rows_in) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 1>(d, dd, x, y, tw, st);
   if (d.rows_out) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 2>(d, dd, x, y, tw, st);
+  if (d.nt_load && d.nt_store) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 10>(d, dd, x, y, tw, st);
+  if (d.nt_load) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 8>(d, dd, x, y, tw, st);
+  if (d.nt_store) return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 9>(d, dd, x, y, tw, st);
   return launch_p2_io<T, DIR, N, R1, R2, R3, R4, 0>(d, dd, x, y, tw, st);
 }
 
@@ -560,6 +594,9 @@ int resident_p2(const PassDesc& d) {
   if (d.rs_io) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 7>(d);
   if (d.rows_in) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 1>(d);
   if (d.rows_out) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 2>(d);
+  if (d.nt_load && d.nt_store) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 10>(d);
+  if (d.nt_load) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 8>(d);
+  if (d.nt_store) return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 9>(d);
   return resident_p2_io<T, DIR, N, R1, R2, R3, R4, 0>(d);
 }
 
