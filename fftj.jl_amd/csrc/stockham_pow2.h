@@ -1,11 +1,13 @@
 #pragma once
-// Implementation of the pass kernel for power-of-two lengths 256..16384 with a compile-time radix
-// chain (radices from {2, 4, 8, 16}).
+// Implementation of the pass kernel for lengths with a compile-time radix
+// chain: the powers of two 32..16384 (radices from {2, 4, 8, 16}) and the
+// listed fixed chains of other lengths (100 = 10 x 10).
 //
-// Each thread owns E = 16 elements of a transform, rows jt + u*J (u < 16,
-// J = n/16).  With the autosort indexing a radix-R stage's butterflies
-// j = jt + b*J (b < 16/R) read rows j + r*n/R = jt + J*(b + r*16/R) -- the
-// same 16 rows -- and the last stage (k == j) writes those rows too.  So:
+// Each thread owns E elements of a transform (16 for the powers of two, 10
+// for 100), rows jt + u*J (u < E, J = n/E); every radix R of the chain
+// divides E.  With the autosort indexing a radix-R stage's butterflies
+// j = jt + b*J (b < E/R) read rows j + r*n/R = jt + J*(b + r*E/R) -- the
+// same E rows -- and the last stage (k == j) writes those rows too.  So:
 //   stage 1   runs on the registers the HBM loads filled, writes LDS;
 //   stages    LDS -> registers -> LDS (two barriers each);
 //   last      LDS -> registers -> HBM, with scale and four-step twiddle.
