@@ -103,6 +103,18 @@ def test_schedule_config3():
     assert len(s) == 3 and "in_stride=1 " in s[0] and "in_stride=256 " in s[1] and "in_stride=65536 " in s[2]
 
 
+def test_stream_hints_rule():
+    # footprints of 384 MiB and up: the pass reading x streams its loads, the
+    # pass writing y last streams its stores; smaller plans keep the caches
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1 << 24]))
+    assert s[0].endswith("stream=in") and "stream=" not in s[1] and s[2].endswith("stream=out")
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [4096], batch=16384))
+    assert len(s) == 1 and s[0].endswith("stream=inout")
+    for kind, dims, batch in ((_lib.KIND_C2C, [256, 256, 256], 1), (_lib.KIND_C2C, [10**6], 8),
+                              (_lib.KIND_R2C, [1 << 22], 4), (_lib.KIND_C2C, [1000003], 4)):
+        assert "stream=" not in _lib.describe_request(kind, _lib.F32, dims, batch)
+
+
 def test_fixed_chain_schedule():
     # 10^6 = 100^3: FP32 passes of 100 = 10 x 10 on the compile-time chain
     # kernel (10 elements per thread, 32 transforms per workgroup); FP64's
