@@ -441,7 +441,11 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
       return !e || std::atoi(e) != 0;
     }();
     const bool p2 = pow2_chain(n, pr, &pc);
-    if (!no_pow2 && !L.mat && !(pt && pt->nradix > 0) && (p2 || (fixed_env && fixed_chain(n, pr, &pc, &pe)))) {
+    // 81 = 9 x 9 (27 elements per thread) wins on strided loads (F32
+    // 531,441 x 64 column passes 183 -> 130 us) and loses on contiguous rows
+    // (81 x 200,000: 93 -> 104 us), so it takes only passes with strided loads
+    const bool fixed = fixed_env && fixed_chain(n, pr, &pc, &pe) && (n != 81 || in.stride > 1);
+    if (!no_pow2 && !L.mat && !(pt && pt->nradix > 0) && (p2 || fixed)) {
       s.pow2 = true;
       s.radices.assign(pr, pr + pc);
     }

@@ -630,7 +630,8 @@ int resident_fixed(const PassDesc& d) {
 // n -> radices, elements per thread (non-power-of-two lengths of the
 // multi-pass plans: 10^6 = 100^3)
 #define FFTJH_FIXED_CHAINS(X) \
-  X(100, 10, 10, 1, 1, 10)
+  X(100, 10, 10, 1, 1, 10)    \
+  X(81, 9, 9, 1, 1, 27)
 
 template <typename T, int DIR>
 hipError_t dispatch_launch(const PassDesc& d, const PassDesc* dd, const void* x, void* y, const void* tw,
