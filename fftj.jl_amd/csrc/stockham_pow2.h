@@ -628,7 +628,8 @@ int resident_fixed(const PassDesc& d) {
   X(8192, 16, 16, 16, 2)      \
   X(16384, 16, 16, 16, 4)
 // n -> radices, elements per thread (non-power-of-two lengths of the
-// multi-pass plans: 10^6 = 100^3)
+// multi-pass plans: 10^6 = 100^3; 81 for the passes of 3^k lengths whose
+// load is strided)
 #define FFTJH_FIXED_CHAINS(X) \
   X(100, 10, 10, 1, 1, 10)    \
   X(81, 9, 9, 1, 1, 27)
