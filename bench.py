@@ -5,12 +5,12 @@
 A step is one forward and one inverse transform of the whole batch
 (synthetic inputs uniform in [-1, 1), seed 0, already resident in HBM).
 Prints one JSON line: value = GFLOP/s by the 5 N log2 N convention (both
-directions counted), the roofline of the dominant kernel (algorithmic bytes
-per launch over its mean launch duration in the HIP-event-timed region: every
-launch of a step is one pass of the same kernel over the whole array, and the
-launches run back to back on the plan's stream), and the CPU baseline (the
-oracle's FP32 port of FFTJ's threaded four-step, timed on this host on a
-bounded sample).
+directions counted), the roofline (the transform's algorithmic bytes, 16 N
+per direction, over the HIP-event-timed step; every launch of a step is one
+pass of the pass kernel over the whole array, the launches back to back on
+the plan's stream, and the per-launch rate is a secondary field), and the
+CPU baseline (the oracle's FP32 port of FFTJ's threaded four-step, timed on
+this host on a bounded sample).
 """
 import argparse
 import json
@@ -105,6 +105,34 @@ def copy_probe():
     print(json.dumps({"copy_probe": out}), flush=True)
 
 
+def host_cpu():
+    """What the CPU baseline ran on: model, logical CPUs, the CPUs this
+    process may use, AVX-512 presence, and how the oracle was compiled."""
+    model, avx512 = "unknown", False
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags"):
+                    avx512 = avx512 or " avx512f" in line
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": os.cpu_count(), "cpus_allowed": allowed, "avx512f": avx512,
+            "oracle_build": "gcc -O3 -march=x86-64-v3 (AVX2; the compiler's auto-vectorisation of scalar C, "
+                            "no explicit SIMD lanes)"}
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: the CPUs this process may use, at most 16
+    (the host share of one GPU)."""
+    return min(host_cpu()["cpus_allowed"], 16)
+
+
 def cpu_baseline(n, batch, target_s):
     """The oracle's FP32 restatement of FFTJ's threaded path, timed here."""
     import numpy as np
@@ -112,7 +140,7 @@ def cpu_baseline(n, batch, target_s):
     import __graft_entry__
 
     F = __graft_entry__.load_package()
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     x = F.rng.uniform_complex(0, n * batch).astype(np.complex64)
     flops_step = 2 * 5.0 * n * math.log2(n) * batch
 
@@ -143,6 +171,7 @@ def cpu_baseline(n, batch, target_s):
         "kind": "port",
         "sample": f"{reps} forward+inverse steps of the same workload on {threads} threads "
                   f"(oracle FP32 port of FFTJ's threaded {'FourStepPlan' if batch == 1 else 'per-transform FFTPlan loop'})",
+        "host": host_cpu(),
     }
 
 
@@ -158,7 +187,7 @@ def cpu_configs_leg(F, per_config_s=3.0):
     import numpy as np
     from oracle import oracle as O
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     pool = ThreadPoolExecutor(threads)
     rng = F.rng
 
@@ -447,33 +476,37 @@ def main():
     # oracle by the tests; here only a cheap property)
     rt_err = (torch.linalg.vector_norm(z - x) / torch.linalg.vector_norm(x)).item()
 
-    # dominant kernel: every launch of a step is one pass of the pass kernel
-    # (one template, instantiated per direction) over the whole array
-    # (16*N*batch algorithmic bytes), issued back to back on the plan's stream
-    # (a kernel trace of this command shows no gaps between them), so the
-    # timed region's HIP events give its mean launch duration as
-    # ms_per_step / launches per step (events between launches would add
-    # gaps to what they time)
+    # Roofline on the transform's algorithmic bytes: one read and one write of
+    # the complex FP32 array per direction (16*N*batch bytes), two directions
+    # per step, over the HIP-event-timed step.  The plan's launches run back
+    # to back on torch's current stream, which the events are recorded on (a
+    # kernel trace of this command shows no gaps between them), so the step
+    # time is the kernels' time.  The per-launch figure (the same bytes over
+    # the mean launch duration, one launch being one pass over the whole
+    # array) is kept as a secondary field: it is what the kernel trace's
+    # per-launch average is compared with.
     import re as _re
     desc_f, desc_i = pf.describe(), pi.describe()
     nl_f = int(_re.search(r"launches=(\d+)", desc_f).group(1))
     nl_i = int(_re.search(r"launches=(\d+)", desc_i).group(1))
     kernel = _re.search(r"n=(\d+) radices=\S+ kernel=(\w+)", desc_f)
     t_launch = ms_step * 1e-3 / (nl_f + nl_i)
-    bytes_per_launch = 16.0 * n * batch  # one read + one write of the complex FP32 array
-    achieved = bytes_per_launch / t_launch / 1e9
+    bytes_per_direction = 16.0 * n * batch  # one read + one write of the complex FP32 array
+    achieved = 2 * bytes_per_direction / (ms_step * 1e-3) / 1e9
+    launch_gbs = bytes_per_direction / t_launch / 1e9
     # per-launch durations of each plan run alone (events between launches:
     # each includes the gap before it), on request
     launches = None
     if args.launch_breakdown:
         launches = pf.profile_launches(y, x, iters=20) + pi.profile_launches(z, y, iters=20)
-    # HBM bytes per pass launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    # HBM bytes per direction (the sum over one direction's launches) from the
+    # committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this workload's plan
     traffic = None
     tp = os.path.join(ROOT, "profiles", f"pmc_traffic_{args.workload}.md")
     if os.path.exists(tp):
         with open(tp) as f:
             for line in f:
-                if line.startswith("hbm_bytes_per_launch:"):
+                if line.startswith("hbm_bytes_per_direction:"):
                     traffic = int(line.split(":")[1])
 
     result = {
@@ -492,14 +525,16 @@ def main():
         "data": "synthetic: uniform [-1,1) complex, splitmix64 seed 0, resident in HBM",
         "config": {"workload": args.workload, "description": desc, "n": n, "batch": batch,
                    "launches_per_direction": nl_f, "parallelism": "single GPU"},
-        # whole transform: 16*N*batch algorithmic bytes per direction, two directions per step
-        "hbm_frac_transform": round(2 * bytes_per_launch / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roundtrip_rel_err": rt_err,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": (f"{kernel.group(2)} pass kernel, {kernel.group(1)}-point transforms, both "
-                                f"directions: the {nl_f + nl_i} launches of a step" if kernel else "pass kernel"),
-                     "launch_us_mean": round(t_launch * 1e6, 2)},
+                     "basis": "16*N*batch algorithmic bytes per direction, 2 directions per step, over ms_per_step; "
+                              "traffic = counter bytes of one direction's launches",
+                     "us_per_direction": round(ms_step * 1e3 / 2, 2),
+                     "kernel": (f"{kernel.group(2)} pass kernel, {kernel.group(1)}-point transforms, "
+                                f"{nl_f} launches per direction" if kernel else "pass kernel"),
+                     "per_launch": {"launch_us_mean": round(t_launch * 1e6, 2), "launch_gbs": round(launch_gbs, 1),
+                                    "launch_frac": round(launch_gbs / HBM_PEAK_GBS, 4)}},
     }
     if launches is not None:
         result["roofline"]["launch_ms_isolated"] = [round(v, 5) for v in launches]
