@@ -14,11 +14,12 @@ LIB_PATH = os.environ.get("FFTJH_LIB") or os.path.join(HERE, "lib", "libfftjh.so
 OK, ERR_ARGUMENT, ERR_DIMENSION, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
 KIND_C2C, KIND_R2C, KIND_C2R, KIND_RAGGED = 0, 1, 2, 3
+STATIC_SCRATCH_BYTES = 64 << 20  # FFTJH_STATIC_SCRATCH_BYTES of include/fftjh.h
 
 # every entry point include/fftjh.h declares
 EXPORTS = (
     "fftjh_plan_c2c", "fftjh_plan_r2c", "fftjh_plan_c2r", "fftjh_plan_r2c_nd", "fftjh_plan_c2r_nd",
-    "fftjh_execute", "fftjh_execute_checked",
+    "fftjh_execute", "fftjh_execute_checked", "fftjh_execute_static",
     "fftjh_plan_info", "fftjh_plan_describe", "fftjh_describe_request", "fftjh_describe_tuned", "fftjh_plan_destroy",
     "fftjh_last_error", "fftjh_version", "fftjh_profile_launches", "fftjh_plan_c2c_tuned", "fftjh_plan_ragged",
     "fftjh_plan_ragged_tuned", "fftjh_mlp_create", "fftjh_mlp_destroy", "fftjh_mlp_param_count",
@@ -118,6 +119,7 @@ def lib():
         L.fftjh_plan_c2r_nd.argtypes = [ctypes.POINTER(vp), ci, ci, ctypes.POINTER(i64), i64, ci, ctypes.c_uint]
         L.fftjh_execute.argtypes = [vp, vp, vp, vp]
         L.fftjh_execute_checked.argtypes = [vp, vp, i64, vp, i64, vp]
+        L.fftjh_execute_static.argtypes = [vp, vp, vp, vp]
         L.fftjh_plan_info.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci),
                                       ctypes.POINTER(ci), P64, P64, ctypes.POINTER(ctypes.c_double), P64, P64]
         L.fftjh_plan_describe.argtypes = [vp, ctypes.c_char_p, i64]

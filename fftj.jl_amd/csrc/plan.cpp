@@ -1921,7 +1921,52 @@ static void materialize(fftjh_plan_s& P) {
     if (L.scratch_elems[i]) P.scratch[i] = std::make_shared<DevBuf>(L.scratch_elems[i] * es);
 }
 
-static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const* ptr, hipStream_t st) {
+// ------------------------------------------- fixed per-device scratch --
+//
+// fftjh_execute_static runs a plan with its scratch (the intermediates of
+// multi-pass plans, the ragged per-workgroup workspace) carved out of this
+// array instead of the buffers the plan allocated, so the call itself
+// allocates nothing.  Every GPU the library runs on holds its own copy from
+// module load to process end.  In host code `scratch` names a host-side
+// stand-in, never the device memory: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[FFTJH_STATIC_SCRATCH_BYTES];
+
+static std::mutex g_static_mu;
+static std::vector<char*> g_static_ptr;  // device address of `scratch`, per device
+
+// the current device's copy of `scratch` (looked up on its first use there)
+static char* static_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  if (static_cast<int>(g_static_ptr.size()) <= device) g_static_ptr.resize(device + 1, nullptr);
+  if (!g_static_ptr[device]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) fail(FFTJH_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_static_ptr[device] = static_cast<char*>(p);
+  }
+  return g_static_ptr[device];
+}
+
+static int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
+
+// Byte offsets of the scratch roles and of the ragged workspace inside the
+// fixed array (each 256-byte aligned); returns the bytes the plan needs.
+static int64_t static_layout(const fftjh_plan_s& P, int64_t role_off[4], int64_t* ws_off) {
+  const int64_t es = esize(P.L.dtype);
+  int64_t at = 0;
+  for (int i = 0; i < 4; ++i) {
+    role_off[i] = at;
+    at += align256(P.L.scratch_elems[i] * es);
+  }
+  *ws_off = at;
+  for (const Step& s : P.L.steps)
+    if (s.kind == ST_RAGGED) at += align256(s.rg->ws_total * es);
+  return at;
+}
+
+static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const* ptr, hipStream_t st,
+                              void* ws = nullptr) {
   const int dt = P.L.dtype;
   switch (s.kind) {
     case ST_PASS:
@@ -1964,7 +2009,7 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
       rl.corr = R.d_corr->p;
       rl.x = ptr[s.src];
       rl.y = ptr[s.dst];
-      rl.ws = R.d_ws->p;
+      rl.ws = ws ? ws : R.d_ws->p;
       rl.wg_ws = R.d_wgws->p;
       return launch_ragged(dt, s.dir, rl, st);
     }
@@ -1972,12 +2017,23 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
   return hipErrorInvalidValue;
 }
 
-static void run(const fftjh_plan_s& P, void* y, const void* x, hipStream_t st, hipEvent_t* ev) {
+// static_base: the device's fixed scratch array (fftjh_execute_static), or
+// NULL for the plan's own scratch buffers
+static void run(const fftjh_plan_s& P, void* y, const void* x, hipStream_t st, hipEvent_t* ev,
+                char* static_base = nullptr) {
   void* ptr[kRoles] = {const_cast<void*>(x), y, nullptr, nullptr, nullptr, nullptr};
-  for (int i = 0; i < 4; ++i) ptr[RS0 + i] = P.scratch[i] ? P.scratch[i]->p : nullptr;
+  void* ws = nullptr;
+  if (static_base) {
+    int64_t role_off[4], ws_off = 0;
+    static_layout(P, role_off, &ws_off);
+    for (int i = 0; i < 4; ++i) ptr[RS0 + i] = P.L.scratch_elems[i] ? static_base + role_off[i] : nullptr;
+    ws = static_base + ws_off;
+  } else {
+    for (int i = 0; i < 4; ++i) ptr[RS0 + i] = P.scratch[i] ? P.scratch[i]->p : nullptr;
+  }
   for (size_t i = 0; i < P.L.steps.size(); ++i) {
     if (ev) hip_check(hipEventRecord(ev[2 * i], st), "hipEventRecord");
-    hip_check(launch_step(P, P.L.steps[i], ptr, st), kStepNames[P.L.steps[i].kind]);
+    hip_check(launch_step(P, P.L.steps[i], ptr, st, ws), kStepNames[P.L.steps[i].kind]);
     if (ev) hip_check(hipEventRecord(ev[2 * i + 1], st), "hipEventRecord");
   }
 }
@@ -2122,6 +2178,27 @@ int fftjh_execute(fftjh_plan plan, void* y, const void* x, fftjh_stream stream) 
     if (!plan) fail(FFTJH_ERR_ARGUMENT, "plan is NULL");
     if (!x || !y) fail(FFTJH_ERR_ARGUMENT, "x and y must be device pointers");
     run(*plan, y, x, static_cast<hipStream_t>(stream), nullptr);
+  });
+}
+
+int fftjh_execute_static(fftjh_plan plan, void* y, const void* x, fftjh_stream stream) {
+  return guarded([&] {
+    if (!plan) fail(FFTJH_ERR_ARGUMENT, "plan is NULL");
+    if (!x || !y) fail(FFTJH_ERR_ARGUMENT, "x and y must be device pointers");
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != plan->device)
+      fail(FFTJH_ERR_ARGUMENT, "plan was built on device " + std::to_string(plan->device) +
+                                   ", the current device is " + std::to_string(dev));
+    int64_t role_off[4], ws_off = 0;
+    const int64_t need = static_layout(*plan, role_off, &ws_off);
+    if (need > FFTJH_STATIC_SCRATCH_BYTES)
+      fail(FFTJH_ERR_UNSUPPORTED, "plan needs " + std::to_string(need) +
+                                      " bytes of scratch; the fixed scratch of fftjh_execute_static holds " +
+                                      std::to_string(static_cast<int64_t>(FFTJH_STATIC_SCRATCH_BYTES)) +
+                                      " (use fftjh_execute)");
+    char* base = need > 0 ? static_scratch(dev) : nullptr;
+    run(*plan, y, x, static_cast<hipStream_t>(stream), nullptr, base);
   });
 }
 

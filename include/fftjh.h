@@ -164,6 +164,27 @@ int fftjh_plan_c2r_nd(fftjh_plan* plan, int dtype, int rank, const int64_t* dims
  * api.jl:34-55).  No length checks (see fftjh_execute_checked). */
 int fftjh_execute(fftjh_plan plan, void* y, const void* x, fftjh_stream stream);
 
+/* Bytes of the fixed per-device scratch array of fftjh_execute_static. */
+#define FFTJH_STATIC_SCRATCH_BYTES (64ll << 20)
+
+/* As fftjh_execute, but allocation-free by construction: the plan's scratch
+ * (the intermediates of multi-pass plans and the ragged per-workgroup
+ * workspace) is carved out of one fixed array of FFTJH_STATIC_SCRATCH_BYTES
+ * that the library holds on each GPU it runs on, from module load to process
+ * end, instead of the buffers the plan allocated.  The call makes no
+ * hipMalloc, hipFree or other allocating call (its first call on a device
+ * looks up that device's array).  Plans needing more scratch than the array
+ * holds are rejected with FFTJH_ERR_UNSUPPORTED before any work is queued.
+ * Every call on one GPU shares that GPU's array and it keeps the last call's
+ * contents (no plan relies on scratch starting at zero), so calls that
+ * overlap in time on one GPU (two streams, two host threads) race on it and
+ * must be serialised by the caller.  The current device must be the plan's.
+ * No counterpart in the reference (its plans own their scratch, plan.jl:52-55,
+ * and allocate nothing when nthreads = 1): this entry point is for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc. */
+int fftjh_execute_static(fftjh_plan plan, void* y, const void* x, fftjh_stream stream);
+
 /* As fftjh_execute, after checking the element counts of y and x against
  * the plan (complex elements for complex sides, reals for real sides);
  * returns FFTJH_ERR_DIMENSION before queuing anything on a mismatch, as
