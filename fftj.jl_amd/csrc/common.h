@@ -116,6 +116,30 @@ struct PassDesc {
   int32_t nt_store;
 };
 
+// One launch of the lean column-pass kernel (stockham_lean.hip): `groups`
+// workgroups, each 16 adjacent transforms of n = 64, 128 or 256 points.
+// Group g has digits d0 = g % D0, d1 = (g / D0) % D1, d2 = g / (D0*D1); its
+// input starts at d0*in_s0 + d1*in_s1 + d2*in_s2, column c at +c, element m
+// at +m*istride.  Output: rows == 0 the same form with out_s* and ostride;
+// rows == 1 column c at out_base + c*ostride, elements contiguous.
+// Multi-pass twiddle w_P^e, P = 2^psh (psh = 0: none),
+// e = d1*(tw_a0*(16*d0 + c) + tw_b*k) mod P, from the FP64 tables ptw
+// (w_P^(i*2^ptw_sh) for i < ptw_lo, then w_P^i).
+struct LeanDesc {
+  int32_t n;
+  int32_t rows;
+  int32_t groups;
+  int32_t D0, D1;
+  int64_t in_s0, in_s1, in_s2, istride;
+  int64_t out_s0, out_s1, out_s2, ostride;
+  int32_t psh;
+  int32_t ptw_sh;
+  int64_t ptw_lo;
+  int64_t tw_a0, tw_b;
+  const void* ptw;
+  double scale;
+};
+
 // LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)
 inline int lds_elems(int cn) { return cn + (cn >> 4) + 1; }
 

@@ -25,6 +25,11 @@ hipError_t launch_pass_pow2(int dtype, int dir, const PassDesc& d, const PassDes
 int pass_pow2_resident_groups(int dtype, int dir, const PassDesc& d);
 int pow2_max_threads(int dtype);
 
+// lean column-pass kernel (FP32, n = 64, 128, 256); tws: the table
+// w_n^i, i < n, of the pass's direction
+bool lean_length(int n);
+hipError_t launch_lean(int dir, const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t stream);
+
 // split-precision MFMA pass kernels (FP32): mode is a MatMode (stages_mat.h),
 // mat the operand table launch_mat_table wrote for that mode and direction
 hipError_t launch_pass_mat(int mode, int dir, const PassDesc& d, const PassDesc* dev_desc, const void* x, void* y,

@@ -309,15 +309,21 @@ enum StepKind {
   ST_RS_INTERP,
   ST_STRIDED_COPY,
   ST_RAGGED,
+  ST_LEAN,
 };
 
 static const char* kStepNames[] = {"pass",           "r2c_twist", "c2r_pre",   "real_to_complex", "copy_prefix",
                                    "hermitian_fill", "take_real", "rs_deconv", "rs_interp",       "strided_copy",
-                                   "ragged"};
+                                   "ragged",         "lean"};
 
 struct Step {
   int kind = ST_PASS;
   int src = RX, dst = RY;
+  // roles when the caller's x and y are the same array (in-place call);
+  // -1: the same as src / dst
+  int alias_src = -1, alias_dst = -1;
+  // ST_LEAN
+  LeanDesc ld{};
   int dir = -1;
   // ST_PASS
   PassDesc d{};
@@ -343,6 +349,7 @@ struct Layout {
   int mat = kMatNone;   // split-precision MFMA mode of every pass (FP32 C2C)
   unsigned route = 0;   // FFTJH_FLAG_ROUTE_* for non-smooth lengths (0 = planner)
   bool two_pass512 = true;  // the smaller 1-D lengths may take two passes of up to 512 points
+  bool lean = false;        // a 1-D complex plan: its three-pass schedule may take the lean layout
   int dtype = 0, kind = FFTJH_KIND_C2C, sign = -1, rank = 1;
   int64_t dims[8] = {0};
   int64_t batch = 1;
@@ -846,6 +853,98 @@ static bool use_composite(const Layout& L, int64_t n, int64_t s) {
   return single_pass_ok(L.dtype, rs_params(L.dtype, p).m) && !single_pass_ok(L.dtype, rs_params(L.dtype, n).m);
 }
 
+// Three power-of-two passes run by the lean kernel inside the output array
+// (FP32, factors 64..256, contiguous rows, default tuning).  With
+// j = j1 + N1*j2 + N1*N2*j3 and k = k3 + N3*k2 + N2*N3*k1:
+//   A  x -> y: columns j3 -> k3 of (j1, j2) at stride N1*N2, written as the
+//      contiguous row k3 at N3*j2 + N2*N3*j1, times w_{N2 N3}^(j2*k3);
+//   B  y -> y: j2 -> k2 of (j1, k3) at stride N3, in place, times
+//      w_n^(j1*(k3 + N3*k2));
+//   C  y -> y: j1 -> k1 of k' = k3 + N3*k2 at stride N2*N3, in place, to the
+//      natural-order output k' + N2*N3*k1.
+// The plan touches only x and y (two arrays where the rotating layout
+// touches three: measured on one MI355X, a 128 MiB pass of 128-byte runs
+// copies in 33 us within two buffers and 48-50 us across three).  When the
+// caller passes x == y the same launches run x -> z -> z -> y through the
+// scratch array z.  FFTJH_LEAN=0 keeps the generic three-pass layout.
+static bool tune_is_default(const fftjh_tuning& t) {
+  static const fftjh_tuning zero{};
+  return std::memcmp(&t, &zero, sizeof t) == 0;
+}
+
+static bool lean3_ok(const Layout& L, const std::vector<int64_t>& F, int64_t n, int64_t batch, int src, int dst) {
+  static const bool env = [] {
+    const char* e = std::getenv("FFTJH_LEAN");
+    return !e || std::atoi(e) != 0;
+  }();
+  if (!env || !L.lean || L.dtype != FFTJH_F32 || L.mat || !tune_is_default(L.tune) || src != RX || dst != RY)
+    return false;
+  if (F.size() != 3) return false;
+  for (int64_t f : F)
+    if (!lean_length(static_cast<int>(f))) return false;
+  // 32-bit lane offsets (< n/16 + 16) and group counts (< 2^31)
+  return n < (int64_t(1) << 31) && batch * (n / 1024) < (int64_t(1) << 31);
+}
+
+static void lean_twiddle(LeanDesc& d, int64_t P, int64_t a0, int64_t b) {
+  int bits = 0;
+  while ((int64_t(1) << bits) < P) ++bits;
+  d.psh = bits;
+  d.ptw_sh = (bits + 1) / 2;
+  d.ptw_lo = ((P - 1) >> d.ptw_sh) + 1;
+  d.tw_a0 = a0;
+  d.tw_b = b;
+}
+
+static void add_lean3(Layout& L, const std::vector<int64_t>& F, int64_t n, int64_t batch, int dir, double scale,
+                      int src, int dst, int z) {
+  const int64_t N1 = F[0], N2 = F[1], N3 = F[2], N23 = N2 * N3;
+  need_scratch(L, z, n * batch);  // the x == y variant
+  auto mk = [&](int N, int rows, int64_t D0, int64_t D1, int s_, int d_, int as_, int ad_) {
+    Step st;
+    st.kind = ST_LEAN;
+    st.dir = dir;
+    st.src = s_;
+    st.dst = d_;
+    st.alias_src = as_;
+    st.alias_dst = ad_;
+    st.ld.n = N;
+    st.ld.rows = rows;
+    st.ld.D0 = static_cast<int32_t>(D0);
+    st.ld.D1 = static_cast<int32_t>(D1);
+    st.ld.groups = static_cast<int32_t>(batch * (n / (16 * N)));
+    st.ld.scale = 1.0;
+    st.n = N;
+    st.batch = batch * (n / N);
+    return st;
+  };
+  Step a = mk(static_cast<int>(N3), 1, N1 / 16, N2, src, dst, src, z);
+  a.ld.in_s0 = 16;
+  a.ld.in_s1 = N1;
+  a.ld.in_s2 = n;
+  a.ld.istride = N1 * N2;
+  a.ld.out_s0 = 16 * N23;
+  a.ld.out_s1 = N3;
+  a.ld.out_s2 = n;
+  a.ld.ostride = N23;
+  lean_twiddle(a.ld, N23, 0, 1);
+  Step b = mk(static_cast<int>(N2), 0, N3 / 16, N1, dst, dst, z, z);
+  b.ld.in_s0 = b.ld.out_s0 = 16;
+  b.ld.in_s1 = b.ld.out_s1 = N23;
+  b.ld.in_s2 = b.ld.out_s2 = n;
+  b.ld.istride = b.ld.ostride = N3;
+  lean_twiddle(b.ld, n, 1, N3);
+  Step c = mk(static_cast<int>(N1), 0, N23 / 16, batch, dst, dst, z, dst);
+  c.ld.in_s0 = c.ld.out_s0 = 16;
+  c.ld.in_s1 = c.ld.out_s1 = n;
+  c.ld.istride = c.ld.ostride = N23;
+  c.ld.scale = scale;
+  L.steps.push_back(a);
+  L.steps.push_back(b);
+  L.steps.push_back(c);
+  L.npass += 3;
+}
+
 static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry gin, Geometry gout, int dir,
                       double scale, int src, int dst, std::vector<int> free) {
   const bool split_asked = L.tune.splits[0] > 0 && L.npass == 0;
@@ -933,6 +1032,10 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
     // the row schedule below scales by cs, and the column joins the low
     // digits, b % (cs*S_d) = column + cs*jl
     const int64_t cs = colblock ? Bi : 1;
+    if (P == 3 && cs == 1 && lean3_ok(L, F, n, batch, src, dst)) {
+      add_lean3(L, F, n, batch, dir, scale, src, dst, z);
+      return;
+    }
     if (P == 3 && mid_inplace && cs == 1) {
       // Three passes through one intermediate z: x -> z (digit j3 -> k3 in
       // place), z -> z (j2 -> k2 in place, twiddle w_n^{j1 (k3 + N3 k2)}),
@@ -1137,6 +1240,7 @@ static Layout layout_c2c(int dtype, int rank, const int64_t* dims, int64_t batch
   L.scale = scale;
   L.in_elems = L.out_elems = len * batch;
   if (rank == 1) {
+    L.lean = true;
     build_c2c_rows(L, dims[0], batch, sign, scale, RX, RY, {RS0, RS1, RS2, RS3});
     return L;
   }
@@ -1684,6 +1788,19 @@ static std::string describe(const Layout& L) {
   for (int i = 0; i < L.rank; ++i) o << (i ? "x" : "") << L.dims[i];
   o << " batch=" << L.batch << " sign=" << L.sign << " launches=" << L.steps.size() << "\n";
   for (const Step& s : L.steps) {
+    if (s.kind == ST_LEAN) {
+      // a pass line (the planners' features read pass lines)
+      const LeanDesc& d = s.ld;
+      o << "  pass " << roles[s.src] << "->" << roles[s.dst] << " n=" << d.n << " radices=16," << d.n / 16
+        << " kernel=lean" << (d.rows ? " rows_out" : "") << " batch=" << s.batch << " C=16 threads=" << d.n
+        << " groups=" << d.groups << " in_stride=" << d.istride << " out_stride=" << (d.rows ? 1 : d.ostride);
+      if (d.psh) o << " twiddle=w_" << (int64_t(1) << d.psh);
+      if (d.scale != 1.0) o << " scale=" << d.scale;
+      if (s.alias_src != s.src || s.alias_dst != s.dst)
+        o << " aliased=" << roles[s.alias_src] << "->" << roles[s.alias_dst];
+      o << "\n";
+      continue;
+    }
     o << "  " << kStepNames[s.kind] << " " << roles[s.src] << "->" << roles[s.dst];
     if (s.kind == ST_PASS) {
       o << " n=" << s.d.n << " radices=";
@@ -1915,6 +2032,21 @@ static void materialize(fftjh_plan_s& P) {
                      });
     } else if (s.kind == ST_RS_DECONV) {
       s.aux = rs_corr(s);
+    } else if (s.kind == ST_LEAN) {
+      // stage table w_n^i (i < n) and the multi-pass twiddle's FP64 tables
+      const int N = s.ld.n, dir = s.dir;
+      s.tw = cached(key_of("lean", {dev, double(dir), double(N)}), int64_t(N) * 8,
+                    [&](void* p) { hip_check(launch_tw_pow(FFTJH_F32, p, N, 1, N, dir, 0), "twiddle kernel"); });
+      if (s.ld.psh) {
+        const int64_t P = int64_t(1) << s.ld.psh, hi = s.ld.ptw_lo, lo = int64_t(1) << s.ld.ptw_sh;
+        std::ostringstream pk;
+        pk << "ptw:" << dev << ":" << dir << ":" << P;
+        s.ptw = cached(pk.str(), (hi + lo) * 16, [&](void* p) {
+          hip_check(launch_tw_pow(FFTJH_F64, p, hi, lo, P, dir, 0), "twiddle kernel");
+          hip_check(launch_tw_pow(FFTJH_F64, static_cast<char*>(p) + hi * 16, lo, 1, P, dir, 0), "twiddle kernel");
+        });
+        s.ld.ptw = s.ptw->p;
+      }
     }
   }
   for (int i = 0; i < 4; ++i)
@@ -1995,6 +2127,8 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
     case ST_RS_INTERP:
       return launch_rs_interp(dt, ptr[s.src], ptr[s.dst], s.aux->p, s.pieces, s.degree, s.n, s.m, s.w, s.batch, s.Bi,
                               s.gb, st);
+    case ST_LEAN:
+      return launch_lean(s.dir, s.ld, ptr[s.src], ptr[s.dst], s.tw->p, st);
     case ST_STRIDED_COPY:
       return launch_strided_copy(dt, ptr[s.src], ptr[s.dst], s.n, s.batch, s.Bi, s.ga, s.gb, st);
     case ST_RAGGED: {
@@ -2031,9 +2165,18 @@ static void run(const fftjh_plan_s& P, void* y, const void* x, hipStream_t st, h
   } else {
     for (int i = 0; i < 4; ++i) ptr[RS0 + i] = P.scratch[i] ? P.scratch[i]->p : nullptr;
   }
+  // an in-place call (x == y) takes the steps' alias roles where they have them
+  const bool alias = x == static_cast<const void*>(y);
   for (size_t i = 0; i < P.L.steps.size(); ++i) {
+    const Step& s = P.L.steps[i];
     if (ev) hip_check(hipEventRecord(ev[2 * i], st), "hipEventRecord");
-    hip_check(launch_step(P, P.L.steps[i], ptr, st, ws), kStepNames[P.L.steps[i].kind]);
+    if (s.kind == ST_LEAN) {
+      const int sr = alias && s.alias_src >= 0 ? s.alias_src : s.src;
+      const int dr = alias && s.alias_dst >= 0 ? s.alias_dst : s.dst;
+      hip_check(launch_lean(s.dir, s.ld, ptr[sr], ptr[dr], s.tw->p, st), kStepNames[s.kind]);
+    } else {
+      hip_check(launch_step(P, s, ptr, st, ws), kStepNames[s.kind]);
+    }
     if (ev) hip_check(hipEventRecord(ev[2 * i + 1], st), "hipEventRecord");
   }
 }

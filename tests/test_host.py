@@ -65,15 +65,20 @@ def test_schedule_config2():
     # (a) 4096 x 16384 contiguous: one launch, one transform per workgroup row
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [4096], batch=16384))
     assert len(s) == 1 and "batch=16384" in s[0]
-    # (b) 2^24: three 256-point passes (nested four-step), twiddles fused
-    # into the first two, natural-order output from the third; the middle
-    # pass works in place on the one intermediate
+    # (b) 2^24: three 256-point passes (nested four-step) on the lean kernel
+    # inside y: columns of x to rows of y with the w_65536 twiddle, then two
+    # in-place column passes (the second with w_2^24); an in-place call
+    # (x == y) runs the same launches through s0
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1 << 24]))
-    assert len(s) == 3 and all("n=256" in p and "kernel=pow2" in p for p in s)
-    assert s[0].startswith("pass x->s0") and s[1].startswith("pass s0->s0") and s[2].startswith("pass s0->y")
-    assert "in_stride=65536" in s[0] and "twiddle=w_65536" in s[0]
+    assert len(s) == 3 and all("n=256" in p and "kernel=lean" in p for p in s)
+    assert s[0].startswith("pass x->y") and s[1].startswith("pass y->y") and s[2].startswith("pass y->y")
+    assert "in_stride=65536" in s[0] and "rows_out" in s[0] and "twiddle=w_65536" in s[0]
     assert "in_stride=256" in s[1] and "twiddle=w_16777216" in s[1]
-    assert "in_stride=1 " in s[2] and "out_stride=65536" in s[2] and "twiddle" not in s[2]
+    assert "in_stride=65536" in s[2] and "out_stride=65536" in s[2] and "twiddle" not in s[2]
+    assert s[0].endswith("aliased=x->s0") and s[1].endswith("aliased=s0->s0") and s[2].endswith("aliased=s0->y")
+    # FP64 keeps the generic three-pass layout
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F64, [1 << 24]))
+    assert len(s) == 3 and all("kernel=pow2" in p for p in s) and s[1].startswith("pass s0->s0")
     # (c) 10^6 = 100 x 100 x 100
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [10**6], batch=8))
     assert len(s) == 3 and all("n=100 " in p for p in s)
@@ -106,7 +111,7 @@ def test_schedule_config3():
 def test_stream_hints_rule():
     # footprints of 384 MiB and up: the pass reading x streams its loads, the
     # pass writing y last streams its stores; smaller plans keep the caches
-    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1 << 24]))
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F64, [1 << 24]))
     assert s[0].endswith("stream=in") and "stream=" not in s[1] and s[2].endswith("stream=out")
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [4096], batch=16384))
     assert len(s) == 1 and s[0].endswith("stream=inout")
