@@ -80,24 +80,31 @@ def pmc_probe(args):
 
 
 def copy_probe():
-    """Streaming ceiling on this device for the 2^24 footprint: torch copies
-    of 128 MiB among three rotating buffers (as the three passes of the
-    2^24 plan touch three arrays), and between two fixed buffers."""
+    """Streaming ceilings on this device at the 2^24 footprint: torch copies
+    of 128 MiB among three rotating buffers, between two fixed buffers, and
+    an in-place pass (read and write) over one 128 MiB array -- the three
+    footprints a pass of the 2^24 plans can have (the in-place one fits the
+    256 MB Infinity Cache)."""
     import torch
     n = 1 << 24
     b = [torch.zeros(n, dtype=torch.complex64, device="cuda:0") for _ in range(3)]
     out = {}
-    for name, pairs in (("rotating_3x128MiB", [(0, 1), (1, 2), (2, 0)]), ("fixed_2x128MiB", [(0, 1)])):
-        for i in range(6):
+    cases = (("rotating_3x128MiB", [(0, 1), (1, 2), (2, 0)]), ("fixed_2x128MiB", [(0, 1)]), ("inplace_1x128MiB", [(0, 0)]))
+    for name, pairs in cases:
+        def one(i):
             s, d = pairs[i % len(pairs)]
-            b[d].copy_(b[s])
+            if s == d:
+                b[d].mul_(1.0)
+            else:
+                b[d].copy_(b[s])
+        for i in range(6):
+            one(i)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         it = 60
         e0.record()
         for i in range(it):
-            s, d = pairs[i % len(pairs)]
-            b[d].copy_(b[s])
+            one(i)
         e1.record()
         torch.cuda.synchronize()
         us = e0.elapsed_time(e1) * 1e3 / it
