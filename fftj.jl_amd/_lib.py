@@ -45,8 +45,8 @@ class Tuning(ctypes.Structure):
 
 FLAG_MFMA_F16, FLAG_MFMA_BF16, FLAG_MFMA_BF16X3 = 0x1, 0x2, 0x4
 PRECISION_FLAGS = {"f16": FLAG_MFMA_F16, "bf16": FLAG_MFMA_BF16, "bf16x3": FLAG_MFMA_BF16X3}
-FLAG_ROUTE_COMPOSITE, FLAG_ROUTE_RESAMPLE = 0x10, 0x20
-ROUTE_FLAGS = {"composite": FLAG_ROUTE_COMPOSITE, "resample": FLAG_ROUTE_RESAMPLE}
+FLAG_ROUTE_COMPOSITE, FLAG_ROUTE_RESAMPLE, FLAG_ROUTE_HVDH = 0x10, 0x20, 0x40
+ROUTE_FLAGS = {"composite": FLAG_ROUTE_COMPOSITE, "resample": FLAG_ROUTE_RESAMPLE, "hvdh": FLAG_ROUTE_HVDH}
 
 
 def make_tuning(splits=(), passes=(), flags=0):

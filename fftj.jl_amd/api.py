@@ -392,11 +392,13 @@ def _precision_flags(precision):
 def _route_flags(route):
     """route=None (the planner's choice) or, for a length that is not
     13-smooth, "composite" (one exact step over its smooth part around the
-    resampled rest, composite.jl) or "resample" (the whole length resampled)."""
+    resampled rest, composite.jl), "resample" (the whole length resampled,
+    resample.jl) or "hvdh" (odd lengths, FP32: Gaussian resampling onto a
+    coprime smooth grid and chunked banded solves, hvdh.jl)."""
     if route is None:
         return 0
     if route not in _lib.ROUTE_FLAGS:
-        raise ArgumentError(f"route must be None, 'composite' or 'resample' (got {route!r})")
+        raise ArgumentError(f"route must be None, 'composite', 'resample' or 'hvdh' (got {route!r})")
     return _lib.ROUTE_FLAGS[route]
 
 

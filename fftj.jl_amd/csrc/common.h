@@ -140,6 +140,28 @@ struct LeanDesc {
   double scale;
 };
 
+// HvdH route (hvdh.hip) of `batch` transforms of odd non-smooth n, FP32:
+// the input permuted to Delta order (cinv = Delta^-1 mod n), spread onto the
+// smooth grid t (Delta = t - n; tap o reads permuted element
+// (q + tap[o + h]) mod n, tap[o + h] = (o - 1)*Delta mod n), the grid
+// transform, `chunks` banded solves per transform of 16 core rows with H halo
+// rows per side, and the unknowns permuted back into y.
+struct HvdhDesc {
+  int64_t n, t, delta, cinv;
+  int64_t batch, chunks;
+  int32_t h;         // spread taps o = 1-h .. h
+  int32_t B;         // band half width of the solve (2)
+  int32_t H;         // halo rows per side
+  int32_t pad_;
+  int64_t tap[16];   // (o - 1)*Delta mod n for o = 1-h .. h, at o + h
+  double b, ialpha;  // spread weights exp(-b (delta - o)^2) / alpha
+  double kc[8];      // exp(-b o^2), o = 0..h
+  float a;           // solve entries exp(-a (d + beta)^2)
+  float nt;          // n / t
+  float c1, c2;      // exp(-a), exp(-4a)
+  float scale;
+};
+
 // LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)
 inline int lds_elems(int cn) { return cn + (cn >> 4) + 1; }
 

@@ -30,6 +30,16 @@ int pow2_max_threads(int dtype);
 bool lean_length(int n);
 hipError_t launch_lean(int dir, const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t stream);
 
+// HvdH route kernels (FP32): the Delta permutation dst[Delta*l mod n] =
+// scale*src[l], the spread of the permuted input onto grid rows of t, the
+// chunked banded solve grid -> unknowns in l order; the largest halo and
+// the core rows per chunk the solve kernel is built for
+hipError_t launch_hvdh_permute(const HvdhDesc& d, const void* src, void* dst, float scale, hipStream_t stream);
+hipError_t launch_hvdh_spread(const HvdhDesc& d, const void* xp, void* v, hipStream_t stream);
+hipError_t launch_hvdh_solve(const HvdhDesc& d, const void* Yg, void* y, hipStream_t stream);
+int hvdh_max_halo();
+int hvdh_chunk_rows();
+
 // split-precision MFMA pass kernels (FP32): mode is a MatMode (stages_mat.h),
 // mat the operand table launch_mat_table wrote for that mode and direction
 hipError_t launch_pass_mat(int mode, int dir, const PassDesc& d, const PassDesc* dev_desc, const void* x, void* y,

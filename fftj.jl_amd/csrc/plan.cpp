@@ -310,11 +310,14 @@ enum StepKind {
   ST_STRIDED_COPY,
   ST_RAGGED,
   ST_LEAN,
+  ST_HVDH_PERMUTE,
+  ST_HVDH_SPREAD,
+  ST_HVDH_SOLVE,
 };
 
 static const char* kStepNames[] = {"pass",           "r2c_twist", "c2r_pre",   "real_to_complex", "copy_prefix",
                                    "hermitian_fill", "take_real", "rs_deconv", "rs_interp",       "strided_copy",
-                                   "ragged",         "lean"};
+                                   "ragged",         "lean",      "hvdh_permute", "hvdh_spread", "hvdh_solve"};
 
 struct Step {
   int kind = ST_PASS;
@@ -324,6 +327,8 @@ struct Step {
   int alias_src = -1, alias_dst = -1;
   // ST_LEAN
   LeanDesc ld{};
+  // ST_HVDH_SPREAD, ST_HVDH_SOLVE
+  HvdhDesc hv{};
   int dir = -1;
   // ST_PASS
   PassDesc d{};
@@ -847,10 +852,166 @@ static bool split_column(int dtype, int64_t n) {
 // step too long for 16 columns per workgroup (4096*17 1.78).
 static bool use_composite(const Layout& L, int64_t n, int64_t s) {
   if (L.route & FFTJH_FLAG_ROUTE_COMPOSITE) return true;
-  if (L.route & FFTJH_FLAG_ROUTE_RESAMPLE) return false;
+  if (L.route & (FFTJH_FLAG_ROUTE_RESAMPLE | FFTJH_FLAG_ROUTE_HVDH)) return false;
   const int64_t p = n / s;
   if (s < 32 || s > 1024 || p < 512) return false;
   return single_pass_ok(L.dtype, rs_params(L.dtype, p).m) && !single_pass_ok(L.dtype, rs_params(L.dtype, n).m);
+}
+
+// ---- HvdH parameters (hvdh.jl:60-190) ----
+
+static int64_t gcd64(int64_t a, int64_t b) {
+  while (b) {
+    const int64_t r = a % b;
+    a = b;
+    b = r;
+  }
+  return a;
+}
+
+// t coprime to odd n: the smallest o * 2^k >= ceil(n (1 + 1/128)) over the
+// odd smooth o in {1, 3, 5, 7, 9, 15, 21} coprime to n (hvdh.jl:76-92);
+// 0 when there is none or t - n >= n
+static int64_t hvdh_grid(int64_t n) {
+  if (n % 2 == 0) return 0;
+  const int64_t N = static_cast<int64_t>(std::ceil(static_cast<long double>(n) * (1.0L + 1.0L / 128)));
+  int64_t t = INT64_MAX;
+  for (int64_t o : {1, 3, 5, 7, 9, 15, 21}) {
+    if (gcd64(o, n) != 1) continue;
+    int64_t c = o;
+    while (c < N) c <<= 1;
+    t = std::min(t, c);
+  }
+  return (t != INT64_MAX && t - n < n) ? t : 0;
+}
+
+// x^-1 mod m for gcd(x, m) = 1 (extended Euclid)
+static int64_t inv_mod(int64_t x, int64_t m) {
+  int64_t r0 = m, r1 = ((x % m) + m) % m, s0 = 0, s1 = 1;
+  while (r1) {
+    const int64_t q = r0 / r1, r2 = r0 - q * r1, s2 = s0 - q * s1;
+    r0 = r1;
+    r1 = r2;
+    s0 = s1;
+    s1 = s2;
+  }
+  return ((s0 % m) + m) % m;
+}
+
+// Halo rows per side (hvdh.jl:118-146): for probe unknowns l0 (16 spread over
+// [0, n) and the two with Delta*l0 = n/2, n/2 + 1 mod n, where beta is near
+// +-1/2 and the inverse decays slowest), the row of the inverse of the
+// (2P+1)-window of the l-indexed system around l0, solved densely in long
+// double, decays below tol within H columns; H' = ceil(rho H) + B rows of q.
+// 0 when no window up to P = 8192 shows the decay.
+static int64_t hvdh_halo(int64_t n, int64_t t, long double a, int B, long double tol) {
+  const long double rho = static_cast<long double>(t) / n;
+  const int64_t c = inv_mod(t - n, n);
+  std::vector<int64_t> probes;
+  const int64_t cnt = std::min<int64_t>(n, 16);
+  for (int64_t k = 0; k < cnt; ++k)
+    probes.push_back(cnt > 1 ? static_cast<int64_t>(std::llround(static_cast<long double>(k) * (n - 1) / (cnt - 1))) : 0);
+  probes.push_back(static_cast<int64_t>((static_cast<__int128>(n / 2) * c) % n));
+  probes.push_back(static_cast<int64_t>((static_cast<__int128>(n / 2 + 1) * c) % n));
+  for (int64_t P = 64; P <= 8192; P *= 2) {
+    const int64_t m = 2 * P + 1, w = B + 1;
+    int64_t H = 0;
+    for (int64_t l0 : probes) {
+      // transpose of the window, banded (half width w): MT[j][i] = M[i][j],
+      // M[i][j] = exp(-a (rho l_j - floor((2t l_i + n)/(2n)))^2); solve
+      // MT r = e_P by Gaussian elimination without pivoting on the band
+      std::vector<long double> A(static_cast<size_t>(m * (2 * w + 1)), 0.0L), r(static_cast<size_t>(m), 0.0L);
+      auto at = [&](int64_t row, int64_t col) -> long double& { return A[row * (2 * w + 1) + (col - row + w)]; };
+      for (int64_t i = 0; i < m; ++i) {
+        const int64_t li = l0 - P + i;
+        const int64_t num = 2 * t * li + n, den = 2 * n;
+        const int64_t qi = num >= 0 ? num / den : -((-num + den - 1) / den);
+        for (int64_t j = std::max<int64_t>(0, i - w); j <= std::min<int64_t>(m - 1, i + w); ++j) {
+          const long double dd = rho * static_cast<long double>(l0 - P + j) - static_cast<long double>(qi);
+          at(j, i) = std::exp(-a * dd * dd);
+        }
+      }
+      r[P] = 1.0L;
+      for (int64_t k = 0; k < m; ++k) {
+        const long double piv = at(k, k);
+        if (piv == 0.0L) return 0;
+        for (int64_t i = k + 1; i <= std::min<int64_t>(m - 1, k + w); ++i) {
+          const long double f = at(i, k) / piv;
+          if (f == 0.0L) continue;
+          for (int64_t j = k; j <= std::min<int64_t>(m - 1, k + w); ++j) at(i, j) -= f * at(k, j);
+          r[i] -= f * r[k];
+        }
+      }
+      for (int64_t k = m - 1; k >= 0; --k) {
+        long double sacc = r[k];
+        for (int64_t j = k + 1; j <= std::min<int64_t>(m - 1, k + w); ++j) sacc -= at(k, j) * r[j];
+        r[k] = sacc / at(k, k);
+      }
+      long double mx = 0.0L;
+      for (int64_t j = 0; j < m; ++j) mx = std::max(mx, std::fabs(r[j]));
+      for (int64_t j = 0; j < m; ++j)
+        if (std::fabs(r[j]) > tol * mx) H = std::max<int64_t>(H, j > P ? j - P : P - j);
+    }
+    if (H < P - 2 * B) return static_cast<int64_t>(std::ceil(rho * H)) + B;
+  }
+  return 0;
+}
+
+// The route's parameters for FP32 (hvdh.jl:154-190): alpha = 1, a = pi
+// alpha^2, b = pi/alpha^2, tol = eps/4, h = ceil(alpha sqrt(ln(1/tol)/pi))
+// taps per side, band B = floor(sqrt(ln(1/tol)/a) + 1/2), the probed halo.
+// False when the length has no grid or the kernels' limits do not hold.
+static bool hvdh_params(int dtype, int64_t n, int64_t batch, double scale, HvdhDesc& d) {
+  if (dtype != FFTJH_F32 || n % 2 == 0 || n < 64 || n >= (int64_t(1) << 30)) return false;
+  const int64_t t = hvdh_grid(n);
+  if (!t) return false;
+  const long double alpha = 1.0L, pi = 3.14159265358979323846264338327950288L;
+  const long double a = pi * alpha * alpha, b = pi / (alpha * alpha);
+  const long double tol = std::ldexp(1.0L, -25);
+  const int h = static_cast<int>(std::ceil(alpha * std::sqrt(std::log(1 / tol) / pi)));
+  const int B = static_cast<int>(std::floor(std::sqrt(std::log(1 / tol) / a) + 0.5L));
+  if (B != 2 || h < 1 || h > 7 || 2 * (t - n) >= n) return false;
+  const int64_t H = hvdh_halo(n, t, a, B, tol);
+  if (H < 1 || H > hvdh_max_halo()) return false;
+  d = HvdhDesc{};
+  d.n = n;
+  d.t = t;
+  d.delta = t - n;
+  d.cinv = inv_mod(d.delta, n);
+  for (int o = 1 - h; o <= h; ++o) d.tap[o + h] = (((o - 1) * d.delta) % n + n) % n;
+  d.batch = batch;
+  d.chunks = (t + hvdh_chunk_rows() - 1) / hvdh_chunk_rows();
+  d.h = h;
+  d.B = B;
+  d.H = static_cast<int32_t>(H);
+  d.b = static_cast<double>(b);
+  d.ialpha = static_cast<double>(1 / alpha);
+  for (int o = 0; o <= h; ++o) d.kc[o] = static_cast<double>(std::exp(-b * o * o));
+  d.a = static_cast<float>(a);
+  d.nt = static_cast<float>(static_cast<long double>(n) / t);
+  d.c1 = static_cast<float>(std::exp(-a));
+  d.c2 = static_cast<float>(std::exp(-4 * a));
+  d.scale = static_cast<float>(scale);
+  return true;
+}
+
+// The HvdH route for an odd non-smooth length: the route flag forces it (where
+// its parameters hold), the other route flags exclude it.  By default the
+// planner keeps resampling, measured faster on one MI355X at every length
+// tried (F32 x 4, forward, us: 16,387 49 / 20, 65,537 59 / 34, 100,003 75 /
+// 38, 300,007 100 / 57, 1,000,003 213 / 115): on the GPU the grid transform
+// is cheap, and the route's two permutations and its solve (a sequential
+// recurrence per chunk) cost more than resampling's doubled grid.
+// FFTJH_HVDH=1 makes it the default where it applies.
+static bool use_hvdh(const Layout& L, int64_t n) {
+  if (L.route & FFTJH_FLAG_ROUTE_HVDH) return true;
+  if (L.route & (FFTJH_FLAG_ROUTE_RESAMPLE | FFTJH_FLAG_ROUTE_COMPOSITE)) return false;
+  static const int env = [] {
+    const char* e = std::getenv("FFTJH_HVDH");
+    return e ? std::atoi(e) : 0;
+  }();
+  (void)n;
+  return env != 0;
 }
 
 // Three power-of-two passes run by the lean kernel inside the output array
@@ -1104,6 +1265,51 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
       return;
     }
   }
+  // the HvdH route (hvdh.jl:222-241): x permuted to Delta order, spread onto
+  // grid rows of t, the t-point transform in place, the chunked banded
+  // solves to the unknowns in l order, permuted into y
+  if (use_hvdh(L, n) && is_rows(gin, n, Bi) && is_rows(gout, n, Bi) && free.size() >= 3) {
+    HvdhDesc hd;
+    if (hvdh_params(L.dtype, n, batch, scale, hd)) {
+      const int g = free[0], u = free[1];
+      need_scratch(L, g, hd.t * batch);
+      need_scratch(L, u, n * batch);
+      Step st;
+      st.dir = dir;
+      st.n = n;
+      st.m = hd.t;
+      st.batch = batch;
+      st.hv = hd;
+      Step pin = st;
+      pin.kind = ST_HVDH_PERMUTE;
+      pin.src = src;
+      pin.dst = u;
+      pin.scale = 1.0;
+      L.steps.push_back(pin);
+      Step sp = st;
+      sp.kind = ST_HVDH_SPREAD;
+      sp.src = u;
+      sp.dst = g;
+      L.steps.push_back(sp);
+      build_c2c(L, hd.t, batch, 1, geo(1, 0, hd.t), geo(1, 0, hd.t), dir, 1.0, g, g,
+                std::vector<int>(free.begin() + 2, free.end()));
+      Step so = st;
+      so.kind = ST_HVDH_SOLVE;
+      so.src = g;
+      so.dst = u;
+      L.steps.push_back(so);
+      Step pout = st;
+      pout.kind = ST_HVDH_PERMUTE;
+      pout.src = u;
+      pout.dst = dst;
+      pout.scale = scale;
+      L.steps.push_back(pout);
+      return;
+    }
+    if (L.route & FFTJH_FLAG_ROUTE_HVDH)
+      fail(FFTJH_ERR_UNSUPPORTED, "the HvdH route needs FP32, an odd length with a coprime smooth grid and "
+                                  "contiguous rows (length " + std::to_string(n) + ")");
+  }
   // otherwise resample onto a smooth m (resample.jl:258-276)
   if (free.size() < 2) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for resampling");
   const RsParams rp = rs_params(L.dtype, n);
@@ -1200,12 +1406,15 @@ static void check_dtype(int dtype) {
 }
 
 // split-precision mode from plan flags (at most one FFTJH_FLAG_MFMA_* bit; FP32 only)
-static const unsigned kRouteFlags = FFTJH_FLAG_ROUTE_COMPOSITE | FFTJH_FLAG_ROUTE_RESAMPLE;
+static const unsigned kRouteFlags = FFTJH_FLAG_ROUTE_COMPOSITE | FFTJH_FLAG_ROUTE_RESAMPLE | FFTJH_FLAG_ROUTE_HVDH;
 
 static int mat_mode_of(unsigned flags, int dtype) {
   const unsigned known = FFTJH_FLAG_MFMA_F16 | FFTJH_FLAG_MFMA_BF16 | FFTJH_FLAG_MFMA_BF16X3;
   if (flags & ~(known | kRouteFlags)) fail(FFTJH_ERR_ARGUMENT, "unknown plan flags");
-  if ((flags & kRouteFlags) == kRouteFlags) fail(FFTJH_ERR_ARGUMENT, "at most one route flag");
+  {
+    const unsigned r = flags & kRouteFlags;
+    if (r & (r - 1)) fail(FFTJH_ERR_ARGUMENT, "at most one route flag");
+  }
   flags &= known;
   if (!flags) return kMatNone;
   if (flags & (flags - 1)) fail(FFTJH_ERR_ARGUMENT, "at most one split-precision flag");
@@ -1826,6 +2035,10 @@ static std::string describe(const Layout& L) {
         << " resample=" << R.counts[2] << " resample_two_level=" << R.counts[3] << " tile_lengths=" << R.scheds.size()
         << " radix_pref=" << R.radix_pref << " tile_max=" << R.tile_max;
       if (R.groups) o << " workgroups=" << R.groups;
+    } else if (s.kind == ST_HVDH_PERMUTE || s.kind == ST_HVDH_SPREAD || s.kind == ST_HVDH_SOLVE) {
+      o << " n=" << s.hv.n << " t=" << s.hv.t << " taps=" << 2 * s.hv.h << " band=" << s.hv.B << " halo=" << s.hv.H
+        << " batch=" << s.hv.batch;
+      if (s.kind == ST_HVDH_SOLVE) o << " chunks=" << s.hv.chunks * s.hv.batch << " rows=" << hvdh_chunk_rows();
     } else if (s.kind == ST_RS_DECONV || s.kind == ST_RS_INTERP) {
       o << " n=" << s.n << " m=" << s.m << " w=" << s.w << " beta=" << s.beta;
     } else {
@@ -2129,6 +2342,12 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
                               s.gb, st);
     case ST_LEAN:
       return launch_lean(s.dir, s.ld, ptr[s.src], ptr[s.dst], s.tw->p, st);
+    case ST_HVDH_PERMUTE:
+      return launch_hvdh_permute(s.hv, ptr[s.src], ptr[s.dst], static_cast<float>(s.scale), st);
+    case ST_HVDH_SPREAD:
+      return launch_hvdh_spread(s.hv, ptr[s.src], ptr[s.dst], st);
+    case ST_HVDH_SOLVE:
+      return launch_hvdh_solve(s.hv, ptr[s.src], ptr[s.dst], st);
     case ST_STRIDED_COPY:
       return launch_strided_copy(dt, ptr[s.src], ptr[s.dst], s.n, s.batch, s.Bi, s.ga, s.gb, st);
     case ST_RAGGED: {

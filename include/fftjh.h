@@ -67,6 +67,11 @@ extern "C" {
  *                               (ResamplePlan, resample.jl:258-320) */
 #define FFTJH_FLAG_ROUTE_COMPOSITE 0x10u
 #define FFTJH_FLAG_ROUTE_RESAMPLE 0x20u
+/*   FFTJH_FLAG_ROUTE_HVDH       odd lengths: Gaussian resampling onto a smooth
+ *                               t coprime to n and chunked banded solves (the
+ *                               reference's HvdhPlan, hvdh.jl:154-241); FP32,
+ *                               contiguous rows */
+#define FFTJH_FLAG_ROUTE_HVDH 0x40u
 
 typedef struct fftjh_plan_s* fftjh_plan;
 /* hipStream_t, passed as an opaque pointer (NULL = the null stream) */

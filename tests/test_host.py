@@ -261,3 +261,28 @@ def test_mfma_flags_and_schedules():
         for ln in d.splitlines():
             if "threads=" in ln:
                 assert int(ln.split("threads=")[1].split()[0]) <= 512
+
+
+def test_hvdh_route_schedule():
+    # the reference's HvdhPlan parameters at the configuration primes
+    # (hvdh.jl:154-190): t = 73,728 / 1,048,576, 6 taps, band 2, the probed
+    # halo 9 / 13 rows; permute, spread, the t-point passes, solve, permute
+    L = _lib.lib()
+    import ctypes
+    for n, t, halo in ((65537, 73728, 9), (1000003, 1048576, 13)):
+        buf = ctypes.create_string_buffer(1 << 16)
+        tune = _lib.make_tuning((), (), _lib.FLAG_ROUTE_HVDH)
+        assert L.fftjh_describe_tuned(_lib.F32, 1, _lib.dims_array([n]), 4, -1, ctypes.byref(tune), buf,
+                                      len(buf)) == _lib.OK
+        s = _passes(buf.value.decode())
+        assert s[0].startswith("hvdh_permute x->") and s[1].startswith("hvdh_spread")
+        assert s[-2].startswith("hvdh_solve") and s[-1].startswith("hvdh_permute") and "->y " in s[-1]
+        assert all(f"t={t} taps=6 band=2 halo={halo}" in p for p in (s[0], s[1], s[-2], s[-1]))
+        assert all(p.startswith("pass") for p in s[2:-2])
+    # by default the planner keeps resampling for these lengths
+    assert "rs_interp" in _lib.describe_request(_lib.KIND_C2C, _lib.F32, [1000003], batch=4)
+    # the route needs FP32 and a halo the solve kernel holds
+    tune = _lib.make_tuning((), (), _lib.FLAG_ROUTE_HVDH)
+    buf = ctypes.create_string_buffer(1 << 12)
+    assert L.fftjh_describe_tuned(_lib.F64, 1, _lib.dims_array([65537]), 1, -1, ctypes.byref(tune), buf,
+                                  len(buf)) == _lib.ERR_UNSUPPORTED
