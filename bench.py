@@ -262,7 +262,10 @@ def cpu_configs_leg(F, per_config_s=3.0):
 
 def ragged_leg(F, quick=False):
     """The ragged configuration: 10,000 FP32 transforms, one persistent launch
-    (autotuned) vs the reference's loop of per-length 1-D plans."""
+    (autotuned), against one batched plan per distinct length on the data laid
+    out by length (the per-size baseline) and against the reference's loop of
+    1-D plans, one call per transform; every side also captured in a HIP
+    graph."""
     import numpy as np
     import torch
 
@@ -277,10 +280,18 @@ def ragged_leg(F, quick=False):
     default_ms = F.tune.time_plan(F.plan_ragged(torch.float32, sizes), y, x, iters=20)
     loop = F.ragged.PlainLoop(torch.float32, sizes)
     plain_ms = F.tune.time_plan(loop, y, x, iters=3, warmup=1)
-    # both sides captured in a HIP graph (no host launch cost): the ragged
-    # plan is one node, the plain loop one node per transform
+    # the per-size baseline: one batched plan per distinct length on the
+    # same data laid out by length (no gather counted)
+    per = F.ragged.PerSizeLoop(torch.float32, sizes)
+    xs = per.to_layout(x)
+    ys = torch.empty_like(xs)
+    per_size_ms = F.tune.time_plan(per, ys, xs, iters=5, warmup=1)
+    # every side captured in a HIP graph (no host launch cost): the ragged
+    # plan is one node, the per-size loop one node per distinct length, the
+    # plain loop one node per transform
     graph_ms = {}
-    for name, fn in (("ragged", lambda: plan(y, x)), ("plain_loop", lambda: loop(y, x))):
+    for name, fn in (("ragged", lambda: plan(y, x)), ("per_size", lambda: per(ys, xs)),
+                     ("plain_loop", lambda: loop(y, x))):
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
@@ -291,10 +302,13 @@ def ragged_leg(F, quick=False):
             fn()
         graph_ms[name] = F.tune.time_plan(lambda _y, _x: g.replay(), y, x, iters=5 if name == "plain_loop" else 20)
         del g
+    del per, xs, ys
     bytes_ = 16.0 * float(sizes.sum())
     flops = float((5.0 * sizes * np.log2(np.maximum(sizes, 2))).sum())
     return {"transforms": int(len(sizes)), "distinct_sizes": int(len(np.unique(sizes))),
             "sum_n": int(sizes.sum()), "tuned_ms": round(tuned_ms, 4), "default_ms": round(default_ms, 4),
+            "plain_per_size_ms": round(graph_ms["per_size"], 4), "plain_per_size_eager_ms": round(per_size_ms, 4),
+            "speedup_vs_per_size": round(graph_ms["per_size"] / graph_ms["ragged"], 2),
             "plain_loop_ms": round(plain_ms, 3), "speedup_tuned_vs_plain": round(plain_ms / tuned_ms, 2),
             "graph_ragged_ms": round(graph_ms["ragged"], 4), "graph_plain_loop_ms": round(graph_ms["plain_loop"], 3),
             "speedup_in_graphs": round(graph_ms["plain_loop"] / graph_ms["ragged"], 2),

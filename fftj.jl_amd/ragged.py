@@ -3,8 +3,9 @@ one buffer, run by one persistent-kernel launch (libfftjh's ragged plan),
 plus the seeded batch of the ragged benchmark configuration.
 
 The reference has no ragged (or batch) API; its callers loop over 1-D plans
-(plan.jl:207-213).  plain_loop() is that loop on the GPU plans, the baseline
-the ragged plan is measured against.
+(plan.jl:207-213).  PlainLoop is that loop on the GPU plans; PerSizeLoop, one
+batched plan per distinct length on data laid out by length, is the stronger
+plain baseline the ragged plan is also measured against.
 """
 import ctypes
 
@@ -88,6 +89,45 @@ class PlainLoop:
     def __call__(self, y, x):
         for n, o in zip(self.sizes.tolist(), self.offsets.tolist()):
             self.plans[n](y[o:o + n], x[o:o + n])
+        return y
+
+
+class PerSizeLoop:
+    """The strongest plain baseline: one batched plan per distinct length
+    (plan_fft(T, n, batch=count), the engine's own batch API), on the data
+    laid out by length -- each length's transforms back to back -- so every
+    launch is a plain batched transform with no gather.  `order[i]` is the
+    index, in the ragged layout, of the i-th transform of the by-length
+    layout; to_layout / from_layout move data between the two."""
+
+    def __init__(self, T, sizes, sign=-1):
+        from .api import plan_fft, plan_ifft
+        self.sizes = np.asarray(sizes, dtype=np.int64)
+        self.order = np.argsort(self.sizes, kind="stable")
+        by_len = self.sizes[self.order]
+        uniq, first, counts = np.unique(by_len, return_index=True, return_counts=True)
+        offs = np.concatenate([[0], np.cumsum(by_len)[:-1]]).astype(np.int64)
+        self.runs = [(int(n), int(offs[f]), int(c)) for n, f, c in zip(uniq, first, counts)]
+        mk = plan_fft if sign < 0 else plan_ifft
+        self.plans = {n: mk(T, n, batch=c) for n, _, c in self.runs}
+        ragged_offs = np.concatenate([[0], np.cumsum(self.sizes)[:-1]]).astype(np.int64)
+        # element gather index: by-length position -> ragged position
+        self._gather = torch.from_numpy(np.concatenate(
+            [np.arange(ragged_offs[i], ragged_offs[i] + self.sizes[i]) for i in self.order]))
+
+    def to_layout(self, x):
+        """The ragged-layout tensor x in the by-length layout."""
+        return x[self._gather.to(x.device)]
+
+    def from_layout(self, z):
+        """The by-length-layout tensor z back in the ragged layout."""
+        out = torch.empty_like(z)
+        out[self._gather.to(z.device)] = z
+        return out
+
+    def __call__(self, y, x):
+        for n, o, c in self.runs:
+            self.plans[n](y[o:o + n * c], x[o:o + n * c])
         return y
 
 

@@ -107,3 +107,20 @@ def test_ragged_autotune_seeded_by_cost_model():
     for i in list(range(0, len(sizes), 97)) + [big]:
         n, o = int(sizes[i]), int(offs[i])
         assert rel_l2(y[o:o + n], O.fft(x[o:o + n].astype(np.complex128))) <= tol_for("f32", n), (n, cfg)
+
+
+def test_per_size_baseline_matches_ragged():
+    # the per-size baseline (one batched plan per distinct length, data laid
+    # out by length) computes the same transforms as the ragged plan
+    sizes = F.ragged.benchmark_sizes(count=300, prime_count=15, seed=3)
+    x = to_dev(inputs_c(8, int(sizes.sum()), "f32"))
+    yr = to_host(F.plan_ragged(torch.float32, sizes)(x))
+    per = F.ragged.PerSizeLoop(torch.float32, sizes)
+    xs = per.to_layout(x)
+    ys = torch.empty_like(xs)
+    per(ys, xs)
+    yp = to_host(per.from_layout(ys))
+    assert len(per.runs) == len(np.unique(sizes))
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    for n, o in zip(sizes.tolist(), offs.tolist()):
+        assert rel_l2(yp[o:o + n], yr[o:o + n]) <= 2 * tol_for("f32", n), n
