@@ -462,7 +462,7 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
       s.radices.assign(pr, pr + pc);
     }
   }
-  const int chainE = s.pow2 ? chain_elems(n) : 16;
+  int chainE = s.pow2 ? chain_elems(n) : 16;
   // transforms per workgroup: strided passes take adjacent columns for
   // contiguous HBM runs; row passes group short transforms to fill a workgroup
   const bool colmode = in.stride > 1 || out.stride > 1;
@@ -482,7 +482,7 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   };
   // composite radices: fewer stages, taken where the chain still fits as
   // many transforms per workgroup (FFTJH_COMPOSITE_RADICES=0 turns it off)
-  {
+  auto apply_compact = [&] {
     static const bool comp_env = [] {
       const char* e = std::getenv("FFTJH_COMPOSITE_RADICES");
       return !e || std::atoi(e) != 0;
@@ -494,7 +494,16 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
         if (c_new > 0 && c_new >= c_old) s.radices = rc;
       }
     }
-  }
+  };
+  apply_compact();
+  // a compile-time chain that no group size fits: the generic kernel, with
+  // its own elements per thread and radix choice
+  auto to_generic = [&] {
+    s.pow2 = false;
+    s.radices = default_radices(n);
+    chainE = 16;
+    apply_compact();
+  };
   auto fits = [&](int C, int* t) {
     if (L.mat) return pass_fits_mat(n, C, s.radices, t);
     return s.pow2 ? pow2_fits(L.dtype, n, C, t) : pass_fits(L.dtype, n, C, s.radices, t);
@@ -507,8 +516,14 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
   } else if (pt && pt->C > 0) {
     int t;
     if ((pt->C & (pt->C - 1)) || pt->C > 64) fail(FFTJH_ERR_ARGUMENT, "tuning C must be a power of two <= 64");
-    if (!fits(pt->C, &t))
-      fail(FFTJH_ERR_ARGUMENT, "tuning C=" + std::to_string(pt->C) + " does not fit one workgroup");
+    // (a chain that fits only other group sizes -- the 81 chain runs three
+    // threads per transform and needs C = 64 -- gives way to the generic kernel)
+    bool ok = fits(pt->C, &t);
+    if (!ok && s.pow2 && !(pt->nradix > 0)) {
+      to_generic();
+      ok = fits(pt->C, &t);
+    }
+    if (!ok) fail(FFTJH_ERR_ARGUMENT, "tuning C=" + std::to_string(pt->C) + " does not fit one workgroup");
     bestC = pt->C;
     bestT = t;
   } else {
@@ -517,8 +532,7 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
     for (int attempt = 0; attempt < 2 && !bestC; ++attempt) {
     if (attempt == 1) {
       if (!s.pow2) break;
-      s.pow2 = false;  // no group size fits the power-of-two kernel: generic chain
-      s.radices = default_radices(n);
+      to_generic();  // no group size fits the compile-time chain
     }
     if (!colmode && s.pow2) {
       // power-of-two row passes: the fewest rows per workgroup that fill a

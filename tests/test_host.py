@@ -130,6 +130,23 @@ def test_fixed_chain_schedule():
     assert len(s) == 3 and not any("kernel=chain" in p for p in s)
 
 
+def test_fixed_chain_gives_way_to_generic():
+    # the 81 = 9 x 9 chain runs three threads per transform, so only 64
+    # columns fill its waves; 64 columns take it, 40 fall back to the generic
+    # kernel with its own group search (16 columns) and compact radices 9,9
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [64, 81]))
+    assert "n=81 radices=9,9 kernel=chain" in s[1] and "C=64 " in s[1]
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [40, 81]))
+    assert "n=81 radices=9,9" in s[1] and "kernel=chain" not in s[1] and "C=16 " in s[1]
+    # a tuned group size the chain cannot take runs the generic kernel
+    # instead of refusing the plan; one that fits no kernel is still refused
+    t = _lib.make_tuning((81, 6561), [{}, {"C": 4}])
+    s = _passes(_lib.describe_tuned(_lib.F32, [3**12], batch=64, tuning=t))
+    assert "n=81 radices=9,9" in s[1] and "kernel=chain" not in s[1] and "C=4 " in s[1]
+    with pytest.raises(_lib.ArgumentError):
+        _lib.describe_tuned(_lib.F32, [3**12], batch=64, tuning=_lib.make_tuning((81, 6561), [{"C": 4}, {}]))
+
+
 def test_nd_long_and_nonsmooth_axes():
     # a long strided axis runs the multi-pass schedule on its column block
     # directly (offsets scaled by the column count), no gather / scatter
