@@ -29,6 +29,8 @@ WORKLOADS = {
     "c2c_f32_2pow24": (1 << 24, 1, "C2C FP32 N=2^24 batch 1, three 256-point passes, forward+inverse"),
     "c2c_f32_4096x16384": (4096, 16384, "C2C FP32 N=4096 batch 16384, forward+inverse"),
     "c2c_f32_1e6x8": (10**6, 8, "C2C FP32 N=10^6 batch 8, forward+inverse"),
+    "c2c_f32_4x3pow12x64": (4 * 3**12, 64, "C2C FP32 N=4*3^12 batch 64, two 81-point column passes and a "
+                                            "324-point row pass, forward+inverse"),
 }
 
 

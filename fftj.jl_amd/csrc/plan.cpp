@@ -684,6 +684,47 @@ static std::vector<int64_t> choose_factors(int dtype, int64_t n, int64_t max_fac
   return {};
 }
 
+// The P-part split with every column factor F[1..P-1] on the 81 = 9 x 9 or
+// 100 = 10 x 10 compile-time chain, at least one of them 81, and the row
+// factor F[0] a single pass of at most 512 points, most balanced first;
+// empty when there is none.  Out of the Infinity Cache (FP32, 256-512 MiB
+// per array) the 81 chain's strided passes run at about twice the rate of
+// the runtime chain's (4*3^12 x 64: 0.51 ms per 81-point column pass against
+// 1.01-1.09 ms per 162-point one), and every such split measured was
+// faster than the balanced one (52 lengths x 2 batches, 1.01x - 1.57x); with
+// power-of-two or 100-only columns the result was mixed (0.75x - 1.35x), and
+// in cache (32 MiB) mixed too (0.88x - 1.21x).
+static std::vector<int64_t> chain_column_factors(int dtype, int64_t n, int parts) {
+  std::vector<int64_t> best, cur;
+  double best_score = 1e300;
+  std::function<void(int64_t, int)> rec = [&](int64_t m, int left) {
+    if (left == 0) {
+      if (m > 512 || !single_pass_ok(dtype, m)) return;
+      if (std::find(cur.begin(), cur.end(), int64_t(81)) == cur.end()) return;
+      double lo = static_cast<double>(m), hi = lo;
+      for (int64_t f : cur) {
+        lo = std::min(lo, static_cast<double>(f));
+        hi = std::max(hi, static_cast<double>(f));
+      }
+      const double score = std::log(hi / lo);
+      if (score < best_score - 1e-12) {
+        best_score = score;
+        best.assign(1, m);
+        best.insert(best.end(), cur.begin(), cur.end());
+      }
+      return;
+    }
+    for (int64_t f : {int64_t(81), int64_t(100)}) {
+      if (m % f) continue;
+      cur.push_back(f);
+      rec(m / f, left - 1);
+      cur.pop_back();
+    }
+  };
+  rec(n, parts - 1);
+  return best;
+}
+
 // largest factor the default multi-pass planner uses (the tile of one pass)
 static int64_t default_max_factor(int dtype) {
   static const int64_t env = [] {
@@ -1182,6 +1223,23 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
         factor_search(L.dtype, n, 2, 512, cur, F, score);
       }
       if (F.empty()) F = choose_factors(L.dtype, n, default_max_factor(L.dtype));
+      // FP32 plans of 128 MiB and more per array: column passes on the 81
+      // and 100 chains where the balanced split leaves one to the runtime
+      // chain (FFTJH_CHAIN_COLUMNS=0: off)
+      static const bool chain_cols = [] {
+        const char* e = std::getenv("FFTJH_CHAIN_COLUMNS");
+        return !e || std::atoi(e) != 0;
+      }();
+      bool generic_col = false;
+      for (size_t i = 1; i < F.size(); ++i) {
+        int pr[8], pc = 0, pe = 0;
+        generic_col |= !fixed_chain(static_cast<int>(F[i]), pr, &pc, &pe);
+      }
+      const bool big = n * batch * esize(L.dtype) >= (int64_t(128) << 20);
+      if (chain_cols && !mf_env && L.dtype == FFTJH_F32 && !L.mat && !colblock && big && generic_col) {
+        std::vector<int64_t> alt = chain_column_factors(L.dtype, n, static_cast<int>(F.size()));
+        if (!alt.empty()) F = alt;
+      }
     }
     if (F.size() < 2) fail(FFTJH_ERR_UNSUPPORTED, "no multi-pass split of length " + std::to_string(n));
     if (free.empty()) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for a multi-pass plan");

@@ -110,3 +110,21 @@ def test_fixed_chain_fallbacks():
     assert "C=4 " in p.describe()
     y = to_host(p(to_dev(x))).reshape(batch, n)
     assert rel_l2(y, O.fft(x.astype(np.complex128))) <= tol_for("f32", n)
+
+
+def test_chain_column_split_out_of_cache():
+    # 4*3^12 x 16 (272 MB per array): the default split runs both column
+    # passes on the 81-point chain and the 324-point pass on the rows
+    n, batch = 4 * 3**12, 16
+    print(f"c2c f32 n={n} batch={batch} chain columns", flush=True)
+    x = inputs_c(3, n * batch, "f32").reshape(batch, n)
+    xd = to_dev(x)
+    p = F.plan_fft(torch.float32, n, batch=batch)
+    d = p.describe()
+    assert d.count("n=81 radices=9,9 kernel=chain") == 2 and "n=324 " in d
+    y = p(xd)
+    rows = [0, 7, batch - 1]
+    yh = to_host(y).reshape(batch, n)
+    assert rel_l2(yh[rows], O.fft(x[rows].astype(np.complex128))) <= tol_for("f32", n)
+    back = to_host(F.plan_ifft(torch.float32, n, batch=batch)(y)).reshape(batch, n)
+    assert rel_l2(back, x) <= 2 * tol_for("f32", n)

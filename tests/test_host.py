@@ -130,6 +130,21 @@ def test_fixed_chain_schedule():
     assert len(s) == 3 and not any("kernel=chain" in p for p in s)
 
 
+def test_chain_column_split():
+    # out of the Infinity Cache (128 MiB per array and more), 3-smooth-heavy
+    # lengths put their strided passes on the 81-point chain; in cache, or
+    # where no such split exists, the balanced split stays
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [4 * 3**12], batch=64))
+    assert [p.split(" n=")[1].split()[0] for p in s] == ["81", "81", "324"]
+    assert "kernel=chain" in s[0] and "kernel=chain" in s[1] and "in_stride=1 " in s[2]
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [4 * 3**12], batch=2))
+    assert [p.split(" n=")[1].split()[0] for p in s] == ["162", "162", "81"]
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F64, [4 * 3**12], batch=64))
+    assert "n=324 " not in " ".join(s)
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1 << 24], batch=8))
+    assert all("n=81 " not in p for p in s)
+
+
 def test_fixed_chain_gives_way_to_generic():
     # the 81 = 9 x 9 chain runs three threads per transform, so only 64
     # columns fill its waves; 64 columns take it, 40 fall back to the generic
