@@ -50,6 +50,8 @@ def parse():
                     help="measure the device's streaming copy rate at the 2^24 footprint and exit")
     ap.add_argument("--split-only", action="store_true",
                     help="run only the split-precision leg (one JSON line; for kernel traces)")
+    ap.add_argument("--planner-only", action="store_true",
+                    help="run only the learned-planner leg (one JSON line)")
     ap.add_argument("--configs-only", action="store_true",
                     help="run only the configurations leg (one JSON line; for kernel traces)")
     ap.add_argument("--cpu-configs", action="store_true",
@@ -415,8 +417,10 @@ def configs_leg(F):
 
 
 def planner_leg(F):
-    """Held-out lengths of the learned planner's training set: the default
-    schedule vs the one the trained cost model picks, timed now."""
+    """Held-out lengths of the learned planner's training set, at both
+    training footprints (32 MiB and 256 MiB of complex FP32 per plan): the
+    default schedule vs the learned pick, timed now (a pick that is the
+    default counts as equal)."""
     import numpy as np
     import torch
 
@@ -426,17 +430,21 @@ def planner_leg(F):
     lp = P.LearnedPlanner()
     rows = []
     for n in lengths:
-        batch = P.batch_for(n)
-        x = torch.randn(n * batch, dtype=torch.complex64, device="cuda:0")
-        y = torch.empty_like(x)
-        t_def = P.time_plan(P.make_plan(n, batch, {}), y, x) * 1e3
-        p, cand, _ = lp.choose(n, batch)
-        t_l = P.time_plan(p, y, x) * 1e3
-        rows.append((n, t_def, t_l))
-    r = np.array(rows, dtype=np.float64)
-    return {"held_out_lengths": len(rows), "geomean_speedup_learned_over_default":
-            round(float(np.exp(np.mean(np.log(r[:, 1] / r[:, 2])))), 4),
-            "cases": [[int(a), round(b, 2), round(c, 2)] for a, b, c in rows]}
+        for fp in P.FOOTPRINTS:
+            batch = P.batch_for(n, fp)
+            x = torch.randn(n * batch, dtype=torch.complex64, device="cuda:0")
+            y = torch.empty_like(x)
+            t_def = P.time_plan(P.make_plan(n, batch, {}), y, x) * 1e3
+            p, cand, _ = lp.choose(n, batch)
+            t_l = P.time_plan(p, y, x) * 1e3 if cand else t_def
+            rows.append((n, batch, t_def, t_l))
+            del p, x, y
+    r = np.array(rows, dtype=np.float64)[:, 1:]
+    return {"held_out_cases": len(rows), "margin": lp.model.margin,
+            "geomean_speedup_learned_over_default": round(float(np.exp(np.mean(np.log(r[:, 1] / r[:, 2])))), 4),
+            "worst_learned_over_default": round(float(np.max(r[:, 2] / r[:, 1])), 4),
+            "cases_learned_5pct_slower": int(np.sum(r[:, 2] > 1.05 * r[:, 1])),
+            "cases": [[int(a), int(b), round(c, 2), round(e, 2)] for a, b, c, e in rows]}
 
 
 def main():
@@ -465,6 +473,9 @@ def main():
         return
     if args.configs_only:
         print(json.dumps({"configs": configs_leg(F)}), flush=True)
+        return
+    if args.planner_only:
+        print(json.dumps({"learned_planner": planner_leg(F)}), flush=True)
         return
     n, batch, desc = WORKLOADS[args.workload]
     dev = torch.device("cuda:0")

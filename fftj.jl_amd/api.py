@@ -426,20 +426,31 @@ def _plan_c2c(T_or_x, n, sign, scale_kind, region, batch, planner=None, precisio
 
 
 def _planner_tuning(planner, code, n, batch, sign):
-    """Search-space choice for a 1-D complex plan: None/"heuristic" is the
-    library's own planner; "learned" ranks the candidate schedules with the
-    trained cost model (FP32 plans; FP64 keeps the heuristic); "exhaustive"
-    times every candidate on the current stream and keeps the fastest."""
-    if planner in (None, "heuristic"):
+    """Search-space choice for a 1-D complex plan.  None: the learned
+    planner's pick where it is confident (FP32, 13-smooth lengths 1024..2^24
+    at n*batch near its training cases, a committed model) and the pick is
+    predicted faster than the library's own schedule by the model's
+    calibrated margin, else the library's own schedule; "heuristic": always
+    the library's own schedule; "learned": the learned pick for any FP32
+    length (FP64 keeps the heuristic); "exhaustive": times every candidate
+    on the current stream and keeps the fastest."""
+    if planner == "heuristic":
         return None
+    if planner not in (None, "learned", "exhaustive"):
+        raise ArgumentError(f"planner must be None, 'heuristic', 'learned' or 'exhaustive' (got {planner!r})")
     from . import planner as _pl
-    if planner == "learned":
-        if code != _lib.F32:
-            return None
-        return _pl._tuning(_pl.default_learned().rank(n, batch, sign)[0][0])
     if planner == "exhaustive":
         return _pl._tuning(_pl.exhaustive_choose(n, batch, sign, dtype=code)[1])
-    raise ArgumentError(f"planner must be None, 'heuristic', 'learned' or 'exhaustive' (got {planner!r})")
+    if code != _lib.F32:
+        return None
+    if planner is None:
+        if not _pl.model_available():
+            return None
+        lp = _pl.default_learned()
+        if not lp.confident(n, batch):
+            return None
+        return _pl._tuning(lp.pick_cached(n, batch, sign))
+    return _pl._tuning(_pl.default_learned().pick(n, batch, sign)[0])
 
 
 def plan_fft(T_or_x, n=None, region=None, *, batch=1, nthreads=None, planner=None, precision=None,

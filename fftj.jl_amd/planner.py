@@ -45,7 +45,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DATA_DIR = os.path.join(HERE, "planner_data")
 DATASET = "c2c_f32_mi355x"
 MAX_PASSES = 4
-PASS_FEATURES = 11
+PASS_FEATURES = 19
 N_FEATURES = 6 + MAX_PASSES * PASS_FEATURES
 HIDDEN = 64
 _ONE_PASS_MAX = 16384
@@ -109,8 +109,17 @@ def radix_chain(n, top):
     return pw + rad if n == 1 else None
 
 
+def _chain_factor(f):
+    """A pass length with a compile-time chain kernel (powers of two, 81, 100)."""
+    return f in (81, 100) or (f & (f - 1)) == 0
+
+
 def candidates(n, max_per_p=4):
-    """Tuning dicts (splits, passes) for length n; {} is the default plan."""
+    """Tuning dicts (splits, passes) for length n; {} is the default plan.
+    Multi-pass lengths get the most balanced factorizations of each pass
+    count (also with a 4-column first pass) and the most balanced ones whose
+    column factors (all but the first, the row pass) run compile-time
+    chains."""
     cands = [{}]
     if n <= _ONE_PASS_MAX:
         for C in (1, 2, 4, 8):
@@ -127,6 +136,8 @@ def candidates(n, max_per_p=4):
         for f in fl[:max_per_p]:
             cands.append({"splits": f})
             cands.append({"splits": f, "passes": [{"C": 4}]})
+        for f in [f for f in fl if f[0] <= 1024 and all(_chain_factor(c) for c in f[1:])][:2]:
+            cands.append({"splits": f})
     seen, out = set(), []
     for c in cands:
         k = repr(c)
@@ -183,11 +194,13 @@ def features(desc, n, batch, dtype=0):
     f[5] = sum(1 for ln in lines if not ln.startswith("pass")) / 4.0
     passes = [ln for ln in lines if ln.startswith("pass")][:MAX_PASSES]
     for k, ln in enumerate(passes):
-        kv = dict(tok.split("=", 1) for tok in ln.split() if "=" in tok)
+        toks = ln.split()
+        kv = dict(tok.split("=", 1) for tok in toks if "=" in tok)
+        kernels = {tok.split("=", 1)[1] for tok in toks if tok.startswith("kernel=")}
         base = 6 + k * PASS_FEATURES
         rad = [int(r) for r in kv.get("radices", "1").split(",") if r]
         f[base + 0] = 1.0
-        f[base + 1] = 1.0 if kv.get("kernel") == "pow2" else 0.0
+        f[base + 1] = 1.0 if "pow2" in kernels else 0.0
         f[base + 2] = _lg(int(kv.get("n", 1)))
         f[base + 3] = _lg(int(kv.get("C", 1))) * 4.0
         f[base + 4] = _lg(int(kv.get("threads", 64))) * 2.0
@@ -197,6 +210,21 @@ def features(desc, n, batch, dtype=0):
         f[base + 8] = len(rad) / 8.0
         f[base + 9] = sum(1 for r in rad if r < 8 or r % 2) / 8.0
         f[base + 10] = 1.0 if "twiddle" in kv else 0.0
+        f[base + 11] = 1.0 if "chain" in kernels else 0.0
+        f[base + 12] = 1.0 if "lean" in kernels else 0.0
+        f[base + 13] = 1.0 if "fused" in kernels else 0.0
+        st = kv.get("stream", "")
+        f[base + 14] = 1.0 if st in ("in", "inout") else 0.0
+        f[base + 15] = 1.0 if st in ("out", "inout") else 0.0
+        f[base + 16] = 1.0 if "rows_out" in toks else 0.0
+        # occupancy: waves per group, and the share of a CU's 16 wave slots
+        # (4 per SIMD at the kernels' register budgets) that whole groups
+        # fill, also bounded by a group's LDS tile
+        waves = max(1, int(kv.get("threads", 64)) // 64)
+        tile = int(kv.get("C", 1)) * int(kv.get("n", 1)) * 8 * 17 // 16
+        groups = min(16 // waves if waves <= 16 else 1, max(1, (160 << 10) // max(tile, 1)))
+        f[base + 17] = waves / 16.0
+        f[base + 18] = min(1.0, groups * waves / 16.0)
     return f
 
 
@@ -238,34 +266,42 @@ def training_sizes(count, lo=1 << 10, hi=1 << 24, seed=7):
     return out
 
 
-def collect(sizes, log=None):
-    """Measure every candidate plan of every size (the exhaustive planner's
-    work); returns a dict of arrays, one row per (size, candidate)."""
-    feats, logt, size_id, cand_id, is_def, ns, bs = [], [], [], [], [], [], []
+FOOTPRINTS = (1 << 22, 1 << 25)
+
+
+def collect(sizes, footprints=FOOTPRINTS, log=None):
+    """Measure every candidate plan of every size at each footprint (complex
+    elements per plan: 32 MiB, inside the Infinity Cache, and 256 MiB, out of
+    it), the exhaustive planner's work; returns a dict of arrays, one row per
+    (size, footprint, candidate).  size_id numbers the lengths, case_id the
+    (length, batch) pairs."""
+    feats, logt, size_id, case_id, cand_id, is_def, ns, bs = [], [], [], [], [], [], [], []
+    case = 0
     for si, n in enumerate(sizes):
-        batch = batch_for(n)
-        x = torch.randn(n * batch, dtype=torch.complex64, device="cuda")
-        y = torch.empty_like(x)
-        got = 0
-        for ci, (cand, _) in enumerate(valid_candidates(n, batch)):
-            p = make_plan(n, batch, cand)
-            ms = time_plan(p, y, x)
-            feats.append(features(p.describe(), n, batch))
-            logt.append(math.log(ms * 1e3))  # log microseconds
-            size_id.append(si)
-            cand_id.append(ci)
-            is_def.append(1 if not cand else 0)
-            ns.append(n)
-            bs.append(batch)
-            got += 1
-            del p
-        del x, y
-        if log and (si % 20 == 0 or si == len(sizes) - 1):
-            log(f"size {si + 1}/{len(sizes)} n={n} batch={batch}: {got} candidates, {len(logt)} timings so far")
+        for fp in footprints:
+            batch = batch_for(n, fp)
+            x = torch.randn(n * batch, dtype=torch.complex64, device="cuda")
+            y = torch.empty_like(x)
+            for ci, (cand, desc) in enumerate(valid_candidates(n, batch)):
+                p = make_plan(n, batch, cand)
+                ms = time_plan(p, y, x, repeats=5)
+                feats.append(features(desc, n, batch))
+                logt.append(math.log(ms * 1e3))  # log microseconds
+                size_id.append(si)
+                case_id.append(case)
+                cand_id.append(ci)
+                is_def.append(1 if not cand else 0)
+                ns.append(n)
+                bs.append(batch)
+                del p
+            case += 1
+            del x, y
+        if log and (si % 10 == 0 or si == len(sizes) - 1):
+            log(f"size {si + 1}/{len(sizes)} n={n}: {len(logt)} timings so far")
     return {"features": np.array(feats, dtype=np.float32), "log_us": np.array(logt, dtype=np.float32),
-            "size_id": np.array(size_id, dtype=np.int32), "cand_id": np.array(cand_id, dtype=np.int32),
-            "is_default": np.array(is_def, dtype=np.int8), "n": np.array(ns, dtype=np.int64),
-            "batch": np.array(bs, dtype=np.int64)}
+            "size_id": np.array(size_id, dtype=np.int32), "case_id": np.array(case_id, dtype=np.int32),
+            "cand_id": np.array(cand_id, dtype=np.int32), "is_default": np.array(is_def, dtype=np.int8),
+            "n": np.array(ns, dtype=np.int64), "batch": np.array(bs, dtype=np.int64)}
 
 
 # --------------------------------------------------------------- the model
@@ -297,6 +333,8 @@ class CostModel:
         self.mu = np.zeros(in_features, np.float32)
         self.sd = np.ones(in_features, np.float32)
         self.log_range = (-np.inf, np.inf)  # training targets' range (log us)
+        self.margin = 0.0  # predicted gain a non-default pick must show (calibrated)
+        self.elems_range = (0, 0)  # n*batch range of the training cases
         self.set_params(init_params(in_features, hidden, seed))
 
     def __del__(self):
@@ -362,12 +400,20 @@ class CostModel:
                                             ctypes.byref(loss) if want_loss else None, self._stream()))
         return loss.value if want_loss else None
 
-    def fit(self, X, t, steps=6000, batch=256, lr=3e-3, seed=0, log=None):
-        """Adam on seeded mini-batches (HIP kernels); returns the loss curve
-        sampled every 100 steps.  Sets the feature standardization first."""
+    @staticmethod
+    def standardization(X):
+        """Feature means and spreads; the spread is floored at 0.05 (the
+        features are O(1) by construction), so a feature that hardly varies
+        in training cannot be amplified more than 20-fold at inference."""
         X = np.asarray(X, np.float32)
-        self.mu = X.mean(0).astype(np.float32)
-        self.sd = (X.std(0) + 1e-3).astype(np.float32)
+        return X.mean(0).astype(np.float32), np.maximum(X.std(0), 0.05).astype(np.float32)
+
+    def fit(self, X, t, steps=6000, batch=256, lr=3e-3, seed=0, log=None, norm=None):
+        """Adam on seeded mini-batches (HIP kernels); returns the loss curve
+        sampled every 100 steps.  Sets the feature standardization first
+        (from X, or `norm` = (mean, spread))."""
+        X = np.asarray(X, np.float32)
+        self.mu, self.sd = norm if norm is not None else self.standardization(X)
         tt = np.asarray(t, np.float64)
         self.log_range = (float(tt.min()), float(tt.max()))
         Xd = torch.from_numpy(self.normalize(X)).to("cuda")
@@ -390,7 +436,8 @@ class CostModel:
     def save(self, path):
         np.savez(path, params=self.get_params(), mu=self.mu, sd=self.sd,
                  shape=np.array([self.in_features, self.hidden], np.int64),
-                 log_range=np.array(self.log_range, np.float64))
+                 log_range=np.array(self.log_range, np.float64), margin=np.array(self.margin, np.float64),
+                 elems_range=np.array(self.elems_range, np.int64))
 
     @classmethod
     def load(cls, path):
@@ -400,7 +447,60 @@ class CostModel:
             m.mu, m.sd = z["mu"].astype(np.float32), z["sd"].astype(np.float32)
             if "log_range" in z.files:
                 m.log_range = tuple(float(v) for v in z["log_range"])
+            if "margin" in z.files:
+                m.margin = float(z["margin"])
+            if "elems_range" in z.files:
+                m.elems_range = tuple(int(v) for v in z["elems_range"])
         return m
+
+
+class Ensemble:
+    """K cost models trained on bootstrap resamples (by length) of the
+    training cases; predicts their mean.  Carries the margin a non-default
+    pick must be predicted to gain and the n*batch range of the training
+    cases."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        self.margin = 0.0
+        self.elems_range = (0, 0)
+        self.feature_range = None  # (lo, hi) per feature over the training rows
+        lo = min(m.log_range[0] for m in self.members)
+        hi = max(m.log_range[1] for m in self.members)
+        self.log_range = (lo, hi)
+
+    def predict(self, X):
+        return np.mean([m.predict(X) for m in self.members], axis=0)
+
+    def save(self, path):
+        z = {"members": np.array(len(self.members), np.int64), "margin": np.array(self.margin, np.float64),
+             "elems_range": np.array(self.elems_range, np.int64),
+             "shape": np.array([self.members[0].in_features, self.members[0].hidden], np.int64)}
+        if self.feature_range is not None:
+            z["feature_lo"], z["feature_hi"] = self.feature_range
+        for k, m in enumerate(self.members):
+            z[f"m{k}_params"] = m.get_params()
+            z[f"m{k}_mu"], z[f"m{k}_sd"] = m.mu, m.sd
+            z[f"m{k}_log_range"] = np.array(m.log_range, np.float64)
+        np.savez(path, **z)
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as z:
+            inn, hid = int(z["shape"][0]), int(z["shape"][1])
+            members = []
+            for k in range(int(z["members"])):
+                m = CostModel(inn, hid)
+                m.set_params(z[f"m{k}_params"])
+                m.mu, m.sd = z[f"m{k}_mu"].astype(np.float32), z[f"m{k}_sd"].astype(np.float32)
+                m.log_range = tuple(float(v) for v in z[f"m{k}_log_range"])
+                members.append(m)
+            e = cls(members)
+            e.margin = float(z["margin"])
+            e.elems_range = tuple(int(v) for v in z["elems_range"])
+            if "feature_lo" in z.files:
+                e.feature_range = (z["feature_lo"].astype(np.float32), z["feature_hi"].astype(np.float32))
+        return e
 
 
 # ----------------------------------------------------------------- planners
@@ -411,15 +511,49 @@ def held_out_mask(data, every=5):
     return np.isin(data["size_id"], sids[::every])
 
 
-def regret_report(data, pred, mask):
-    """Over the sizes in `mask`: measured time of the learned pick and of the
-    default plan, relative to the exhaustive planner's best."""
+def _cases(data, mask):
+    key = data["case_id"] if "case_id" in data else data["size_id"]
+    for cid in np.unique(key[mask]):
+        yield key == cid
+
+
+def in_range(X, ranges):
+    """Rows of X whose every feature lies within the training range
+    (lo, hi); all rows when `ranges` is None."""
+    X = np.atleast_2d(np.asarray(X, np.float32))
+    if ranges is None:
+        return np.ones(len(X), bool)
+    lo, hi = ranges
+    return np.all((X >= lo - 1e-6) & (X <= hi + 1e-6), axis=1)
+
+
+def pick_index(pred, is_default, margin=0.0, eligible=None):
+    """Row the learned planner takes among one case's candidates: the
+    predicted fastest of the eligible rows (features inside the training
+    range), unless its predicted time is not below the default's by the
+    factor (1 - margin), or the default itself lies outside the training
+    range -- then the default."""
+    d = np.flatnonzero(is_default == 1)
+    ok = np.ones(len(pred), bool) if eligible is None else np.asarray(eligible, bool)
+    if d.size and not ok[d[0]]:
+        return int(d[0])
+    masked = np.where(ok, pred, np.inf)
+    k = int(np.argmin(masked))
+    if d.size and margin > 0 and not pred[k] < pred[d[0]] + math.log1p(-margin):
+        return int(d[0])
+    return k
+
+
+def regret_report(data, pred, mask, margin=0.0, ranges=None):
+    """Over the cases (length, batch) in `mask`: measured time of the
+    learned pick and of the default plan, relative to the exhaustive
+    planner's best."""
     rows = []
-    for sid in np.unique(data["size_id"][mask]):
-        m = data["size_id"] == sid
+    elig = in_range(data["features"], ranges) if ranges is not None else np.ones(len(pred), bool)
+    for m in _cases(data, mask):
         t = np.exp(data["log_us"][m].astype(np.float64))
         best = t.min()
-        learned = t[int(np.argmin(pred[m]))]
+        learned = t[pick_index(pred[m], data["is_default"][m], margin, elig[m])]
         d = data["is_default"][m] == 1
         default = t[d][0] if d.any() else t.max()
         rows.append((int(data["n"][m][0]), best, learned, default, int(m.sum())))
@@ -428,6 +562,7 @@ def regret_report(data, pred, mask):
     reg_d = r[:, 3] / r[:, 1] - 1.0
     return {
         "sizes": int(len(r)),
+        "margin": float(margin),
         "candidates_per_size_mean": float(r[:, 4].mean()),
         "mean_regret_learned": float(reg_l.mean()),
         "mean_regret_default": float(reg_d.mean()),
@@ -436,9 +571,29 @@ def regret_report(data, pred, mask):
         "p90_regret_learned": float(np.quantile(reg_l, 0.9)),
         "p90_regret_default": float(np.quantile(reg_d, 0.9)),
         "geomean_speedup_learned_over_default": float(np.exp(np.mean(np.log(r[:, 3] / r[:, 2])))),
+        "geomean_speedup_best_over_default": float(np.exp(np.mean(np.log(r[:, 3] / r[:, 1])))),
+        "worst_learned_over_default": float(np.max(r[:, 2] / r[:, 3])),
+        "cases_learned_5pct_slower": int(np.sum(r[:, 2] > 1.05 * r[:, 3])),
         "learned_picks_exhaustive_best": float(np.mean(r[:, 2] <= r[:, 1])),
         "default_is_exhaustive_best": float(np.mean(r[:, 3] <= r[:, 1])),
     }
+
+
+MARGINS = (0.0, 0.02, 0.04, 0.06, 0.08, 0.1, 0.12, 0.15, 0.2, 0.25, 0.3)
+
+
+def calibrate_margin(data, pred, mask, worst=1.05):
+    """The margin, over the cases in `mask`, with the best geomean speedup
+    over the default among those that leave no case slower than the default
+    by more than `worst` (the largest margin if none does)."""
+    best = None
+    for mg in MARGINS:
+        r = regret_report(data, pred, mask, mg)
+        ok = r["worst_learned_over_default"] <= worst
+        key = (ok, r["geomean_speedup_learned_over_default"] if ok else -mg)
+        if best is None or key > best[0]:
+            best = (key, mg)
+    return best[1]
 
 
 def load_dataset(name=DATASET, directory=DATA_DIR):
@@ -446,8 +601,18 @@ def load_dataset(name=DATASET, directory=DATA_DIR):
         return {k: z[k] for k in z.files}
 
 
+def model_available(name=DATASET, directory=DATA_DIR):
+    """Whether a trained cost model is committed beside the dataset."""
+    return os.path.exists(os.path.join(directory, name + "_model.npz"))
+
+
 def load_model(name=DATASET, directory=DATA_DIR):
-    return CostModel.load(os.path.join(directory, name + "_model.npz"))
+    """The committed cost model: an Ensemble, or a single CostModel from an
+    older file."""
+    path = os.path.join(directory, name + "_model.npz")
+    with np.load(path, allow_pickle=False) as z:
+        ensemble = "members" in z.files
+    return Ensemble.load(path) if ensemble else CostModel.load(path)
 
 
 class LearnedPlanner:
@@ -456,6 +621,7 @@ class LearnedPlanner:
 
     def __init__(self, model=None):
         self.model = model if model is not None else load_model()
+        self._picks = {}
 
     def rank(self, n, batch, sign=-1):
         """Candidates of length n ordered by predicted time: [(cand, us)]."""
@@ -464,10 +630,37 @@ class LearnedPlanner:
         order = np.argsort(pred, kind="stable")
         return [(vc[k][0], float(np.exp(pred[k]))) for k in order]
 
-    def choose(self, n, batch, sign=-1):
-        """Builds only the predicted-fastest candidate: (plan, cand, us)."""
-        cand, us = self.rank(n, batch, sign)[0]
+    def pick(self, n, batch, sign=-1, margin=None):
+        """(candidate, predicted us): the predicted fastest, unless it is not
+        predicted faster than the default by the model's calibrated margin,
+        then the default ({})."""
+        vc = valid_candidates(n, batch, sign)
+        X = np.array([features(d, n, batch) for _, d in vc])
+        pred = self.model.predict(X)
+        is_def = np.array([0 if c else 1 for c, _ in vc])
+        elig = in_range(X, getattr(self.model, "feature_range", None))
+        k = pick_index(pred, is_def, self.model.margin if margin is None else margin, elig)
+        return vc[k][0], float(np.exp(pred[k]))
+
+    def pick_cached(self, n, batch, sign=-1):
+        """pick(n, batch, sign)'s candidate, remembered per (n, batch, sign)."""
+        key = (int(n), int(batch), int(sign))
+        if key not in self._picks:
+            self._picks[key] = self.pick(n, batch, sign)[0]
+        return self._picks[key]
+
+    def choose(self, n, batch, sign=-1, margin=None):
+        """Builds only the picked candidate: (plan, cand, us)."""
+        cand, us = self.pick(n, batch, sign, margin)
         return make_plan(n, batch, cand, sign=sign), cand, us
+
+    def confident(self, n, batch):
+        """Whether (n, batch) lies where the model was trained: 13-smooth
+        lengths from 1024 to 2^24 at n*batch within a factor 2 of the
+        training cases' range."""
+        lo, hi = self.model.elems_range
+        return (smooth13(n) and 1024 <= n <= 1 << 24 and hi > 0
+                and lo / 2 <= n * batch <= 2 * hi)
 
     # resampled lengths: the m-point transform's cost times this factor for
     # the deconvolution and interpolation edges (the model saw no resampled
@@ -524,6 +717,65 @@ def exhaustive_choose(n, batch, sign=-1, x=None, y=None, dtype=0):
 
 # ------------------------------------------------------------------ script
 
+MEMBERS = 5
+
+
+def train(data, steps=6000, members=MEMBERS, log=None):
+    """Fit the cost model on the training cases (every length but the
+    held-out fifth): `members` networks, each on a bootstrap resample of the
+    training lengths (seeded).  The margin is the median absolute error, out
+    of bag, of the predicted log time ratio of a candidate to its default,
+    rounded up to 0.01 -- a pick must be predicted faster than the default
+    by more than the model's typical error on that ratio.  Returns
+    (ensemble, report)."""
+    test = held_out_mask(data)
+    sids = np.unique(data["size_id"][~test])
+    train_rows = np.flatnonzero(~test)
+    norm = CostModel.standardization(data["features"][train_rows])
+    models, inbag = [], []
+    for k in range(members):
+        r = np.random.default_rng(100 + k)
+        boot = r.choice(sids, size=len(sids), replace=True)
+        cnt = np.bincount(boot, minlength=int(data["size_id"].max()) + 1)
+        rows = np.repeat(train_rows, cnt[data["size_id"][train_rows]])
+        m = CostModel(seed=k)
+        m.fit(data["features"][rows], data["log_us"][rows], steps=steps, seed=k, log=log, norm=norm)
+        models.append(m)
+        inbag.append(cnt > 0)
+        if log:
+            log(f"member {k + 1}/{members} trained")
+    preds = np.array([m.predict(data["features"]) for m in models])
+    # out-of-bag error of the log ratio to the default, per training case
+    errs = []
+    key = data["case_id"] if "case_id" in data else data["size_id"]
+    for cid in np.unique(key[~test]):
+        rows = np.flatnonzero(key == cid)
+        out = [k for k in range(members) if not inbag[k][data["size_id"][rows[0]]]]
+        d = rows[data["is_default"][rows] == 1]
+        if not out or d.size == 0:
+            continue
+        po = preds[out][:, rows].mean(0)
+        pd = preds[out][:, d[0]].mean()
+        t = data["log_us"][rows].astype(np.float64)
+        errs.extend(np.abs((po - pd) - (t - data["log_us"][d[0]])).tolist())
+    ens = Ensemble(models)
+    ens.margin = float(math.ceil(100.0 * float(np.median(errs))) / 100.0) if errs else 0.05
+    elems = data["n"] * data["batch"]
+    ens.elems_range = (int(elems.min()), int(elems.max()))
+    Xt = data["features"][train_rows]
+    ens.feature_range = (Xt.min(0).astype(np.float32), Xt.max(0).astype(np.float32))
+    fr = ens.feature_range
+    pred = preds.mean(0)
+    rmse = lambda m: float(np.sqrt(np.mean((pred[m] - data["log_us"][m]) ** 2)))  # noqa: E731
+    report = {"timings": int(len(pred)), "members": members, "train_rmse_log": rmse(~test),
+              "test_rmse_log": rmse(test), "oob_median_abs_log_ratio_error": float(np.median(errs)),
+              "margin": ens.margin,
+              "held_out": regret_report(data, pred, test, ens.margin, fr),
+              "held_out_no_margin": regret_report(data, pred, test, 0.0, fr),
+              "train": regret_report(data, pred, ~test, ens.margin, fr)}
+    return ens, report
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="collect plan timings on the GPU and train the cost model")
     ap.add_argument("--out", required=True)
@@ -545,15 +797,8 @@ def main(argv=None):
         data = collect(sizes, log=log)
         np.savez_compressed(os.path.join(a.out, DATASET + ".npz"), **data)
     log(f"{len(data['log_us'])} timings over {len(np.unique(data['size_id']))} sizes ({time.time() - t0:.0f} s)")
-    test = held_out_mask(data)
-    model = CostModel()
-    curve = model.fit(data["features"][~test], data["log_us"][~test], steps=a.steps, log=log)
+    model, report = train(data, steps=a.steps, log=log)
     model.save(os.path.join(a.out, DATASET + "_model.npz"))
-    pred = model.predict(data["features"])
-    rmse = lambda m: float(np.sqrt(np.mean((pred[m] - data["log_us"][m]) ** 2)))  # noqa: E731
-    report = {"timings": int(len(pred)), "train_rmse_log": rmse(~test), "test_rmse_log": rmse(test),
-              "loss_first": curve[0], "loss_last": curve[-1],
-              "held_out": regret_report(data, pred, test), "train": regret_report(data, pred, ~test)}
     with open(os.path.join(a.out, DATASET + "_report.json"), "w") as fh:
         json.dump(report, fh, indent=1)
     log(json.dumps(report["held_out"]))

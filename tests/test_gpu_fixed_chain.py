@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 def test_chain_rows_and_multipass(prec, n, batch):
     import torch
     T = torch.float32 if prec == "f32" else torch.float64
-    p = F.plan_fft(T, n, batch=batch)
+    p = F.plan_fft(T, n, batch=batch, planner="heuristic")
     assert "kernel=chain" in p.describe()
     x = inputs_c(n + batch, n * batch, prec).reshape(batch, n)
     y = to_host(p(to_dev(x))).reshape(batch, n)

@@ -119,7 +119,7 @@ def test_chain_column_split_out_of_cache():
     print(f"c2c f32 n={n} batch={batch} chain columns", flush=True)
     x = inputs_c(3, n * batch, "f32").reshape(batch, n)
     xd = to_dev(x)
-    p = F.plan_fft(torch.float32, n, batch=batch)
+    p = F.plan_fft(torch.float32, n, batch=batch, planner="heuristic")
     d = p.describe()
     assert d.count("n=81 radices=9,9 kernel=chain") == 2 and "n=324 " in d
     y = p(xd)

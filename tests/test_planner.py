@@ -152,6 +152,11 @@ def test_features_from_schedule():
     assert f[b0 + 3] == pytest.approx(4 * 4 / 24) and f[b1 + 3] == pytest.approx(4 * 2 / 24)
     assert f[b1 + 9] == pytest.approx(3 / 8)  # radices 4, 5, 3 are below 8 or odd
     assert np.all(f[6 + 2 * P.PASS_FEATURES:] == 0)
+    # occupancy: one-wave groups of a 139 KB tile (one per CU by LDS), then
+    # four-wave groups of 35 KB (four per CU fill all 16 wave slots)
+    assert f[b0 + 17] == pytest.approx(1 / 16) and f[b0 + 18] == pytest.approx(1 / 16)
+    assert f[b1 + 17] == pytest.approx(4 / 16) and f[b1 + 18] == pytest.approx(1.0)
+    assert f[b0 + 11] == f[b0 + 12] == f[b0 + 13] == f[b0 + 14] == f[b0 + 16] == 0
 
 
 def test_training_sizes_deterministic_and_smooth():
@@ -185,11 +190,13 @@ def test_committed_dataset_shape():
     assert n >= 4000
     assert data["features"].shape == (n, P.N_FEATURES)
     assert np.all(np.isfinite(data["log_us"]))
-    # one default row per size
-    sids = np.unique(data["size_id"])
-    assert int(data["is_default"].sum()) == len(sids)
+    # one default row per (length, batch) case, two footprints per length
+    cids = np.unique(data["case_id"])
+    assert int(data["is_default"].sum()) == len(cids) == 2 * len(np.unique(data["size_id"]))
     with np.load(_artifacts()[1], allow_pickle=False) as z:
-        assert z["params"].size == M.param_count(P.N_FEATURES, P.HIDDEN)
+        k = int(z["members"])
+        assert k >= 3 and all(z[f"m{i}_params"].size == M.param_count(P.N_FEATURES, P.HIDDEN) for i in range(k))
+        assert 0.0 <= float(z["margin"]) <= 0.3
 
 
 def test_planner_option_validated():
