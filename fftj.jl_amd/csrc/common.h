@@ -138,6 +138,9 @@ struct LeanDesc {
   int64_t tw_a0, tw_b;
   const void* ptw;
   double scale;
+  int32_t mat;      // split-precision MFMA mode of the radix-16 stages (kMatNone: VALU)
+  int32_t pad_;
+  const void* matp; // that mode's operand table for the pass's direction
 };
 
 // HvdH route (hvdh.hip) of `batch` transforms of odd non-smooth n, FP32:

@@ -1093,7 +1093,16 @@ static bool lean3_ok(const Layout& L, const std::vector<int64_t>& F, int64_t n, 
     const char* e = std::getenv("FFTJH_LEAN");
     return !e || std::atoi(e) != 0;
   }();
-  if (!env || !L.lean || L.dtype != FFTJH_F32 || L.mat || !tune_is_default(L.tune) || src != RX || dst != RY)
+  // split-precision plans take the lean kernel's MFMA variant (radix-16
+  // stages on MFMA; FFTJH_LEAN_MAT=0 keeps them on the generic MFMA chain)
+  static const bool mat_env = [] {
+    const char* e = std::getenv("FFTJH_LEAN_MAT");
+    return !e || std::atoi(e) != 0;
+  }();
+  fftjh_tuning t = L.tune;
+  t.flags &= ~(FFTJH_FLAG_MFMA_F16 | FFTJH_FLAG_MFMA_BF16 | FFTJH_FLAG_MFMA_BF16X3);
+  if (!env || !L.lean || L.dtype != FFTJH_F32 || (L.mat && !mat_env) || !tune_is_default(t) || src != RX ||
+      dst != RY)
     return false;
   if (F.size() != 3) return false;
   for (int64_t f : F)
@@ -1130,6 +1139,7 @@ static void add_lean3(Layout& L, const std::vector<int64_t>& F, int64_t n, int64
     st.ld.D1 = static_cast<int32_t>(D1);
     st.ld.groups = static_cast<int32_t>(batch * (n / (16 * N)));
     st.ld.scale = 1.0;
+    st.ld.mat = L.mat;
     st.n = N;
     st.batch = batch * (n / N);
     return st;
@@ -2073,7 +2083,9 @@ static std::string describe(const Layout& L) {
       // a pass line (the planners' features read pass lines)
       const LeanDesc& d = s.ld;
       o << "  pass " << roles[s.src] << "->" << roles[s.dst] << " n=" << d.n << " radices=16," << d.n / 16
-        << " kernel=lean" << (d.rows ? " rows_out" : "") << " batch=" << s.batch << " C=16 threads=" << d.n
+        << " kernel=lean" << (d.mat == kMatF16 ? " kernel=mfma_f16" : d.mat == kMatBF16 ? " kernel=mfma_bf16"
+                              : d.mat == kMatBF16x3 ? " kernel=mfma_bf16x3" : "")
+        << (d.rows ? " rows_out" : "") << " batch=" << s.batch << " C=16 threads=" << d.n
         << " groups=" << d.groups << " in_stride=" << d.istride << " out_stride=" << (d.rows ? 1 : d.ostride);
       if (d.psh) o << " twiddle=w_" << (int64_t(1) << d.psh);
       if (d.scale != 1.0) o << " scale=" << d.scale;
@@ -2331,6 +2343,15 @@ static void materialize(fftjh_plan_s& P) {
           hip_check(launch_tw_pow(FFTJH_F64, static_cast<char*>(p) + hi * 16, lo, 1, P, dir, 0), "twiddle kernel");
         });
         s.ld.ptw = s.ptw->p;
+      }
+      if (s.ld.mat) {
+        // split DFT-matrix operands of this mode and direction (FP64-generated)
+        std::ostringstream mk;
+        mk << "mat:" << dev << ":" << s.ld.mat << ":" << dir;
+        const int mode = s.ld.mat;
+        s.mat_tab = cached(mk.str(), mat_table_bytes(mode),
+                           [&](void* p) { hip_check(launch_mat_table(mode, dir, p, 0), "operand table kernel"); });
+        s.ld.matp = s.mat_tab->p;
       }
     }
   }

@@ -43,6 +43,7 @@
 #include "common.h"
 #include "launch.h"
 #include "passcommon.h"
+#include "stages_mat.h"
 
 #include <cstdlib>
 
@@ -234,6 +235,246 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
   }
 }
 
+// Split-precision MFMA variant (FP32 plans with precision f16, bf16 or
+// bf16x3): the radix-16 stages -- stage 1 always, stage 2 when R2 = 16 --
+// run as 16x16x32 MFMA products of the split DFT-16 matrix (stages_mat.h),
+// 16 butterflies per tile, a tile being the 16 columns at one index.
+// Lane l of wave w (g = l/16, col = l%16) loads, for tile t (stage-1 index
+// jt = 4w + t), points u = 4g..4g+3 of column col: the instruction's B
+// layout, with the loads still 16 columns wide.  It receives outputs
+// k1 = 4g..4g+3, which go to the VALU kernel's LDS slots.  Stage 2 with
+// R2 = 16 takes tile t = output k1 = 4w + t over m_lo = 4g..4g+3 and leaves
+// lane l holding X[(4w + t) + 16*(4g + i)], i < 4; with R2 = 8 or 4 it is
+// the VALU stage of k_lean.  The multi-pass twiddle of output k = jt' + R2*u'
+// is w0(jt') * S^u' as in k_lean, with one start value per tile.
+template <int DIR, int N, int ROWS, int MODE>
+__global__ void __launch_bounds__(256) k_lean_mat(LeanDesc d, const float2* __restrict__ x, float2* y,
+                                                  const float2* __restrict__ tws) {
+  using Ops = MatOps<MODE>;
+  using V = typename Ops::V;
+  constexpr int T = Ops::terms;
+  constexpr int R2 = N / 16;
+  constexpr int BPT = 16 / R2;
+  constexpr bool MAT2 = R2 == 16;
+  constexpr int LDS_ELEMS = 16 * N + N + 1;
+  constexpr int CS = N + 4;
+  static_assert(16 * CS <= LDS_ELEMS, "transposition fits the exchange buffer");
+  __shared__ __align__(16) float lf[LDS_ELEMS];
+  __shared__ float2 ltw[N];
+  __shared__ float2 lsu[16];
+  const int tid = static_cast<int>(threadIdx.x);
+  const int lane = tid & 63, wave = tid >> 6;
+  const int g4 = lane >> 4, col = lane & 15;
+  const int jt = tid >> 4;  // the VALU stage's thread role (column col)
+  const uint32_t g = blockIdx.x;
+  if (static_cast<int>(g) >= d.groups) return;
+
+  const double2* ptw = static_cast<const double2*>(d.ptw);
+  const uint32_t pmask = d.psh >= 32 ? 0xffffffffu : ((1u << d.psh) - 1u);
+  const uint32_t lmask = (1u << d.ptw_sh) - 1u;
+  auto tw_at = [&](uint32_t e) -> double2 {
+    e &= pmask;
+    return cmul64(ptw[e >> d.ptw_sh], ptw[d.ptw_lo + (e & lmask)]);
+  };
+  const uint32_t d0 = g % static_cast<uint32_t>(d.D0);
+  const uint32_t gr = g / static_cast<uint32_t>(d.D0);
+  const uint32_t d1 = gr % static_cast<uint32_t>(d.D1);
+  const uint32_t d2 = gr / static_cast<uint32_t>(d.D1);
+  const int64_t ib = d0 * d.in_s0 + d1 * d.in_s1 + d2 * d.in_s2;
+  const int64_t ob = d0 * d.out_s0 + d1 * d.out_s1 + d2 * d.out_s2;
+
+  using GV = const __attribute__((address_space(1))) V*;
+  const GV A = (GV)(d.matp) + mat_base(16, T);
+  V a1[2][T];
+#pragma unroll
+  for (int comp = 0; comp < 2; ++comp)
+#pragma unroll
+    for (int t = 0; t < T; ++t) a1[comp][t] = A[(comp * T + t) * 64 + lane];
+
+  // multi-pass twiddle starts (gathers before the data loads)
+  constexpr int NW0 = (!ROWS && MAT2) ? 4 : 1;
+  float2 w0[NW0];
+  double2 sud = make_double2(1.0, 0.0);
+#pragma unroll
+  for (int t = 0; t < NW0; ++t) w0[t] = make_float2(1.f, 0.f);
+  if (d.psh) {
+    const uint32_t bb = static_cast<uint32_t>(d.tw_b);
+    auto f2 = [](double2 a) { return make_float2(static_cast<float>(a.x), static_cast<float>(a.y)); };
+    if constexpr (ROWS) {
+      w0[0] = f2(tw_at(d1 * bb * static_cast<uint32_t>(tid)));
+    } else {
+      const uint32_t a = static_cast<uint32_t>(d.tw_a0) * (16u * d0 + static_cast<uint32_t>(col));
+      if constexpr (MAT2) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) w0[t] = f2(tw_at(d1 * (a + bb * static_cast<uint32_t>(4 * wave + t))));
+      } else {
+        w0[0] = f2(tw_at(d1 * (a + bb * static_cast<uint32_t>(jt))));
+      }
+      if (tid < 16) sud = tw_at(d1 * bb * static_cast<uint32_t>(R2) * static_cast<uint32_t>(tid));
+    }
+  }
+  const float2 tw_own = tws[tid];
+  // loads: tile t, point u = 4*g4 + q of column col is element
+  // (4w + t + R2*q) + R2*4*g4
+  float2 v[4][4];
+  {
+    const float2* xb = x + ib;
+    const int lo = col + 4 * g4 * R2 * static_cast<int>(d.istride);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[t][q] = (xb + static_cast<int64_t>(4 * wave + t + R2 * q) * d.istride)[lo];
+  }
+  ltw[tid] = tw_own;
+  if (!ROWS && tid < 16) lsu[tid] = make_float2(static_cast<float>(sud.x), static_cast<float>(sud.y));
+  __syncthreads();
+
+  // one radix-16 MFMA stage on tile values p[4] (points 4*g4..+3 of
+  // butterfly col): outputs 4*g4..+3
+  auto dft16 = [&](const float2 (&p)[4], float2 (&r)[4]) {
+    int ex = 0;
+    if constexpr (Ops::scaled) {
+      float mx = 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) mx = fmaxf(mx, fmaxf(fabsf(p[q].x), fabsf(p[q].y)));
+      mx = fmaxf(mx, __shfl_xor(mx, 16));
+      mx = fmaxf(mx, __shfl_xor(mx, 32));
+      if (mx > 0.f) (void)frexpf(mx, &ex);
+    }
+    float xs[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      xs[2 * q] = Ops::scaled ? ldexpf(p[q].x, -ex) : p[q].x;
+      xs[2 * q + 1] = Ops::scaled ? ldexpf(p[q].y, -ex) : p[q].y;
+    }
+    V b[T];
+    Ops::split(xs, b);
+    mat_f4 zr = {0.f, 0.f, 0.f, 0.f}, zi = {0.f, 0.f, 0.f, 0.f};
+    mat_dot2<Ops>(a1[0], a1[1], b, zr, zi);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      r[i] = Ops::scaled ? make_float2(ldexpf(zr[i], ex), ldexpf(zi[i], ex)) : make_float2(zr[i], zi[i]);
+  };
+
+  // stage 1 and its twiddle w_N^(jt*k1), jt = 4w + t, k1 = 4*g4 + i
+  float2 o[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    dft16(v[t], o[t]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[t][i] = cmul<float>(o[t][i], ltw[(4 * wave + t) * (4 * g4 + i)]);
+  }
+  // exchange (real parts, then imaginary parts) through slots [k1][jt][col]
+  auto wslot = [&](int t, int i) { return lp(((4 * g4 + i) * R2 + 4 * wave + t) * 16 + col); };
+  float2 in[16];
+  auto rslot = [&](int q) {
+    if constexpr (MAT2) return lp(((4 * wave + (q >> 2)) * 16 + 4 * g4 + (q & 3)) * 16 + col);
+    return lp(((jt + R2 * (q / R2)) * R2 + q % R2) * 16 + col);
+  };
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) lf[wslot(t, i)] = o[t][i].x;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) in[q].x = lf[rslot(q)];
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) lf[wslot(t, i)] = o[t][i].y;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) in[q].y = lf[rslot(q)];
+
+  // stage 2: out[u] holds output k(u) of column col, with
+  //   MAT2: u = 4t + i, k = (4w + t) + 16*(4*g4 + i);
+  //   else: k = jt + R2*u (the VALU kernel's order)
+  float2 out[16];
+  if constexpr (MAT2) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float2 p[4] = {in[4 * t], in[4 * t + 1], in[4 * t + 2], in[4 * t + 3]};
+      float2 r[4];
+      dft16(p, r);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) out[4 * t + i] = r[i];
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      float2 a[R2];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) a[m] = in[b * R2 + m];
+      dft_pow2<float, DIR, R2>(a);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) out[b + BPT * k2] = a[k2];
+    }
+  }
+  auto kof = [&](int u) {
+    if constexpr (MAT2) return (4 * wave + (u >> 2)) + 16 * (4 * g4 + (u & 3));
+    return jt + R2 * u;
+  };
+  const float scale = static_cast<float>(d.scale);
+  const bool scaled = d.scale != 1.0;
+  float2* yb = y + ob;
+  if constexpr (ROWS) {
+    // transpose through LDS: column col, element k at [col][k]
+    float2 oo[16];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) lf[col * CS + kof(u)] = out[u].x;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) oo[i].x = lf[i * CS + tid];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) lf[col * CS + kof(u)] = out[u].y;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) oo[i].y = lf[i * CS + tid];
+    float2 w = w0[0];
+    if (scaled) w = cscale<float>(w, scale);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const float2 a = (d.psh || scaled) ? cmul<float>(oo[i], w) : oo[i];
+      (yb + i * d.ostride)[tid] = a;
+    }
+  } else if constexpr (MAT2) {
+    // k = (4w + t + 16 i) + 64*g4: scalar row part, lane part col + 64*g4*ostride
+    const int lo = col + 64 * g4 * static_cast<int>(d.ostride);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      float2 ws = w0[t];
+      if (scaled) ws = cscale<float>(ws, scale);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float2 val = out[4 * t + i];
+        if (d.psh) {
+          val = cmul<float>(val, cmul<float>(ws, lsu[4 * g4 + i]));
+        } else if (scaled) {
+          val = cscale<float>(val, scale);
+        }
+        (yb + static_cast<int64_t>(4 * wave + t + 16 * i) * d.ostride)[lo] = val;
+      }
+    }
+  } else {
+    const int lo = col + jt * static_cast<int>(d.ostride);
+    float2 ws = w0[0];
+    if (scaled) ws = cscale<float>(ws, scale);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      float2 val = out[u];
+      if (d.psh) {
+        val = cmul<float>(val, u ? cmul<float>(ws, lsu[u]) : ws);
+      } else if (scaled) {
+        val = cscale<float>(val, scale);
+      }
+      (yb + static_cast<int64_t>(R2) * u * d.ostride)[lo] = val;
+    }
+  }
+}
+
 // FFTJH_LEAN_SPLIT: 1 = split exchanges (float LDS), 0 = float2 exchanges
 static int lean_split() {
   static const int v = [] {
@@ -257,8 +498,37 @@ hipError_t launch_n(const LeanDesc& d, const void* x, void* y, const void* tws, 
   return d.rows ? launch_k<DIR, N, 1, 0>(d, x, y, tws, st) : launch_k<DIR, N, 0, 0>(d, x, y, tws, st);
 }
 
+template <int DIR, int N, int MODE>
+hipError_t launch_mat_m(const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t st) {
+  if (d.rows)
+    hipLaunchKernelGGL((k_lean_mat<DIR, N, 1, MODE>), dim3(d.groups), dim3(N), 0, st, d,
+                       static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+  else
+    hipLaunchKernelGGL((k_lean_mat<DIR, N, 0, MODE>), dim3(d.groups), dim3(N), 0, st, d,
+                       static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+  return hipGetLastError();
+}
+
+template <int DIR, int N>
+hipError_t launch_mat_n(const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t st) {
+  switch (d.mat) {
+    case kMatF16: return launch_mat_m<DIR, N, kMatF16>(d, x, y, tws, st);
+    case kMatBF16: return launch_mat_m<DIR, N, kMatBF16>(d, x, y, tws, st);
+    case kMatBF16x3: return launch_mat_m<DIR, N, kMatBF16x3>(d, x, y, tws, st);
+  }
+  return hipErrorInvalidValue;
+}
+
 template <int DIR>
 hipError_t launch_dir(const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t st) {
+  if (d.mat != kMatNone) {
+    switch (d.n) {
+      case 64: return launch_mat_n<DIR, 64>(d, x, y, tws, st);
+      case 128: return launch_mat_n<DIR, 128>(d, x, y, tws, st);
+      case 256: return launch_mat_n<DIR, 256>(d, x, y, tws, st);
+    }
+    return hipErrorInvalidValue;
+  }
   switch (d.n) {
     case 64: return launch_n<DIR, 64>(d, x, y, tws, st);
     case 128: return launch_n<DIR, 128>(d, x, y, tws, st);

@@ -120,3 +120,26 @@ def test_mfma_study_configs(mode, n, tuning):
         with open(out, "a") as fh:
             fh.write(f"{mode} n={n} batch={batch} tuning={tuning} rel_l2={err:.3e} plain_fp32={err_plain:.3e}\n")
     assert err <= tol_mode(mode, n), (mode, n, err, err_plain)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n,batch", [(1 << 18, 2), (1 << 21, 1), (1 << 24, 1)])
+def test_mfma_lean_layout(mode, n, batch):
+    # three-pass power-of-two plans: the lean kernel with its radix-16
+    # stages on MFMA (stage 1 of every pass, stage 2 of the 256-point ones);
+    # forward vs the oracle, the 1/n inverse back, and the in-place call
+    # (x == y runs through the plan's scratch)
+    print(f"lean mfma {mode} n={n} batch={batch}", flush=True)
+    x = inputs_c(n % 89 + batch, n * batch, "f32").reshape(batch, n)
+    p = F.plan_fft(torch.float32, n, batch=batch, precision=mode)
+    assert p.describe().count("kernel=lean kernel=mfma_" + mode + " ") == 3
+    xd = to_dev(x)
+    yd = p * xd
+    rows = [0, batch - 1] if batch > 1 else [0]
+    y = to_host(yd).reshape(batch, n)
+    assert rel_l2(y[rows], O.fft(x[rows].astype(np.complex128))) <= tol_mode(mode, n)
+    pi = F.plan_ifft(torch.float32, n, batch=batch, precision=mode)
+    assert rel_l2(to_host(pi * yd).reshape(batch, n), x) <= 2 * tol_mode(mode, n)
+    zd = to_dev(x)
+    p(zd, zd)
+    assert rel_l2(to_host(zd).reshape(batch, n), y) <= tol_mode(mode, n)

@@ -287,6 +287,13 @@ def test_mfma_flags_and_schedules():
         L.describe_tuned(L.F32, (4096,), 1, -1, L.make_tuning((), (), 0x100))
     with pytest.raises(L.ArgumentError):  # radix 32 has no VALU kernel
         L.describe_tuned(L.F32, (1024,), 1, -1, L.make_tuning((), [{"radices": [32, 32]}]))
+    # three-pass power-of-two plans take the lean kernel's MFMA variant; a
+    # tuned split or FFTJH_LEAN_MAT=0 keeps the generic MFMA chain
+    for bits, name in ((L.FLAG_MFMA_F16, "mfma_f16"), (L.FLAG_MFMA_BF16X3, "mfma_bf16x3")):
+        s = _passes(L.describe_tuned(L.F32, (1 << 20,), 64, -1, L.make_tuning((), (), bits)))
+        assert len(s) == 3 and all(f"kernel=lean kernel={name} " in p for p in s) and "rows_out" in s[0]
+    s = _passes(L.describe_tuned(L.F32, (1 << 20,), 64, -1, L.make_tuning((1024, 1024), (), L.FLAG_MFMA_F16)))
+    assert len(s) == 2 and not any("kernel=lean" in p for p in s)
     # every MFMA pass stays within the 512-thread bound of its kernel
     for n in (1 << 20, 3 ** 12 * 4, 1 << 16, 1458, 4374, 9 * 16 * 16 * 16):
         d = L.describe_tuned(L.F32, (n,), 64, -1, L.make_tuning((), (), L.FLAG_MFMA_BF16X3))
