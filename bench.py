@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter runs: 3 forward plans of the workload, then 3 copies of the same bytes "
                          "(the calibration kernel of known traffic); no timing, no JSON")
+    ap.add_argument("--pmc-ragged", action="store_true",
+                    help="counter runs of the ragged batch: 3 launches of its default plan, then 3 copies of "
+                         "the same bytes; no timing, no JSON")
     return ap.parse_args()
 
 
@@ -81,6 +84,25 @@ def pmc_probe(args):
         y.copy_(x)
     torch.cuda.synchronize()
     print("pmc probe done", flush=True)
+
+
+def pmc_ragged():
+    import numpy as np
+    import torch
+    import __graft_entry__
+
+    F = __graft_entry__.load_package()
+    sizes = F.ragged.benchmark_sizes()
+    x = torch.from_numpy(F.rng.uniform_complex(1, int(sizes.sum())).astype(np.complex64)).to("cuda:0")
+    y = torch.empty_like(x)
+    p = F.plan_ragged(torch.float32, sizes)
+    for _ in range(3):
+        p(y, x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("pmc ragged probe done", flush=True)
 
 
 def copy_probe():
@@ -451,6 +473,9 @@ def main():
     args = parse()
     if args.pmc_probe:
         pmc_probe(args)
+        return
+    if args.pmc_ragged:
+        pmc_ragged()
         return
     if args.copy_probe:
         copy_probe()
