@@ -345,7 +345,8 @@ struct Step {
   Geometry ga{1, 0, 0}, gb{1, 0, 0};
   // device resources
   DevBufPtr tw, desc, aux, mat_tab, ptw;
-  std::shared_ptr<struct RaggedData> rg;  // ST_RAGGED
+  std::shared_ptr<struct RaggedData> rg;  // ST_RAGGED: items of at most kRaggedTile points
+  std::shared_ptr<struct RaggedData> rg2; // ST_RAGGED: longer items, on the large-item kernel
 };
 
 struct Layout {
@@ -1828,6 +1829,9 @@ struct RaggedData {
   int groups = 0;
   int radix_pref = 16;     // 16: radix-16 towers; 8: radix-8 towers
   int tile_max = kRaggedTile;
+  int big = 0;                      // 1: the large-item kernel (one workgroup per CU)
+  int threads = kRaggedThreads;     // its workgroup size
+  std::map<int64_t, int64_t> corr_of;  // resampled length -> offset in the correction pool
   std::vector<RaggedItem> items;
   std::vector<RaggedSched> scheds;
   std::map<int, int> sched_of;  // tile length -> schedule
@@ -1862,7 +1866,7 @@ static std::vector<int> radices_pref(int64_t n, int pref) {
 static int ragged_tile_C(const RaggedData& R, int64_t L, const std::vector<int>& rad) {
   for (int C = 64; C >= 1; C >>= 1) {
     if (C * L > R.tile_max) continue;
-    if (pass_threads(static_cast<int>(L), C, rad) > kRaggedThreads) continue;
+    if (pass_threads(static_cast<int>(L), C, rad) > R.threads) continue;
     return C;
   }
   return 0;
@@ -1964,9 +1968,21 @@ static Layout layout_ragged(int dtype, int64_t count, const int64_t* sizes, cons
       fail(FFTJH_ERR_ARGUMENT, "tile_max must be in [256, " + std::to_string(kRaggedTile) + "]");
     R->tile_max = tune->tile_max;
   }
+  // items longer than the small kernel's tile go to the large-item kernel
+  // (FFTJH_RAGGED_BIG=0: every item on the small one, as two-level tiles)
+  static const bool big_env = [] {
+    const char* e = std::getenv("FFTJH_RAGGED_BIG");
+    return !e || std::atoi(e) != 0;
+  }();
+  auto R2 = std::make_shared<RaggedData>();
+  R2->dir = sign;
+  R2->radix_pref = R->radix_pref;
+  R2->big = 1;
+  R2->threads = ragged_big_threads(dtype);
+  R2->tile_max = ragged_big_tile(dtype);
   const int64_t nmax = dtype == FFTJH_F32 ? 65536 : 32768;
   int64_t extent = 0, off = 0;
-  std::map<int64_t, int64_t> corr_of;
+  std::vector<double> cost_small, cost_big;
   R->items.reserve(count);
   for (int64_t i = 0; i < count; ++i) {
     const int64_t n = sizes[i];
@@ -1976,47 +1992,49 @@ static Layout layout_ragged(int dtype, int64_t count, const int64_t* sizes, cons
     if (o < 0) fail(FFTJH_ERR_ARGUMENT, "offsets must be non-negative");
     off = o + n;
     extent = std::max(extent, o + n);
+    int a, cnt[5];
+    const bool smooth = factor13(n, a, cnt);
+    const RsParams rp = smooth ? RsParams{n, 0, 0.0} : rs_params(dtype, n);
+    const int64_t e = smooth ? n : rp.m;  // the length one tile must hold
+    RaggedData& C = (big_env && e > R->tile_max) ? *R2 : *R;
     RaggedItem it{};
     it.off = o;
     it.n = static_cast<int>(n);
     it.scale = (sign > 0 && normalized) ? 1.0 / static_cast<double>(n) : 1.0;
-    int a, cnt[5];
-    if (factor13(n, a, cnt)) {
-      if (ragged_direct_ok(*R, n)) {
+    if (smooth) {
+      if (ragged_direct_ok(C, n)) {
         it.kind = kRaggedDirect;
-        it.sA = ragged_sched(*R, n);
+        it.sA = ragged_sched(C, n);
       } else {
         it.kind = kRaggedTwoLevel;
-        ragged_two_level(*R, it, n);
+        ragged_two_level(C, it, n);
       }
     } else {
-      const RsParams rp = rs_params(dtype, n);
       it.m = static_cast<int>(rp.m);
       it.w = rp.w;
       it.beta = rp.beta;
-      auto c = corr_of.find(n);
-      if (c == corr_of.end()) {
-        R->corr_jobs.push_back({R->corr_total, n, rp.m, rp.w, rp.beta, it.scale});
-        c = corr_of.emplace(n, R->corr_total).first;
-        R->corr_total += n;
+      auto c = C.corr_of.find(n);
+      if (c == C.corr_of.end()) {
+        C.corr_jobs.push_back({C.corr_total, n, rp.m, rp.w, rp.beta, it.scale});
+        c = C.corr_of.emplace(n, C.corr_total).first;
+        C.corr_total += n;
       }
       it.corr_off = c->second;
-      if (ragged_direct_ok(*R, rp.m)) {
+      if (ragged_direct_ok(C, rp.m)) {
         it.kind = kRaggedRsDirect;
-        it.sA = ragged_sched(*R, rp.m);
+        it.sA = ragged_sched(C, rp.m);
       } else {
         it.kind = kRaggedRsTwo;
-        ragged_two_level(*R, it, rp.m);
+        ragged_two_level(C, it, rp.m);
       }
     }
-    R->counts[it.kind]++;
-    R->items.push_back(it);
+    C.counts[it.kind]++;
+    C.items.push_back(it);
+    // predicted per-item cost: the caller's (a learned model) or the analytic one
+    C.cost.push_back((tune && tune->costs) ? tune->costs[i] : ragged_cost(it));
   }
-  // predicted per-item cost: the caller's (a learned model) or the analytic one
-  R->cost.resize(count);
-  for (int64_t i = 0; i < count; ++i)
-    R->cost[i] = (tune && tune->costs) ? tune->costs[i] : ragged_cost(R->items[i]);
   // workgroups: resident ones (set at materialize time), or the tuning's
+  // (the small-item kernel's)
   R->groups = (tune && tune->groups > 0) ? tune->groups : 0;
   Layout L;
   L.dtype = dtype;
@@ -2033,6 +2051,7 @@ static Layout layout_ragged(int dtype, int64_t count, const int64_t* sizes, cons
   st.dst = RY;
   st.dir = sign;
   st.rg = R;
+  if (!R2->items.empty()) st.rg2 = R2;
   L.steps.push_back(st);
   return L;
 }
@@ -2115,10 +2134,16 @@ static std::string describe(const Layout& L) {
       if (s.d.nt_load || s.d.nt_store) o << " stream=" << (s.d.nt_load ? "in" : "") << (s.d.nt_store ? "out" : "");
     } else if (s.kind == ST_RAGGED) {
       const RaggedData& R = *s.rg;
-      o << " transforms=" << R.items.size() << " direct=" << R.counts[0] << " two_level=" << R.counts[1]
-        << " resample=" << R.counts[2] << " resample_two_level=" << R.counts[3] << " tile_lengths=" << R.scheds.size()
-        << " radix_pref=" << R.radix_pref << " tile_max=" << R.tile_max;
+      const RaggedData* B = s.rg2.get();
+      auto cnt = [&](int k) { return R.counts[k] + (B ? B->counts[k] : 0); };
+      o << " transforms=" << R.items.size() + (B ? B->items.size() : 0) << " direct=" << cnt(0)
+        << " two_level=" << cnt(1) << " resample=" << cnt(2) << " resample_two_level=" << cnt(3)
+        << " tile_lengths=" << R.scheds.size() + (B ? B->scheds.size() : 0) << " radix_pref=" << R.radix_pref
+        << " tile_max=" << R.tile_max;
       if (R.groups) o << " workgroups=" << R.groups;
+      if (B)
+        o << " large_transforms=" << B->items.size() << " large_tile=" << B->tile_max << " large_direct=" << B->counts[0]
+          << " large_two_level=" << B->counts[1] + B->counts[3];
     } else if (s.kind == ST_HVDH_PERMUTE || s.kind == ST_HVDH_SPREAD || s.kind == ST_HVDH_SOLVE) {
       o << " n=" << s.hv.n << " t=" << s.hv.t << " taps=" << 2 * s.hv.h << " band=" << s.hv.B << " halo=" << s.hv.H
         << " batch=" << s.hv.batch;
@@ -2256,9 +2281,11 @@ static void materialize(fftjh_plan_s& P) {
       s.desc = std::make_shared<DevBuf>(sizeof(PassDesc));
       hip_check(hipMemcpy(s.desc->p, &s.d, sizeof(PassDesc), hipMemcpyHostToDevice), "descriptor upload");
     } else if (s.kind == ST_RAGGED) {
-      RaggedData& R = *s.rg;
+     for (RaggedData* Rp : {s.rg.get(), s.rg2.get()}) {
+      if (!Rp || Rp->items.empty()) continue;
+      RaggedData& R = *Rp;
       if (R.groups <= 0) {
-        const int res = ragged_resident_groups(dt, R.dir);
+        const int res = ragged_resident_groups(dt, R.dir, R.big);
         R.groups = res > 0 ? res : 1024;
       }
       R.groups = static_cast<int>(std::min<int64_t>(R.groups, static_cast<int64_t>(R.items.size())));
@@ -2305,6 +2332,7 @@ static void materialize(fftjh_plan_s& P) {
       }
       hip_check(hipDeviceSynchronize(), "ragged tables");
       R.d_ws = std::make_shared<DevBuf>(std::max<int64_t>(R.ws_total, 1) * es);
+     }
     } else if (s.kind == ST_R2C_TWIST || s.kind == ST_C2R_PRE) {
       // w_k = exp(dir*2*pi*i*k/n), k = 0..h (real.jl:26-39)
       const int64_t cnt = s.h + 1, n = s.n;
@@ -2399,7 +2427,8 @@ static int64_t static_layout(const fftjh_plan_s& P, int64_t role_off[4], int64_t
   }
   *ws_off = at;
   for (const Step& s : P.L.steps)
-    if (s.kind == ST_RAGGED) at += align256(s.rg->ws_total * es);
+    if (s.kind == ST_RAGGED)
+      at += align256(std::max(s.rg->ws_total, s.rg2 ? s.rg2->ws_total : int64_t(0)) * es);
   return at;
 }
 
@@ -2444,20 +2473,28 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
     case ST_STRIDED_COPY:
       return launch_strided_copy(dt, ptr[s.src], ptr[s.dst], s.n, s.batch, s.Bi, s.ga, s.gb, st);
     case ST_RAGGED: {
-      const RaggedData& R = *s.rg;
-      RaggedLaunch rl;
-      rl.groups = R.groups;
-      rl.items = R.d_items->p;
-      rl.wg_ptr = R.d_wgptr->p;
-      rl.wg_items = R.d_wgitems->p;
-      rl.scheds = R.d_scheds->p;
-      rl.tw = R.d_tw->p;
-      rl.corr = R.d_corr->p;
-      rl.x = ptr[s.src];
-      rl.y = ptr[s.dst];
-      rl.ws = ws ? ws : R.d_ws->p;
-      rl.wg_ws = R.d_wgws->p;
-      return launch_ragged(dt, s.dir, rl, st);
+      // the small-item launch, then the large-item one (same stream: they
+      // may share a caller's workspace)
+      for (const RaggedData* Rp : {s.rg.get(), s.rg2.get()}) {
+        if (!Rp || Rp->items.empty()) continue;
+        const RaggedData& R = *Rp;
+        RaggedLaunch rl;
+        rl.big = R.big;
+        rl.groups = R.groups;
+        rl.items = R.d_items->p;
+        rl.wg_ptr = R.d_wgptr->p;
+        rl.wg_items = R.d_wgitems->p;
+        rl.scheds = R.d_scheds->p;
+        rl.tw = R.d_tw->p;
+        rl.corr = R.d_corr->p;
+        rl.x = ptr[s.src];
+        rl.y = ptr[s.dst];
+        rl.ws = ws ? ws : R.d_ws->p;
+        rl.wg_ws = R.d_wgws->p;
+        const hipError_t e = launch_ragged(dt, s.dir, rl, st);
+        if (e != hipSuccess) return e;
+      }
+      return hipSuccess;
     }
   }
   return hipErrorInvalidValue;

@@ -6,8 +6,13 @@
 
 namespace fftjh {
 
-constexpr int kRaggedThreads = 256;  // workgroup size
-constexpr int kRaggedTile = 4096;    // complex elements one LDS tile holds
+constexpr int kRaggedThreads = 256;  // workgroup size of the small-item launch
+constexpr int kRaggedTile = 4096;    // complex elements one LDS tile holds (small items)
+// the large-item launch: one workgroup per CU whose LDS tile holds 16384
+// complex FP32 (8192 FP64) elements, so lengths up to that run in one tile
+// and longer ones take 64-column tiles in their two phases
+constexpr int ragged_big_threads(int dtype) { return dtype == 0 ? 1024 : 512; }
+constexpr int ragged_big_tile(int dtype) { return dtype == 0 ? 16384 : 8192; }
 
 enum RaggedKind : int32_t { kRaggedDirect = 0, kRaggedTwoLevel = 1, kRaggedRsDirect = 2, kRaggedRsTwo = 3 };
 
@@ -36,6 +41,7 @@ struct RaggedItem {
 };
 
 struct RaggedLaunch {
+  int big;  // 0: the small-item kernel, 1: the large-item kernel
   int groups;
   const void* items;
   const void* wg_ptr;
@@ -50,6 +56,6 @@ struct RaggedLaunch {
 };
 
 hipError_t launch_ragged(int dtype, int dir, const RaggedLaunch& L, hipStream_t st);
-int ragged_resident_groups(int dtype, int dir);
+int ragged_resident_groups(int dtype, int dir, int big);
 
 }  // namespace fftjh

@@ -338,8 +338,10 @@ class CostModel:
         self.set_params(init_params(in_features, hidden, seed))
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib._LIB is not None:
-            _lib._LIB.fftjh_mlp_destroy(self._h)
+        # (at interpreter exit the module globals may already be gone)
+        L = getattr(_lib, "_LIB", None) if _lib is not None else None
+        if getattr(self, "_h", None) and L is not None:
+            L.fftjh_mlp_destroy(self._h)
             self._h = None
 
     @staticmethod

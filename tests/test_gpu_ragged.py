@@ -39,6 +39,8 @@ def test_ragged_all_kinds(prec):
     for kind in ("direct=", "two_level=", "resample=", "resample_two_level="):
         assert kind in d
     assert " two_level=0" not in d and " resample=0" not in d and "resample_two_level=0" not in d
+    # lengths above 4096 points run on the large-tile launch (16384 FP32, 8192 FP64)
+    assert " large_transforms=" in d and (" large_tile=16384" if prec == "f32" else " large_tile=8192") in d
 
 
 def test_ragged_inverse_and_inplace():
