@@ -60,7 +60,14 @@ __device__ __forceinline__ int lp(int f) { return f + (f >> 4); }
 // group's data while transforming the current one (138-151 VGPRs, 3 waves
 // per SIMD) measured slower on the 2^24 plan (0.285 -> 0.303 ms per
 // forward + inverse), as did half the occupancy (LDS padding).
-template <int DIR, int N, int ROWS, int SPLIT>
+// XLANE = 1 (N = 64: one wave, four threads per column in lanes c, c + 16,
+// c + 32, c + 48): the exchange between the stages is a 4 x 4 transpose of
+// 4-element blocks between the wave's rows of 16 lanes, done in registers by
+// v_permlane32_swap (rows r <-> r ^ 2) and v_permlane16_swap (r <-> r ^ 1),
+// with no LDS traffic and no barrier.  The pairs (k1 = 4b + s, 4b + 2 + s)
+// swap across halves and (4b, 4b + 1), (4b + 2, 4b + 3) across neighbouring
+// rows, after which register q holds the stage-2 input q of the LDS path.
+template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0>
 __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restrict__ x, float2* y,
                                               const float2* __restrict__ tws) {
   constexpr int R2 = N / 16;   // threads per column
@@ -151,7 +158,43 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
     auto wslot = [&](int k1) { return lp((k1 * R2 + jt) * 16 + c); };
     auto rslot = [&](int q) { return lp(((jt + R2 * (q / R2)) * R2 + q % R2) * 16 + c); };
     float2 in[16];
-    if constexpr (SPLIT) {
+    if constexpr (XLANE) {
+      static_assert(N == 64, "the register exchange is for one-wave groups");
+      auto swap32 = [](float& a, float& b) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a),
+                                                        __builtin_bit_cast(unsigned, b), false, false);
+        // (the elements are copied out before the casts: a bit cast of
+        // r[1] itself compiled to element 0 at -O3 with this toolchain)
+        const unsigned r0 = r[0], r1 = r[1];
+        a = __uint_as_float(r0);
+        b = __uint_as_float(r1);
+      };
+      auto swap16 = [](float& a, float& b) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, a),
+                                                        __builtin_bit_cast(unsigned, b), false, false);
+        // (the elements are copied out before the casts: a bit cast of
+        // r[1] itself compiled to element 0 at -O3 with this toolchain)
+        const unsigned r0 = r[0], r1 = r[1];
+        a = __uint_as_float(r0);
+        b = __uint_as_float(r1);
+      };
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+          swap32(t[4 * b + s2].x, t[4 * b + 2 + s2].x);
+          swap32(t[4 * b + s2].y, t[4 * b + 2 + s2].y);
+        }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        swap16(t[4 * b].x, t[4 * b + 1].x);
+        swap16(t[4 * b].y, t[4 * b + 1].y);
+        swap16(t[4 * b + 2].x, t[4 * b + 3].x);
+        swap16(t[4 * b + 2].y, t[4 * b + 3].y);
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) in[q] = t[q];
+    } else if constexpr (SPLIT) {
 #pragma unroll
       for (int k1 = 0; k1 < 16; ++k1) lf[wslot(k1)] = t[k1].x;
       __syncthreads();
@@ -484,15 +527,28 @@ static int lean_split() {
   return v;
 }
 
-template <int DIR, int N, int ROWS, int SPLIT>
+// FFTJH_LEAN_XLANE: 1 = the register exchange for N = 64, 0 = through LDS
+static int lean_xlane() {
+  static const int v = [] {
+    const char* e = std::getenv("FFTJH_LEAN_XLANE");
+    return e ? std::atoi(e) : 1;
+  }();
+  return v;
+}
+
+template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0>
 hipError_t launch_k(const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t st) {
-  hipLaunchKernelGGL((k_lean<DIR, N, ROWS, SPLIT>), dim3(d.groups), dim3(N), 0, st, d, static_cast<const float2*>(x),
-                     static_cast<float2*>(y), static_cast<const float2*>(tws));
+  hipLaunchKernelGGL((k_lean<DIR, N, ROWS, SPLIT, XLANE>), dim3(d.groups), dim3(N), 0, st, d,
+                     static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
   return hipGetLastError();
 }
 
 template <int DIR, int N>
 hipError_t launch_n(const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t st) {
+  if constexpr (N == 64) {
+    if (lean_xlane())
+      return d.rows ? launch_k<DIR, N, 1, 1, 1>(d, x, y, tws, st) : launch_k<DIR, N, 0, 1, 1>(d, x, y, tws, st);
+  }
   if (lean_split())
     return d.rows ? launch_k<DIR, N, 1, 1>(d, x, y, tws, st) : launch_k<DIR, N, 0, 1>(d, x, y, tws, st);
   return d.rows ? launch_k<DIR, N, 1, 0>(d, x, y, tws, st) : launch_k<DIR, N, 0, 0>(d, x, y, tws, st);
