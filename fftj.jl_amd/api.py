@@ -393,8 +393,10 @@ def _route_flags(route):
     """route=None (the planner's choice) or, for a length that is not
     13-smooth, "composite" (one exact step over its smooth part around the
     resampled rest, composite.jl), "resample" (the whole length resampled,
-    resample.jl) or "hvdh" (odd lengths, FP32: Gaussian resampling onto a
-    coprime smooth grid and chunked banded solves, hvdh.jl)."""
+    resample.jl) or "hvdh" (FP32: Gaussian resampling onto a coprime smooth
+    grid and chunked banded solves, hvdh.jl; an even length takes one exact
+    step over its smooth part around HvdH transforms of the odd rest, the
+    reference's SplitPlan, composite.jl:103-200)."""
     if route is None:
         return 0
     if route not in _lib.ROUTE_FLAGS:

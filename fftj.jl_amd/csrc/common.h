@@ -163,6 +163,9 @@ struct HvdhDesc {
   float nt;          // n / t
   float c1, c2;      // exp(-a), exp(-4a)
   float scale;
+  // where the output permutation writes: transform b, element r at
+  // (b % o_bi)*o_in + (b / o_bi)*o_out + r*o_stride (rows: 1, 1, n, 0)
+  int64_t o_stride, o_bi, o_out, o_in;
 };
 
 // LDS elements a pass with C*n = cn needs (one pad slot per 16, see lpad)

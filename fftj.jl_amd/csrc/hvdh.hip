@@ -42,6 +42,9 @@ namespace {
 // and writes 16 runs along j, transposed in LDS; the rows r >= I*Delta take
 // l = r*c mod n singly.  Both the spread's input (x into Delta-permuted
 // order) and the solve's output (unknowns x_l to y) are this permutation.
+// src is rows of n; dst element r of transform b is at (b % o_bi)*o_in +
+// (b / o_bi)*o_out + r*o_stride, so the solve's output can land at a stride
+// (the length-p transforms of a split length, each at stride s).
 constexpr int kPermI = 16, kPermJ = 64;
 __global__ void __launch_bounds__(256) k_hvdh_permute(HvdhDesc d, const float2* __restrict__ src, float2* dst,
                                                       float scale) {
@@ -53,13 +56,15 @@ __global__ void __launch_bounds__(256) k_hvdh_permute(HvdhDesc d, const float2* 
   const int64_t per = tiles + extra;
   const int64_t b = blockIdx.x / per, blk = blockIdx.x - b * per;
   const float2* sb = src + b * n;
-  float2* db = dst + b * n;
+  const int64_t bi = b % d.o_bi;
+  float2* db = dst + bi * d.o_in + (b - bi) / d.o_bi * d.o_out;
+  const int64_t os = d.o_stride;
   const int tid = static_cast<int>(threadIdx.x);
   if (blk >= tiles) {
     const int64_t r = I * D + (blk - tiles) * 256 + tid;
     if (r < n) {
       const float2 v = sb[(r * cinv) % n];
-      db[r] = make_float2(v.x * scale, v.y * scale);
+      db[r * os] = make_float2(v.x * scale, v.y * scale);
     }
     return;
   }
@@ -83,7 +88,7 @@ __global__ void __launch_bounds__(256) k_hvdh_permute(HvdhDesc d, const float2* 
     const int64_t i = ib + ii, j = jb + jj;
     if (i < I && j < D) {
       const float2 v = tile[jj][ii];
-      db[i * D + j] = make_float2(v.x * scale, v.y * scale);
+      db[(i * D + j) * os] = make_float2(v.x * scale, v.y * scale);
     }
   }
 }

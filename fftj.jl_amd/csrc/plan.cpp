@@ -1048,6 +1048,10 @@ static bool hvdh_params(int dtype, int64_t n, int64_t batch, double scale, HvdhD
   d.c1 = static_cast<float>(std::exp(-a));
   d.c2 = static_cast<float>(std::exp(-4 * a));
   d.scale = static_cast<float>(scale);
+  d.o_stride = 1;
+  d.o_bi = 1;
+  d.o_out = n;
+  d.o_in = 0;
   return true;
 }
 
@@ -1057,7 +1061,9 @@ static bool hvdh_params(int dtype, int64_t n, int64_t batch, double scale, HvdhD
 // tried (F32 x 4, forward, us: 16,387 49 / 20, 65,537 59 / 34, 100,003 75 /
 // 38, 300,007 100 / 57, 1,000,003 213 / 115): on the GPU the grid transform
 // is cheap, and the route's two permutations and its solve (a sequential
-// recurrence per chunk) cost more than resampling's doubled grid.
+// recurrence per chunk) cost more than resampling's doubled grid.  The same
+// holds for even lengths split around HvdH (F32 x 4, forward, us: 2*65,537
+// 96 / 50, 16*65,537 278 / 157, 4*1,000,003 1,091 / 520).
 // FFTJH_HVDH=1 makes it the default where it applies.
 static bool use_hvdh(const Layout& L, int64_t n) {
   if (L.route & FFTJH_FLAG_ROUTE_HVDH) return true;
@@ -1337,7 +1343,18 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
       while ((n / sm) % f == 0) sm *= f;
     const int64_t p = n / sm;
     const bool ok = sm > 1 && is_rows(gin, n, Bi) && is_rows(gout, n, Bi) && single_pass_ok(L.dtype, sm);
-    if (ok && use_composite(L, n, sm)) {
+    // the HvdH route on a length it cannot take whole (even n: no grid
+    // coprime to it) takes the same step around length-p HvdH transforms,
+    // whose output permutation writes at stride s: the reference's SplitPlan
+    // (composite.jl:103-200, length-s combine after the length-p HvdH; here
+    // the length-s step runs first, as above, so that the HvdH transforms
+    // read contiguous rows)
+    bool split_hv = false;
+    if (ok && (L.route & FFTJH_FLAG_ROUTE_HVDH)) {
+      HvdhDesc whole, part;
+      split_hv = !hvdh_params(L.dtype, n, batch, scale, whole) && hvdh_params(L.dtype, p, batch * sm, scale, part);
+    }
+    if (ok && (split_hv || use_composite(L, n, sm))) {
       PassTw t;
       t.period = n;
       t.qdiv = 1;
@@ -1351,7 +1368,9 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
   // the HvdH route (hvdh.jl:222-241): x permuted to Delta order, spread onto
   // grid rows of t, the t-point transform in place, the chunked banded
   // solves to the unknowns in l order, permuted into y
-  if (use_hvdh(L, n) && is_rows(gin, n, Bi) && is_rows(gout, n, Bi) && free.size() >= 3) {
+  const bool rows_in = gin.stride == 1 && gin.s_mid == 0 &&
+                       (Bi == 1 ? gin.s_out == n : gin.s_in == n && gin.s_out == n * Bi);
+  if (use_hvdh(L, n) && rows_in && gout.s_mid == 0 && free.size() >= 3) {
     HvdhDesc hd;
     if (hvdh_params(L.dtype, n, batch, scale, hd)) {
       const int g = free[0], u = free[1];
@@ -1386,12 +1405,17 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
       pout.src = u;
       pout.dst = dst;
       pout.scale = scale;
+      pout.hv.o_stride = gout.stride;
+      pout.hv.o_bi = Bi;
+      pout.hv.o_in = gout.s_in;
+      pout.hv.o_out = gout.s_out;
       L.steps.push_back(pout);
       return;
     }
     if (L.route & FFTJH_FLAG_ROUTE_HVDH)
-      fail(FFTJH_ERR_UNSUPPORTED, "the HvdH route needs FP32, an odd length with a coprime smooth grid and "
-                                  "contiguous rows (length " + std::to_string(n) + ")");
+      fail(FFTJH_ERR_UNSUPPORTED, "the HvdH route needs FP32, contiguous rows and an odd length with a coprime "
+                                  "smooth grid, or a length whose odd part after its 13-smooth part has one "
+                                  "(length " + std::to_string(n) + ")");
   }
   // otherwise resample onto a smooth m (resample.jl:258-276)
   if (free.size() < 2) fail(FFTJH_ERR_UNSUPPORTED, "internal: no scratch for resampling");
@@ -2148,6 +2172,7 @@ static std::string describe(const Layout& L) {
       o << " n=" << s.hv.n << " t=" << s.hv.t << " taps=" << 2 * s.hv.h << " band=" << s.hv.B << " halo=" << s.hv.H
         << " batch=" << s.hv.batch;
       if (s.kind == ST_HVDH_SOLVE) o << " chunks=" << s.hv.chunks * s.hv.batch << " rows=" << hvdh_chunk_rows();
+      if (s.kind == ST_HVDH_PERMUTE && s.hv.o_stride != 1) o << " out_stride=" << s.hv.o_stride;
     } else if (s.kind == ST_RS_DECONV || s.kind == ST_RS_INTERP) {
       o << " n=" << s.n << " m=" << s.m << " w=" << s.w << " beta=" << s.beta;
     } else {

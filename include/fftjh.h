@@ -69,8 +69,11 @@ extern "C" {
 #define FFTJH_FLAG_ROUTE_RESAMPLE 0x20u
 /*   FFTJH_FLAG_ROUTE_HVDH       odd lengths: Gaussian resampling onto a smooth
  *                               t coprime to n and chunked banded solves (the
- *                               reference's HvdhPlan, hvdh.jl:154-241); FP32,
- *                               contiguous rows */
+ *                               reference's HvdhPlan, hvdh.jl:154-241); an
+ *                               even length (no grid coprime to it): one
+ *                               exact step over s around HvdH transforms of
+ *                               length p (SplitPlan, composite.jl:103-200);
+ *                               FP32, contiguous rows */
 #define FFTJH_FLAG_ROUTE_HVDH 0x40u
 
 typedef struct fftjh_plan_s* fftjh_plan;

@@ -14,6 +14,8 @@
  *   resample                   resample.jl:12-21, 33-55, 202-276, 392-424
  *   hvdh                       hvdh.jl:1-330, 560-688 (Gaussian resampling,
  *                              banded chunk solves)
+ *   split                      composite.jl:103-262 (SplitPlan: length-p
+ *                              HvdH sub-transforms, radix-s combine)
  *   oracle_rfft / irfft        real.jl:26-39, 80-211
  *   oracle_ndfft               ndplan.jl:57-118, 270-327
  *   chunk_range / baselines    threads.jl:15-38

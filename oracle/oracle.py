@@ -40,6 +40,7 @@ def lib():
             getattr(L, f"oracle_resample_{p}").argtypes = [i64, ctypes.c_int, T, vp, vp]
             getattr(L, f"oracle_hvdh_{p}").argtypes = [i64, ctypes.c_int, T, vp, vp]
             getattr(L, f"oracle_hvdh_halo_{p}").argtypes = [i64, ctypes.c_int, T, vp, vp, i64]
+            getattr(L, f"oracle_split_{p}").argtypes = [i64, ctypes.c_int, T, vp, vp]
             getattr(L, f"oracle_rfft_{p}").argtypes = [i64, vp, vp]
             getattr(L, f"oracle_irfft_{p}").argtypes = [i64, vp, vp, ctypes.c_int]
             getattr(L, f"oracle_ndfft_{p}").argtypes = [ctypes.c_int, vp, ctypes.c_int, T, vp, vp]
@@ -118,6 +119,22 @@ def hvdh(x, sign=-1, scale=1.0, prec="f64", halo=None):
         else:
             rc = getattr(lib(), f"oracle_hvdh_halo_{prec}")(n, sign, scale, _ptr(r), _ptr(o), halo)
         _check(rc, "hvdh")
+        y[i] = o
+    return y.reshape(x.shape)
+
+
+def split(x, sign=-1, scale=1.0, prec="f64"):
+    """1-D transform of each row by the reference's SplitPlan (composite.jl:
+    103-200): length-p HvdH sub-transforms of the decimated inputs, twiddled,
+    then the length-s transforms, s the 7-smooth part and p odd."""
+    x = _cplx(x, prec)
+    n = x.shape[-1]
+    rows = x.reshape(-1, n)
+    y = np.empty_like(rows)
+    for i in range(rows.shape[0]):
+        r = np.ascontiguousarray(rows[i])
+        o = np.empty_like(r)
+        _check(getattr(lib(), f"oracle_split_{prec}")(n, sign, scale, _ptr(r), _ptr(o)), "split")
         y[i] = o
     return y.reshape(x.shape)
 

@@ -733,6 +733,61 @@ int F(oracle_hvdh_halo)(int64_t n, int D, T scale, const T* x, T* y, int64_t hal
   return F(hvdh)(n, D, scale, x, y, halo);
 }
 
+/* ---------------------------------------------------------------- split */
+
+/* SplitPlan (composite.jl:103-179, call :181-200): n = s*p with s the
+ * 7-smooth part of n (composite.jl:45-52) and p odd.  Decimation in time:
+ * the s inputs x_r[j] = x[r + s*j] (the gather, :214-220) through length-p
+ * HvdH unnormalized, A[k1 + p*r] = w_n^(r*k1) Y_r[k1] (the twiddle,
+ * :222-262), then the length-s chain over r with p inner lanes, whose
+ * output X[k1 + p*k2] is natural order, scaled (:195-198).  The reference
+ * takes this route for even n with s < 16 (:138-142); any n with a smooth
+ * part and an odd cofactor is accepted here. */
+static int F(split)(int64_t n, int D, T scale, const T* x, T* y) {
+  static const int fs[4] = {2, 3, 5, 7};
+  int64_t s = 1;
+  for (int i = 0; i < 4; ++i)
+    while (n % (s * fs[i]) == 0) s *= fs[i];
+  const int64_t p = n / s;
+  if (s < 2 || p < 3 || p % 2 == 0) return -1;
+  F(Chain) ch;
+  if (F(chain_init)(&ch, s, D)) return -1;
+  T* sub = (T*)malloc(sizeof(T) * 4 * (size_t)p);
+  T* buf = (T*)malloc(sizeof(T) * 4 * (size_t)n);
+  T *xs = sub, *ys = sub + 2 * p;
+  T *are = buf, *aim = buf + n, *bre = buf + 2 * n, *bim = buf + 3 * n;
+  int rc = 0;
+  for (int64_t r = 0; r < s && rc == 0; ++r) {
+    for (int64_t j = 0; j < p; ++j) {
+      xs[2 * j] = x[2 * (r + s * j)];
+      xs[2 * j + 1] = x[2 * (r + s * j) + 1];
+    }
+    rc = F(hvdh)(p, D, (T)1, xs, ys, -1);
+    for (int64_t k1 = 0; k1 < p && rc == 0; ++k1) {
+      long double c, sn;
+      expi_exact(r * k1, n, D, &c, &sn);
+      const T wr = (T)c, wi = (T)sn, zr = ys[2 * k1], zi = ys[2 * k1 + 1];
+      are[k1 + p * r] = zr * wr - zi * wi;
+      aim[k1 + p * r] = zr * wi + zi * wr;
+    }
+  }
+  if (rc == 0) {
+    const int which = F(chain_run)(&ch, are, aim, bre, bim, p, D);
+    const T* rr = which ? bre : are;
+    const T* ri = which ? bim : aim;
+    for (int64_t k = 0; k < n; ++k) {
+      y[2 * k] = rr[k] * scale;
+      y[2 * k + 1] = ri[k] * scale;
+    }
+  }
+  free(sub);
+  free(buf);
+  F(chain_free)(&ch);
+  return rc;
+}
+
+int F(oracle_split)(int64_t n, int D, T scale, const T* x, T* y) { return F(split)(n, D, scale, x, y); }
+
 /* ------------------------------------------------------------------ real */
 
 /* rfft (real.jl:80-142): even n packs to n/2 complex + twist, odd n full */

@@ -69,3 +69,31 @@ def test_hvdh_in_place_and_bfft():
     yb = to_host(pb(to_dev(x))).reshape(batch, n)
     ref = O.fft(x.reshape(batch, n).astype(np.complex128), sign=1)
     assert rel_l2(yb, ref) <= 5.96e-6
+
+
+# the reference's SplitPlan (composite.jl:103-200): an even length n = s*p
+# has no HvdH grid coprime to it, so the route takes one length-s step and
+# runs the s*batch length-p transforms by HvdH, the output permutation
+# writing at stride s; s = 16 is CompositePlan's batched HvdH pass
+# (composite.jl:42-43, 94-96)
+@pytest.mark.parametrize("n,batch", [(2 * 65537, 3), (6 * 65537, 2), (10 * 100003, 1), (16 * 65537, 1),
+                                     (4 * 1000003, 1)])
+def test_split_route(n, batch):
+    print(f"hvdh split c2c f32 n={n} batch={batch}", flush=True)
+    x = inputs_c(n + 7, n * batch, "f32").reshape(batch, n)
+    pf, pi = _plans(n, batch)
+    d = pf.describe()
+    s = n // next(p for p in (65537, 100003, 1000003) if n % p == 0)
+    assert f"pass x->y n={s} " in d and f"out_stride={s}" in d, d
+    y = to_host(pf(to_dev(x))).reshape(batch, n)
+    x64 = x.astype(np.complex128)
+    assert rel_l2(y, O.fft(x64)) <= 5.96e-6
+    # the oracle's SplitPlan restatement (FP64) on the first row
+    assert rel_l2(y[0], O.split(x64[0])) <= 5.96e-6
+    xi = to_host(pi(to_dev(x))).reshape(batch, n)
+    assert rel_l2(xi, O.fft(x64, sign=1, scale=1.0 / n)) <= 5.96e-6
+    back = to_host(pi(to_dev(y.astype(np.complex64)))).reshape(batch, n)
+    assert rel_l2(back, x) <= 2 * 5.96e-6
+    z = to_dev(x)
+    assert pf(z, z) is z
+    assert np.array_equal(to_host(z).reshape(batch, n), y)

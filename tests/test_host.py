@@ -325,3 +325,30 @@ def test_hvdh_route_schedule():
     buf = ctypes.create_string_buffer(1 << 12)
     assert L.fftjh_describe_tuned(_lib.F64, 1, _lib.dims_array([65537]), 1, -1, ctypes.byref(tune), buf,
                                   len(buf)) == _lib.ERR_UNSUPPORTED
+
+
+def test_hvdh_split_schedule():
+    # an even length on the HvdH route (the reference's SplitPlan,
+    # composite.jl:103-200): the twiddled length-s pass, then HvdH over the
+    # s*batch length-p rows, the last permutation writing at stride s
+    L = _lib.lib()
+    import ctypes
+    tune = _lib.make_tuning((), (), _lib.FLAG_ROUTE_HVDH)
+    for n, batch, s in ((2 * 65537, 3, 2), (16 * 65537, 1, 16), (4 * 1000003, 2, 4)):
+        buf = ctypes.create_string_buffer(1 << 16)
+        assert L.fftjh_describe_tuned(_lib.F32, 1, _lib.dims_array([n]), batch, -1, ctypes.byref(tune), buf,
+                                      len(buf)) == _lib.OK
+        st = _passes(buf.value.decode())
+        assert st[0].startswith(f"pass x->y n={s} ") and f"twiddle=w_{n}" in st[0]
+        assert st[1].startswith("hvdh_permute y->") and f"batch={s * batch}" in st[1]
+        assert st[-1].startswith("hvdh_permute") and "->y " in st[-1] and st[-1].endswith(f"out_stride={s}")
+    # an odd length with a grid keeps the whole-length route
+    buf = ctypes.create_string_buffer(1 << 16)
+    assert L.fftjh_describe_tuned(_lib.F32, 1, _lib.dims_array([3 * 65537]), 1, -1, ctypes.byref(tune), buf,
+                                  len(buf)) == _lib.OK
+    assert _passes(buf.value.decode())[0].startswith("hvdh_permute x->")
+    # no grid for the cofactor (10,007) or FP64: refused; by default resampling
+    for dt, n in ((_lib.F32, 2 * 10007), (_lib.F64, 2 * 65537)):
+        assert L.fftjh_describe_tuned(dt, 1, _lib.dims_array([n]), 1, -1, ctypes.byref(tune), buf,
+                                      len(buf)) == _lib.ERR_UNSUPPORTED
+    assert "hvdh" not in _lib.describe_request(_lib.KIND_C2C, _lib.F32, [2 * 65537])

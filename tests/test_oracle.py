@@ -207,6 +207,25 @@ def test_hvdh_planted_mistakes():
     assert rel_l2(y[(np.arange(n) * 2) % n], ref) > 0.1
 
 
+@pytest.mark.parametrize("n", [2 * 1009, 6 * 331, 14 * 101, 10 * 1009, 2 * 10007, 6 * 65537])
+def test_split_matches_the_dft(n):
+    # SplitPlan (decimation in time around length-p HvdH) against the DFT:
+    # brute force in long double for small n, numpy's FFT beyond; a wrong
+    # twiddle sign or exponent, gather stride or output order fails here
+    rng = np.random.default_rng(n + 1)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = O.dft_naive(x, prec="ld").astype(np.complex128) if n <= 2100 else np.fft.fft(x)
+    assert rel_l2(O.split(x), ref) < 1e-14
+    assert rel_l2(O.split(x, sign=1, scale=1.0 / n), np.fft.ifft(x)) < 1e-14
+    assert rel_l2(O.split(x.astype(np.complex64), prec="f32"), ref) < 1e-6
+
+
+def test_split_rejects_lengths_without_an_odd_cofactor():
+    for n in (1024, 1009, 2 * 4):
+        with pytest.raises(ValueError):
+            O.split(np.ones(n, np.complex128))
+
+
 # ------------------------------------------------------------------ N-d (A11)
 
 @pytest.mark.parametrize("shape", [(16, 11), (11, 16), (12, 10, 14), (33, 17), (8, 1), (24, 40, 3)])
