@@ -139,7 +139,7 @@ struct LeanDesc {
   const void* ptw;
   double scale;
   int32_t mat;      // split-precision MFMA mode of the radix-16 stages (kMatNone: VALU)
-  int32_t pad_;
+  int32_t nt;       // bit 0: non-temporal loads of x, bit 1: non-temporal stores of y
   const void* matp; // that mode's operand table for the pass's direction
 };
 

@@ -53,6 +53,25 @@ namespace {
 
 __device__ __forceinline__ int lp(int f) { return f + (f >> 4); }
 
+// non-temporal (streaming) element load, and the store of an output
+// (non-temporal when bit 1 of NT is set)
+__device__ __forceinline__ float2 ld_nt2(const float2* p) {
+  using NV = float __attribute__((ext_vector_type(2)));
+  const NV v = __builtin_nontemporal_load(reinterpret_cast<const NV*>(p));
+  return make_float2(v.x, v.y);
+}
+template <int NT> __device__ __forceinline__ void st_out(float2* p, const float2 v) {
+  if constexpr (NT & 2) {
+    using NV = float __attribute__((ext_vector_type(2)));
+    NV w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<NV*>(p));
+  } else {
+    *p = v;
+  }
+}
+
 // SPLIT = 1: the exchanges move the real parts, then the imaginary parts,
 // through an LDS array of floats: half the LDS of a float2 exchange (twice
 // the groups per CU) for two more barriers per exchange.
@@ -67,7 +86,9 @@ __device__ __forceinline__ int lp(int f) { return f + (f >> 4); }
 // with no LDS traffic and no barrier.  The pairs (k1 = 4b + s, 4b + 2 + s)
 // swap across halves and (4b, 4b + 1), (4b + 2, 4b + 3) across neighbouring
 // rows, after which register q holds the stage-2 input q of the LDS path.
-template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0>
+// NT: bit 0 loads x non-temporally (streaming: the plan does not read it
+// again), bit 1 stores y non-temporally.
+template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0, int NT = 0>
 __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restrict__ x, float2* y,
                                               const float2* __restrict__ tws) {
   constexpr int R2 = N / 16;   // threads per column
@@ -129,7 +150,10 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
     const float2* xb = x + G.ib;
     const int lo = c + jt * static_cast<int>(d.istride);
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = (xb + static_cast<int64_t>(R2) * u * d.istride)[lo];
+    for (int u = 0; u < 16; ++u) {
+      const float2* p = xb + static_cast<int64_t>(R2) * u * d.istride + lo;
+      v[u] = (NT & 1) ? ld_nt2(p) : *p;
+    }
   };
 
   const uint32_t g = blockIdx.x;
@@ -254,7 +278,7 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const float2 a = (d.psh || scaled) ? cmul<float>(o[i], w) : o[i];
-        (yb + i * d.ostride)[tid] = a;
+        st_out<NT>(yb + i * d.ostride + tid, a);
       }
     } else {
       float2* yb = y + G.ob;
@@ -265,13 +289,13 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
           const float2 val = cmul<float>(t[u], u ? cmul<float>(ws, lsu[u]) : ws);
-          (yb + static_cast<int64_t>(R2) * u * d.ostride)[lo] = val;
+          st_out<NT>(yb + static_cast<int64_t>(R2) * u * d.ostride + lo, val);
         }
       } else {
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
           const float2 val = scaled ? cscale<float>(t[u], scale) : t[u];
-          (yb + static_cast<int64_t>(R2) * u * d.ostride)[lo] = val;
+          st_out<NT>(yb + static_cast<int64_t>(R2) * u * d.ostride + lo, val);
         }
       }
     }
@@ -538,6 +562,19 @@ static int lean_xlane() {
 
 template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0>
 hipError_t launch_k(const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t st) {
+  // streaming variants for the split-exchange kernel only
+  if constexpr (SPLIT == 1) {
+    if (d.nt == 1) {
+      hipLaunchKernelGGL((k_lean<DIR, N, ROWS, SPLIT, XLANE, 1>), dim3(d.groups), dim3(N), 0, st, d,
+                         static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+      return hipGetLastError();
+    }
+    if (d.nt == 2) {
+      hipLaunchKernelGGL((k_lean<DIR, N, ROWS, SPLIT, XLANE, 2>), dim3(d.groups), dim3(N), 0, st, d,
+                         static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((k_lean<DIR, N, ROWS, SPLIT, XLANE>), dim3(d.groups), dim3(N), 0, st, d,
                      static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
   return hipGetLastError();
