@@ -30,7 +30,7 @@ EXPORTS = (
 
 class RaggedTuning(ctypes.Structure):
     _fields_ = [("groups", ctypes.c_int32), ("radix_pref", ctypes.c_int32), ("tile_max", ctypes.c_int32),
-                ("pad_", ctypes.c_int32), ("costs", ctypes.c_void_p)]
+                ("pass_min", ctypes.c_int32), ("costs", ctypes.c_void_p)]
 
 
 class PassTuning(ctypes.Structure):

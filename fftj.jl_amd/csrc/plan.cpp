@@ -347,6 +347,7 @@ struct Step {
   DevBufPtr tw, desc, aux, mat_tab, ptw;
   std::shared_ptr<struct RaggedData> rg;  // ST_RAGGED: items of at most kRaggedTile points
   std::shared_ptr<struct RaggedData> rg2; // ST_RAGGED: longer items, on the large-item kernel
+  std::shared_ptr<struct RaggedData> rg3; // ST_RAGGED: long smooth items as two tile-granular passes
 };
 
 struct Layout {
@@ -1888,7 +1889,8 @@ struct RaggedData {
   std::vector<CorrJob> corr_jobs;
   std::vector<double> cost;     // per item
   int64_t counts[4] = {0, 0, 0, 0};
-  DevBufPtr d_items, d_wgptr, d_wgitems, d_scheds, d_tw, d_corr, d_ws, d_wgws;
+  std::vector<RaggedTile> tiles_a, tiles_b;  // tile-granular passes: phase A and phase B tiles
+  DevBufPtr d_items, d_wgptr, d_wgitems, d_scheds, d_tw, d_corr, d_ws, d_wgws, d_tiles_a, d_tiles_b;
 };
 
 static std::vector<int> radices_pref(int64_t n, int pref) {
@@ -2021,6 +2023,22 @@ static Layout layout_ragged(int dtype, int64_t count, const int64_t* sizes, cons
   R2->big = 1;
   R2->threads = ragged_big_threads(dtype);
   R2->tile_max = ragged_big_tile(dtype);
+  // smooth items longer than pass_min run as two tile-granular passes on the
+  // small kernel's tiles (phase A: every column tile of every such item in
+  // one launch, phase B every row tile in a second); the others stay on the
+  // item-granular kernels.  pass_min: the tuning's, else FFTJH_RAGGED_PASS_MIN,
+  // else the default; negative: never.
+  static const int pass_env = [] {
+    const char* e = std::getenv("FFTJH_RAGGED_PASS_MIN");
+    return e ? std::atoi(e) : 0;
+  }();
+  int pass_min = kRaggedPassMin;
+  if (pass_env) pass_min = pass_env;
+  if (tune && tune->pass_min) pass_min = tune->pass_min;
+  auto R3 = std::make_shared<RaggedData>();
+  R3->dir = sign;
+  R3->radix_pref = R->radix_pref;
+  R3->big = 2;
   const int64_t nmax = dtype == FFTJH_F32 ? 65536 : 32768;
   int64_t extent = 0, off = 0;
   std::vector<double> cost_small, cost_big;
@@ -2037,7 +2055,8 @@ static Layout layout_ragged(int dtype, int64_t count, const int64_t* sizes, cons
     const bool smooth = factor13(n, a, cnt);
     const RsParams rp = smooth ? RsParams{n, 0, 0.0} : rs_params(dtype, n);
     const int64_t e = smooth ? n : rp.m;  // the length one tile must hold
-    RaggedData& C = (big_env && e > R->tile_max) ? *R2 : *R;
+    const bool as_pass = smooth && pass_min >= 0 && n > pass_min && n > R3->tile_max;
+    RaggedData& C = as_pass ? *R3 : (big_env && e > R->tile_max) ? *R2 : *R;
     RaggedItem it{};
     it.off = o;
     it.n = static_cast<int>(n);
@@ -2049,6 +2068,13 @@ static Layout layout_ragged(int dtype, int64_t count, const int64_t* sizes, cons
       } else {
         it.kind = kRaggedTwoLevel;
         ragged_two_level(C, it, n);
+        if (as_pass) {
+          const int32_t idx = static_cast<int32_t>(C.items.size());
+          it.woff = C.ws_total;
+          C.ws_total += n;
+          for (int c0 = 0; c0 < it.n1; c0 += it.cA) C.tiles_a.push_back({idx, c0});
+          for (int r0 = 0; r0 < it.n2; r0 += it.cB) C.tiles_b.push_back({idx, r0});
+        }
       }
     } else {
       it.m = static_cast<int>(rp.m);
@@ -2093,6 +2119,7 @@ static Layout layout_ragged(int dtype, int64_t count, const int64_t* sizes, cons
   st.dir = sign;
   st.rg = R;
   if (!R2->items.empty()) st.rg2 = R2;
+  if (!R3->items.empty()) st.rg3 = R3;
   L.steps.push_back(st);
   return L;
 }
@@ -2177,15 +2204,20 @@ static std::string describe(const Layout& L) {
     } else if (s.kind == ST_RAGGED) {
       const RaggedData& R = *s.rg;
       const RaggedData* B = s.rg2.get();
-      auto cnt = [&](int k) { return R.counts[k] + (B ? B->counts[k] : 0); };
-      o << " transforms=" << R.items.size() + (B ? B->items.size() : 0) << " direct=" << cnt(0)
-        << " two_level=" << cnt(1) << " resample=" << cnt(2) << " resample_two_level=" << cnt(3)
-        << " tile_lengths=" << R.scheds.size() + (B ? B->scheds.size() : 0) << " radix_pref=" << R.radix_pref
-        << " tile_max=" << R.tile_max;
+      const RaggedData* Pp = s.rg3.get();
+      auto cnt = [&](int k) { return R.counts[k] + (B ? B->counts[k] : 0) + (Pp ? Pp->counts[k] : 0); };
+      o << " transforms=" << R.items.size() + (B ? B->items.size() : 0) + (Pp ? Pp->items.size() : 0)
+        << " direct=" << cnt(0) << " two_level=" << cnt(1) << " resample=" << cnt(2)
+        << " resample_two_level=" << cnt(3)
+        << " tile_lengths=" << R.scheds.size() + (B ? B->scheds.size() : 0) + (Pp ? Pp->scheds.size() : 0)
+        << " radix_pref=" << R.radix_pref << " tile_max=" << R.tile_max;
       if (R.groups) o << " workgroups=" << R.groups;
       if (B)
         o << " large_transforms=" << B->items.size() << " large_tile=" << B->tile_max << " large_direct=" << B->counts[0]
           << " large_two_level=" << B->counts[1] + B->counts[3];
+      if (Pp)
+        o << " pass_transforms=" << Pp->items.size() << " pass_tiles=" << Pp->tiles_a.size() << "+"
+          << Pp->tiles_b.size();
     } else if (s.kind == ST_HVDH_PERMUTE || s.kind == ST_HVDH_SPREAD || s.kind == ST_HVDH_SOLVE) {
       o << " n=" << s.hv.n << " t=" << s.hv.t << " taps=" << 2 * s.hv.h << " band=" << s.hv.B << " halo=" << s.hv.H
         << " batch=" << s.hv.batch;
@@ -2376,6 +2408,28 @@ static void materialize(fftjh_plan_s& P) {
       hip_check(hipDeviceSynchronize(), "ragged tables");
       R.d_ws = std::make_shared<DevBuf>(std::max<int64_t>(R.ws_total, 1) * es);
      }
+     if (s.rg3) {
+      RaggedData& R = *s.rg3;
+      auto upload = [&](const void* src, size_t bytes) {
+        auto b = std::make_shared<DevBuf>(std::max<size_t>(bytes, 16));
+        if (bytes) hip_check(hipMemcpy(b->p, src, bytes, hipMemcpyHostToDevice), "ragged table upload");
+        return b;
+      };
+      if (R.tiles_a.size() > (size_t(1) << 31) - 1 || R.tiles_b.size() > (size_t(1) << 31) - 1)
+        fail(FFTJH_ERR_UNSUPPORTED, "too many ragged pass tiles");
+      R.d_items = upload(R.items.data(), R.items.size() * sizeof(RaggedItem));
+      R.d_scheds = upload(R.scheds.data(), R.scheds.size() * sizeof(RaggedSched));
+      R.d_tiles_a = upload(R.tiles_a.data(), R.tiles_a.size() * sizeof(RaggedTile));
+      R.d_tiles_b = upload(R.tiles_b.data(), R.tiles_b.size() * sizeof(RaggedTile));
+      R.d_tw = std::make_shared<DevBuf>(std::max<int64_t>(R.tw_total, 1) * es);
+      for (const auto& j : R.stage_jobs)
+        hip_check(launch_tw_stage(dt, static_cast<char*>(R.d_tw->p) + j.off * es, j.ns, j.R, R.dir, 0), "twiddles");
+      for (const auto& j : R.pow_jobs)
+        hip_check(launch_tw_pow(dt, static_cast<char*>(R.d_tw->p) + j.off * es, j.count, j.mult, j.L, R.dir, 0),
+                  "twiddles");
+      hip_check(hipDeviceSynchronize(), "ragged tables");
+      R.d_ws = std::make_shared<DevBuf>(std::max<int64_t>(R.ws_total, 1) * es);
+     }
     } else if (s.kind == ST_R2C_TWIST || s.kind == ST_C2R_PRE) {
       // w_k = exp(dir*2*pi*i*k/n), k = 0..h (real.jl:26-39)
       const int64_t cnt = s.h + 1, n = s.n;
@@ -2471,7 +2525,9 @@ static int64_t static_layout(const fftjh_plan_s& P, int64_t role_off[4], int64_t
   *ws_off = at;
   for (const Step& s : P.L.steps)
     if (s.kind == ST_RAGGED)
-      at += align256(std::max(s.rg->ws_total, s.rg2 ? s.rg2->ws_total : int64_t(0)) * es);
+      at += align256(std::max({s.rg->ws_total, s.rg2 ? s.rg2->ws_total : int64_t(0),
+                                s.rg3 ? s.rg3->ws_total : int64_t(0)}) *
+                     es);
   return at;
 }
 
@@ -2536,6 +2592,24 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
         rl.wg_ws = R.d_wgws->p;
         const hipError_t e = launch_ragged(dt, s.dir, rl, st);
         if (e != hipSuccess) return e;
+      }
+      if (s.rg3) {
+        // then the long items' column tiles, then their row tiles
+        const RaggedData& R = *s.rg3;
+        for (int ph = 0; ph < 2; ++ph) {
+          RaggedPassLaunch pl;
+          pl.phase = ph;
+          pl.tiles = static_cast<int>(ph ? R.tiles_b.size() : R.tiles_a.size());
+          pl.items = R.d_items->p;
+          pl.tile_list = ph ? R.d_tiles_b->p : R.d_tiles_a->p;
+          pl.scheds = R.d_scheds->p;
+          pl.tw = R.d_tw->p;
+          pl.x = ptr[s.src];
+          pl.y = ptr[s.dst];
+          pl.ws = ws ? ws : R.d_ws->p;
+          const hipError_t e = launch_ragged_pass(dt, s.dir, pl, st);
+          if (e != hipSuccess) return e;
+        }
       }
       return hipSuccess;
     }

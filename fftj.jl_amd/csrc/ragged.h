@@ -8,6 +8,8 @@ namespace fftjh {
 
 constexpr int kRaggedThreads = 256;  // workgroup size of the small-item launch
 constexpr int kRaggedTile = 4096;    // complex elements one LDS tile holds (small items)
+// smooth items longer than this run as two tile-granular passes (default)
+constexpr int kRaggedPassMin = 16384;
 // the large-item launch: one workgroup per CU whose LDS tile holds 16384
 // complex FP32 (8192 FP64) elements, so lengths up to that run in one tile
 // and longer ones take 64-column tiles in their two phases
@@ -38,6 +40,27 @@ struct RaggedItem {
   int64_t corr_off;  // into the correction pool (reals)
   double beta;
   double scale;
+  int64_t woff;      // tile-granular passes: the item's region of the workspace (n elements)
+};
+
+// One tile of a tile-granular pass over long items: phase A tiles are
+// column blocks c0 of an item's n1 columns, phase B tiles row blocks r0 of
+// its n2 rows.
+struct RaggedTile {
+  int32_t item;
+  int32_t first;  // c0 (phase A) or r0 (phase B)
+};
+
+struct RaggedPassLaunch {
+  int phase;      // 0: columns of x into the workspace, 1: rows of the workspace into y
+  int tiles;      // one workgroup per tile
+  const void* items;
+  const void* tile_list;
+  const void* scheds;
+  const void* tw;
+  const void* x;
+  void* y;
+  void* ws;
 };
 
 struct RaggedLaunch {
@@ -56,6 +79,7 @@ struct RaggedLaunch {
 };
 
 hipError_t launch_ragged(int dtype, int dir, const RaggedLaunch& L, hipStream_t st);
+hipError_t launch_ragged_pass(int dtype, int dir, const RaggedPassLaunch& L, hipStream_t st);
 int ragged_resident_groups(int dtype, int dir, int big);
 
 }  // namespace fftjh

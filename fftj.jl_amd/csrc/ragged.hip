@@ -24,6 +24,7 @@
 #include "butterflies.h"
 #include "common.h"
 #include "launch.h"
+#include "pass_kernel.h"
 #include "passcommon.h"
 #include "ragged.h"
 #include "stages.h"
@@ -272,6 +273,218 @@ k_ragged(const RaggedItem* __restrict__ items, const int32_t* __restrict__ wg_pt
       }
     }
   }
+}
+
+// Tile-granular passes over the long two-level items: one workgroup per
+// tile, every tile of every item in one launch per phase, so that small
+// workgroups (several per CU) share the load at tile granularity instead of
+// one workgroup per CU walking whole items.  Phase A: the column block
+// c0..c0+C-1 of an item (columns j1 at stride n1, length n2) from x, twiddle
+// w_n^{j1*k2}, into the item's workspace region mid[j1 + n1*k2].  Phase B:
+// the row block r0..r0+C-1 of mid (rows k2, length n1) to y[k2 + n2*k1].
+// The first stage of a tile's chain runs on the registers its HBM loads
+// fill and the last one stores from registers (the pass kernel's fused
+// edges; the phase-A twiddle from an FP64 start and steps), so a tile of
+// two stages makes one LDS round trip.
+template <typename T, int DIR>
+__device__ __forceinline__ void tile_first(const int radix, const int n, const int C, TwPtr<T> xb, const int64_t ist,
+                                           const bool valid, LdsPtr<T> lds, const int c, const int jt,
+                                           const int jstep) {
+#define FFTJH_RF(R_) first_stage<T, DIR, R_>(n, C, xb, ist, valid, lds, c, jt, jstep)
+  switch (radix) {
+    case 2: FFTJH_RF(2); break;
+    case 3: FFTJH_RF(3); break;
+    case 4: FFTJH_RF(4); break;
+    case 5: FFTJH_RF(5); break;
+    case 7: FFTJH_RF(7); break;
+    case 8: FFTJH_RF(8); break;
+    case 11: FFTJH_RF(11); break;
+    case 13: FFTJH_RF(13); break;
+    default: FFTJH_RF(16); break;
+  }
+#undef FFTJH_RF
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void tile_last(const StageDesc& sl, const int n, const int C, LdsPtr<T> lds, TwPtr<T> tw,
+                                          GPtr<T> yb, const int64_t ost, const bool valid, const T scale,
+                                          const bool twon, const double2 w0, const double2 Sr, const double2 Sb,
+                                          const int c, const int jt, const int jstep) {
+  const bool scaled = scale != T(1);
+#define FFTJH_RL(R_) \
+  last_stage<T, DIR, R_>(n, C, sl, lds, tw, yb, ost, valid, scale, scaled, twon, w0, Sr, Sb, c, jt, jstep)
+  switch (sl.radix) {
+    case 2: FFTJH_RL(2); break;
+    case 3: FFTJH_RL(3); break;
+    case 4: FFTJH_RL(4); break;
+    case 5: FFTJH_RL(5); break;
+    case 7: FFTJH_RL(7); break;
+    case 8: FFTJH_RL(8); break;
+    case 11: FFTJH_RL(11); break;
+    case 13: FFTJH_RL(13); break;
+    default: FFTJH_RL(16); break;
+  }
+#undef FFTJH_RL
+}
+
+// the stages [FROM, S-1) of a chain in LDS (FROM = 1: after a fused first stage)
+template <typename T, int DIR, int kRT, int FROM = 1>
+__device__ void tile_mid(LdsPtr<T> lds, const RaggedSched& sc, int C, int logC, TwPtr<T> tw) {
+  const int tid = static_cast<int>(threadIdx.x);
+  const int c = tid & (C - 1), jt = tid >> logC, jstep = kRT >> logC;
+  for (int s = FROM; s < sc.nstages - 1; ++s) {
+    const StageDesc& st = sc.st[s];
+    switch (st.radix) {
+      case 2: run_stage<T, DIR, 2>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 3: run_stage<T, DIR, 3>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 4: run_stage<T, DIR, 4>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 5: run_stage<T, DIR, 5>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 7: run_stage<T, DIR, 7>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 8: run_stage<T, DIR, 8>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 11: run_stage<T, DIR, 11>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 13: run_stage<T, DIR, 13>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      default: run_stage<T, DIR, 16>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+    }
+  }
+}
+
+#ifndef FFTJH_RAGGED_PASS_WAVES
+#define FFTJH_RAGGED_PASS_WAVES 4
+#endif
+// Coalesced load of a row tile: the tile's rows r0..r0+C-1 of length n1 are
+// `total` contiguous elements at src (C*n1 in all; the rows past the item's
+// end load as zeros); element f = row*n1 + e goes to LDS slot e*C + row.
+// (Lanes along the rows: one row per lane would touch a line per lane.)
+template <typename T, int kRT, int kTile>
+__device__ __noinline__ void tile_load_rows(GPtr<T> src_, const int n1_, const int C_, const int total_,
+                                            LdsPtr<T> lds_) {
+  constexpr int kU = kTile / kRT;
+  const int tid = static_cast<int>(threadIdx.x);
+  const GPtr<T> src = uni_ptr(src_);
+  const int n1 = uni(n1_), C = uni(C_), total = uni(total_);
+  const LdsPtr<T> lds = uni_lds<T>(lds_);
+  const int all = C * n1;
+  // row = f / n1 by a multiply-high: exact for f < 2^20 (f, n1 <= kTile)
+  const unsigned magic = 0xffffffffu / static_cast<unsigned>(n1) + 1u;
+  V2<T> v[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int f = tid + u * kRT;
+    v[u] = f < total ? ldv<T>(src, f) : mk<T>(T(0), T(0));
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int f = tid + u * kRT;
+    if (f < all) {
+      const int row = static_cast<int>(__umulhi(static_cast<unsigned>(f), magic));
+      const int e = f - row * n1;
+      stv(lds, lpad(e * C + row), v[u]);
+    }
+  }
+}
+
+// w_n^e = exp(DIR*2*pi*i*e/n) in FP64, e reduced exactly mod n
+template <int DIR>
+__device__ __forceinline__ double2 ragged_w(uint64_t e, int n) {
+  e %= static_cast<uint64_t>(n);
+  double sv, cv;
+  sincospi(2.0 * static_cast<double>(e) / static_cast<double>(n), &sv, &cv);
+  return make_double2(cv, DIR * sv);
+}
+
+template <typename T, int DIR, int PH>
+__global__ void __launch_bounds__(kRaggedThreads) __attribute__((amdgpu_waves_per_eu(FFTJH_RAGGED_PASS_WAVES)))
+k_ragged_pass(const RaggedItem* __restrict__ items, const RaggedTile* __restrict__ tiles,
+              const RaggedSched* __restrict__ scheds, const V2<T>* tw, const V2<T>* x, V2<T>* y, V2<T>* ws) {
+  constexpr int kRT = kRaggedThreads, kTile = kRaggedTile;
+  extern __shared__ __align__(16) unsigned char smem[];
+  V2<T>* lds = reinterpret_cast<V2<T>*>(smem);
+  const LdsPtr<T> l3 = as_lds<T>(lds);
+  const TwPtr<T> t1 = as_tw<T>(tw);
+  const int tid = static_cast<int>(threadIdx.x);
+  const RaggedTile tl = tiles[blockIdx.x];
+  const RaggedItem& it = items[tl.item];
+  const int n1 = it.n1, n2 = it.n2;
+  V2<T>* mid = ws + it.woff;
+  if constexpr (PH == 0) {
+    const int C = it.cA, logC = it.logcA;
+    const int c = tid & (C - 1), jt = tid >> logC, jstep = kRT >> logC;
+    const int c0 = tl.first;
+    const RaggedSched& sc = scheds[it.sA];
+    const bool valid = c0 + c < n1;
+    if (sc.nstages < 2) {
+      tile_load<T, kRT, kTile>(as_tw<T>(x + it.off), c0 + c + static_cast<int64_t>(n1) * jt,
+                               static_cast<int64_t>(n1) * jstep, jt, jstep, n2, valid, l3);
+      __syncthreads();
+      tile_stages<T, DIR, kRT>(l3, sc, C, logC, t1);
+      tile_store_tw<T, kRT, kTile>(l3, as_g<T>(mid), t1 + it.tw_hi, t1 + it.tw_lo, it.lo_bits, n1, C * n2, C, logC,
+                                   c0);
+      return;
+    }
+    tile_first<T, DIR>(sc.st[0].radix, n2, C, as_tw<T>(x + it.off + c0 + c), n1, valid, l3, c, jt, jstep);
+    tile_mid<T, DIR, kRT>(l3, sc, C, logC, t1);
+    // outputs k2 = j + r*ns of column j1 = c0 + c times w_n^(j1*k2)
+    const StageDesc& sl = sc.st[sc.nstages - 1];
+    const int n = it.n;
+    const uint64_t j1 = static_cast<uint64_t>(c0 + c);
+    const double2 w0 = ragged_w<DIR>(j1 * static_cast<uint64_t>(jt), n);
+    const double2 Sr = ragged_w<DIR>(j1 * static_cast<uint64_t>(sl.ns), n);
+    const double2 Sb = ragged_w<DIR>(j1 * static_cast<uint64_t>(jstep), n);
+    tile_last<T, DIR>(sl, n2, C, l3, t1, as_g<T>(mid + c0 + c), n1, valid, T(1), true, w0, Sr, Sb, c, jt, jstep);
+  } else {
+    const int C = it.cB, logC = it.logcB;
+    const int c = tid & (C - 1), jt = tid >> logC, jstep = kRT >> logC;
+    const int r0 = tl.first;
+    const RaggedSched& sc = scheds[it.sB];
+    const bool valid = r0 + c < n2;
+    const T scale = static_cast<T>(it.scale);
+    V2<T>* yi = y + it.off;
+    if (sc.nstages < 2) {
+      tile_load<T, kRT, kTile>(as_g<T>(mid), static_cast<int64_t>(r0 + c) * n1 + jt, jstep, jt, jstep, n1, valid, l3);
+      __syncthreads();
+      tile_stages<T, DIR, kRT>(l3, sc, C, logC, t1);
+      for (int f = tid; f < C * n1; f += kRT) {
+        const int cc = f & (C - 1), k1 = f >> logC;
+        const int row = r0 + cc;
+        if (row < n2) yi[row + static_cast<int64_t>(n2) * k1] = cscale<T>(lds[lpad(k1 * C + cc)], scale);
+      }
+      return;
+    }
+    const int rows = n2 - r0 < C ? n2 - r0 : C;
+    tile_load_rows<T, kRT, kTile>(as_g<T>(mid + static_cast<int64_t>(r0) * n1), n1, C, rows * n1, l3);
+    __syncthreads();
+    tile_mid<T, DIR, kRT, 0>(l3, sc, C, logC, t1);
+    const double2 one = make_double2(1.0, 0.0);
+    tile_last<T, DIR>(sc.st[sc.nstages - 1], n1, C, l3, t1, as_g<T>(yi + r0 + c), n2, valid, scale, false, one, one,
+                      one, c, jt, jstep);
+  }
+}
+
+template <typename T, int DIR, int PH>
+static hipError_t launch_pass_t(const RaggedPassLaunch& L, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ragged_pass<T, DIR, PH>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+    attr = true;
+  }
+  const size_t lds = static_cast<size_t>(lds_elems(kRaggedTile)) * sizeof(V2<T>);
+  hipLaunchKernelGGL((k_ragged_pass<T, DIR, PH>), dim3(L.tiles), dim3(kRaggedThreads), lds, st,
+                     static_cast<const RaggedItem*>(L.items), static_cast<const RaggedTile*>(L.tile_list),
+                     static_cast<const RaggedSched*>(L.scheds), static_cast<const V2<T>*>(L.tw),
+                     static_cast<const V2<T>*>(L.x), static_cast<V2<T>*>(L.y), static_cast<V2<T>*>(L.ws));
+  return hipGetLastError();
+}
+
+template <typename T, int DIR>
+static hipError_t launch_pass_d(const RaggedPassLaunch& L, hipStream_t st) {
+  return L.phase == 0 ? launch_pass_t<T, DIR, 0>(L, st) : launch_pass_t<T, DIR, 1>(L, st);
+}
+
+hipError_t launch_ragged_pass(int dtype, int dir, const RaggedPassLaunch& L, hipStream_t st) {
+  if (L.tiles <= 0) return hipSuccess;
+  if (dtype == 0) return dir < 0 ? launch_pass_d<float, -1>(L, st) : launch_pass_d<float, 1>(L, st);
+  return dir < 0 ? launch_pass_d<double, -1>(L, st) : launch_pass_d<double, 1>(L, st);
 }
 
 template <typename T, int DIR, int kRT, int kTile>

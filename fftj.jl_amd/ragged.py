@@ -68,9 +68,12 @@ def plan_ragged(T, sizes, offsets=None, *, sign=-1, normalized=True, tuning=None
     return RaggedPlan(_code(T), sizes, offsets, sign, normalized, tuning, costs)
 
 
-def make_tuning(groups=0, radix_pref=0, tile_max=0):
+def make_tuning(groups=0, radix_pref=0, tile_max=0, pass_min=0):
+    """pass_min: smooth transforms longer than this run as two tile-granular
+    passes (0: the planner's default, -1: never)."""
     t = _lib.RaggedTuning()
     t.groups, t.radix_pref, t.tile_max = int(groups), int(radix_pref), int(tile_max)
+    t.pass_min = int(pass_min)
     return t
 
 

@@ -1,6 +1,7 @@
 """Autotuner for ragged plans: builds the candidate plans of a small search
-space (radix tower 16 or 8, largest one-tile length, persistent workgroup
-count), times each with HIP events on the current stream, keeps the fastest.
+space (radix tower 16 or 8, largest one-tile length, the length above which
+transforms run as two tile-granular passes), times each with HIP events on
+the current stream, keeps the fastest.
 An optional cost model orders the candidates and supplies the per-transform
 costs that balance the workgroups."""
 import itertools
@@ -26,9 +27,11 @@ def time_plan(plan, y, x, iters=5, warmup=1):
     return a.elapsed_time(b) / iters
 
 
-def ragged_candidates(resident=1024):
-    for rp, tm, g in itertools.product((16, 8), (4096, 2048), (0, resident // 2)):
-        yield {"radix_pref": rp, "tile_max": tm, "groups": g}
+def ragged_candidates():
+    """radix tower, largest one-tile length of the small-item launch, and the
+    length above which smooth transforms run as two tile-granular passes."""
+    for rp, tm, pm in itertools.product((16, 8), (4096, 2048), (16384, 4096)):
+        yield {"radix_pref": rp, "tile_max": tm, "groups": 0, "pass_min": pm}
 
 
 def model_costs(sizes, planner=None, tile_max=4096):

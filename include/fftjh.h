@@ -132,7 +132,8 @@ typedef struct fftjh_ragged_tuning {
   int32_t groups;        /* persistent workgroups (0 = all resident) */
   int32_t radix_pref;    /* 16 or 8: radix of the power-of-two tower */
   int32_t tile_max;      /* largest length done in one LDS tile, 256..4096 */
-  int32_t pad_;
+  int32_t pass_min;      /* smooth transforms longer than this run as two
+                            tile-granular passes (0 = planner, -1 = never) */
   const double* costs;   /* per-transform predicted cost for the workgroup
                             assignment (NULL = the analytic model) */
 } fftjh_ragged_tuning;

@@ -43,6 +43,41 @@ def test_ragged_all_kinds(prec):
     assert " large_transforms=" in d and (" large_tile=16384" if prec == "f32" else " large_tile=8192") in d
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("pass_min", [4096, 16384])
+def test_ragged_tile_passes(prec, pass_min):
+    """Long smooth transforms as two tile-granular passes (every column tile
+    of every such item in one launch, then every row tile), beside items on
+    the item-granular launches, both directions, and in place."""
+    sizes = np.array([64, 4096, 4608, 6000, 8192, 16384, 16807, 20000, 32768, 257, 8191, 100, 30000],
+                     dtype=np.int64)
+    if prec == "f32":
+        sizes = np.concatenate([sizes, [65536, 50000, 65536 - 2 ** 14, 59049]])
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    x = inputs_c(11, int(sizes.sum()), prec)
+    for sign in (-1, 1):
+        p = F.plan_ragged(torch_dtype(prec), sizes, sign=sign, tuning=F.ragged.make_tuning(pass_min=pass_min))
+        d = p.describe()
+        want = int(np.sum([n > max(pass_min, 4096) for n in sizes if n not in (257, 8191)]))
+        assert f" pass_transforms={want} " in d, d
+        y = to_host(p(to_dev(x)))
+        for n, o in zip(sizes.tolist(), offs.tolist()):
+            ref = O.fft(x[o:o + n].astype(np.complex128), sign=sign, scale=1.0 if sign < 0 else 1.0 / n)
+            assert rel_l2(y[o:o + n], ref) <= tol_for(prec, n), (prec, pass_min, sign, n)
+        xd = to_dev(x)
+        assert p(xd, xd) is xd
+        assert np.array_equal(to_host(xd), y)
+    # the planner's default and never: the same transforms to the tolerance
+    for pm in (0, -1):
+        p = F.plan_ragged(torch_dtype(prec), sizes, tuning=F.ragged.make_tuning(pass_min=pm))
+        if pm < 0:
+            assert " pass_transforms=" not in p.describe()
+        y2 = to_host(p(to_dev(x)))
+        ref = to_host(F.plan_ragged(torch_dtype(prec), sizes, tuning=F.ragged.make_tuning(pass_min=pass_min))(to_dev(x)))
+        for n, o in zip(sizes.tolist(), offs.tolist()):
+            assert rel_l2(y2[o:o + n], ref[o:o + n]) <= tol_for(prec, n), (prec, pm, n)
+
+
 def test_ragged_inverse_and_inplace():
     sizes = [64, 4096, 8192, 257, 8191, 6000]
     p, x, y = _check("f32", sizes, sign=1)
