@@ -127,9 +127,10 @@ struct PassDesc {
 // (w_P^(i*2^ptw_sh) for i < ptw_lo, then w_P^i).
 struct LeanDesc {
   int32_t n;
-  int32_t rows;
+  int32_t rows;     // outputs as contiguous rows (an LDS transposition)
   int32_t groups;
   int32_t D0, D1;
+  int32_t rin;      // inputs as contiguous rows at stride istride (an LDS transposition)
   int64_t in_s0, in_s1, in_s2, istride;
   int64_t out_s0, out_s1, out_s2, ostride;
   int32_t psh;

@@ -2172,13 +2172,13 @@ static std::string describe(const Layout& L) {
       o << "  pass " << roles[s.src] << "->" << roles[s.dst] << " n=" << d.n << " radices=16," << d.n / 16
         << " kernel=lean" << (d.mat == kMatF16 ? " kernel=mfma_f16" : d.mat == kMatBF16 ? " kernel=mfma_bf16"
                               : d.mat == kMatBF16x3 ? " kernel=mfma_bf16x3" : "")
-        << (d.rows ? " rows_out" : "") << " batch=" << s.batch << " C=16 threads=" << d.n
-        << " groups=" << d.groups << " in_stride=" << d.istride << " out_stride=" << (d.rows ? 1 : d.ostride);
+        << (d.rin ? " rows_in" : "") << (d.rows ? " rows_out" : "") << " batch=" << s.batch << " C=16 threads=" << d.n
+        << " groups=" << d.groups << " in_stride=" << (d.rin ? 1 : d.istride) << " out_stride=" << (d.rows ? 1 : d.ostride);
       if (d.psh) o << " twiddle=w_" << (int64_t(1) << d.psh);
       if (d.scale != 1.0) o << " scale=" << d.scale;
       if (d.nt) o << " stream=" << ((d.nt & 1) ? "in" : "") << ((d.nt & 2) ? "out" : "");
-      if (s.alias_src != s.src || s.alias_dst != s.dst)
-        o << " aliased=" << roles[s.alias_src] << "->" << roles[s.alias_dst];
+      const int as = s.alias_src >= 0 ? s.alias_src : s.src, ad = s.alias_dst >= 0 ? s.alias_dst : s.dst;
+      if (as != s.src || ad != s.dst) o << " aliased=" << roles[as] << "->" << roles[ad];
       o << "\n";
       continue;
     }
@@ -2691,6 +2691,7 @@ static void stream_hints(Layout& L) {
     if (L.steps[i].dst == RY) last_y = static_cast<int>(i);
   for (size_t i = 0; i < L.steps.size(); ++i) {
     Step& s = L.steps[i];
+    if (s.kind == ST_LEAN && s.ld.rin && (nt & 1) && s.src == RX) s.ld.nt = 1;
     // (the power-of-two kernel's register-I/O instantiations honour them)
     if (s.kind != ST_PASS || !s.pow2 || (s.d.n & (s.d.n - 1)) || s.d.real_io || s.d.rs_io || s.d.rows_in ||
         s.d.rows_out)
@@ -2700,10 +2701,89 @@ static void stream_hints(Layout& L) {
   }
 }
 
+// FP32 power-of-two row passes of 64, 128 or 256 points over contiguous
+// rows in and out (groups of 16 rows, no multi-pass twiddle, no fused real
+// or resampling edge, the planner's own schedule) run on the lean kernel
+// with both edges transposed through LDS: thread t moves element t of each
+// of the group's 16 rows, so every wave instruction reads or writes whole
+// 512-byte runs.  FFTJH_LEAN_ROWS=0 keeps the power-of-two kernel.
+static void lean_rows(Layout& L) {
+  static const bool env = [] {
+    const char* e = std::getenv("FFTJH_LEAN_ROWS");
+    return !e || std::atoi(e) != 0;
+  }();
+  fftjh_tuning t = L.tune;
+  t.flags &= ~(FFTJH_FLAG_MFMA_F16 | FFTJH_FLAG_MFMA_BF16 | FFTJH_FLAG_MFMA_BF16X3);
+  if (!env || L.dtype != FFTJH_F32 || L.mat || !tune_is_default(t)) return;
+  static const bool cols_env = [] {
+    const char* e = std::getenv("FFTJH_LEAN_COLS");
+    return !e || std::atoi(e) != 0;
+  }();
+  for (Step& s : L.steps) {
+    if (s.kind != ST_PASS || !s.pow2 || s.mat) continue;
+    const PassDesc& d = s.d;
+    if (!lean_length(d.n) || d.real_io || d.rs_io || d.tw_period || d.B2 != 1) continue;
+    if (d.batch % 16 || d.batch / 16 >= (int64_t(1) << 31)) continue;
+    // column blocks of an N-d axis (column b % Bi of block b / Bi, element j
+    // at j*Bi), 16 adjacent columns per group: the lean column kernel
+    auto colblock = [&](const Geometry& g) {
+      return g.stride == d.Bi && g.s_in == 1 && g.s_mid == 0 && g.s_out == d.Bi * d.n;
+    };
+    if (cols_env && d.Bi > 1 && d.Bi % 16 == 0 && colblock(d.in) && colblock(d.out) &&
+        d.batch / d.Bi < (int64_t(1) << 31)) {
+      Step st;
+      st.kind = ST_LEAN;
+      st.dir = s.dir;
+      st.src = s.src;
+      st.dst = s.dst;
+      st.alias_src = s.alias_src;
+      st.alias_dst = s.alias_dst;
+      st.n = d.n;
+      st.batch = d.batch;
+      LeanDesc& ld = st.ld;
+      ld.n = d.n;
+      ld.groups = static_cast<int32_t>(d.batch / 16);
+      ld.D0 = static_cast<int32_t>(d.Bi / 16);
+      ld.D1 = static_cast<int32_t>(d.batch / d.Bi);
+      ld.in_s0 = ld.out_s0 = 16;
+      ld.in_s1 = ld.out_s1 = d.Bi * d.n;
+      ld.istride = ld.ostride = d.Bi;
+      ld.scale = d.scale;
+      s = st;
+      continue;
+    }
+    if (d.Bi != 1) continue;
+    if (d.in.stride != 1 || d.out.stride != 1 || d.in.s_out < d.n || d.out.s_out < d.n) continue;
+    Step st;
+    st.kind = ST_LEAN;
+    st.dir = s.dir;
+    st.src = s.src;
+    st.dst = s.dst;
+    st.alias_src = s.alias_src;
+    st.alias_dst = s.alias_dst;
+    st.n = d.n;
+    st.batch = d.batch;
+    LeanDesc& ld = st.ld;
+    ld.n = d.n;
+    ld.rows = 1;
+    ld.rin = 1;
+    ld.groups = static_cast<int32_t>(d.batch / 16);
+    ld.D0 = ld.groups;
+    ld.D1 = 1;
+    ld.in_s0 = 16 * d.in.s_out;
+    ld.istride = d.in.s_out;
+    ld.out_s0 = 16 * d.out.s_out;
+    ld.ostride = d.out.s_out;
+    ld.scale = d.scale;
+    s = st;
+  }
+}
+
 static int make_plan(fftjh_plan* out, Layout&& L) {
   if (!out) fail(FFTJH_ERR_ARGUMENT, "plan output pointer is NULL");
   auto P = std::make_unique<fftjh_plan_s>();
   P->L = std::move(L);
+  lean_rows(P->L);
   stream_hints(P->L);
   materialize(*P);
   *out = P.release();
@@ -2869,6 +2949,7 @@ int fftjh_describe_request(int kind, int dtype, int rank, const int64_t* dims, i
     else if (kind == FFTJH_KIND_C2R)
       L = layout_c2r(dtype, rank, dims, batch, 1);
     else fail(FFTJH_ERR_ARGUMENT, "unknown kind");
+    lean_rows(L);
     stream_hints(L);
     copy_out(describe(L), buf, len);
   });
@@ -2878,6 +2959,7 @@ int fftjh_describe_tuned(int dtype, int rank, const int64_t* dims, int64_t batch
                          const fftjh_tuning* tune, char* buf, int64_t len) {
   return guarded([&] {
     Layout L = layout_c2c(dtype, rank, dims, batch, sign, 1.0, tune);
+    lean_rows(L);
     stream_hints(L);
     copy_out(describe(L), buf, len);
   });

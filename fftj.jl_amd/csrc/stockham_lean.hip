@@ -88,7 +88,11 @@ template <int NT> __device__ __forceinline__ void st_out(float2* p, const float2
 // rows, after which register q holds the stage-2 input q of the LDS path.
 // NT: bit 0 loads x non-temporally (streaming: the plan does not read it
 // again), bit 1 stores y non-temporally.
-template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0, int NT = 0>
+// RIN = 1: the 16 transforms of a group are contiguous rows (row i at
+// istride*i from the group's base): thread t loads element t of every row
+// (whole 256-byte-and-up runs per wave) and an LDS transposition hands
+// thread (c, jt) its elements jt + R2*u of row c.
+template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0, int NT = 0, int RIN = 0>
 __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restrict__ x, float2* y,
                                               const float2* __restrict__ tws) {
   constexpr int R2 = N / 16;   // threads per column
@@ -148,6 +152,15 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
       }
     }
     const float2* xb = x + G.ib;
+    if constexpr (RIN) {
+      // element tid of row u (transposed after the loads)
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const float2* p = xb + static_cast<int64_t>(u) * d.istride + tid;
+        v[u] = (NT & 1) ? ld_nt2(p) : *p;
+      }
+      return;
+    }
     const int lo = c + jt * static_cast<int>(d.istride);
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
@@ -163,6 +176,24 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
   float2 v[16];
   double2 w0d, sud;
   fetch(G, v, w0d, sud);
+  if constexpr (RIN) {
+    static_assert(SPLIT == 1, "the rows-in transposition uses the float exchange buffer");
+    // v[i] = element tid of row i -> v[u] = element jt + R2*u of row c
+    // (rows at column stride N + 4 words, as the rows-out transposition)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lf[i * CS + tid] = v[i].x;
+    __syncthreads();
+    float re[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) re[u] = lf[c * CS + jt + R2 * u];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lf[i * CS + tid] = v[i].y;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = make_float2(re[u], lf[c * CS + jt + R2 * u]);
+    __syncthreads();
+  }
   ltw[tid] = tw_own;
   const float scale = static_cast<float>(d.scale);
   const bool scaled = d.scale != 1.0;
@@ -582,6 +613,18 @@ hipError_t launch_k(const LeanDesc& d, const void* x, void* y, const void* tws, 
 
 template <int DIR, int N>
 hipError_t launch_n(const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t st) {
+  if (d.rin) {
+    // rows in, rows out (the split exchange; non-temporal loads when asked)
+    if (!d.rows) return hipErrorInvalidValue;
+    if (d.nt == 1) {
+      hipLaunchKernelGGL((k_lean<DIR, N, 1, 1, 0, 1, 1>), dim3(d.groups), dim3(N), 0, st, d,
+                         static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+      return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_lean<DIR, N, 1, 1, 0, 0, 1>), dim3(d.groups), dim3(N), 0, st, d,
+                       static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+    return hipGetLastError();
+  }
   if constexpr (N == 64) {
     if (lean_xlane())
       return d.rows ? launch_k<DIR, N, 1, 1, 1>(d, x, y, tws, st) : launch_k<DIR, N, 0, 1, 1>(d, x, y, tws, st);

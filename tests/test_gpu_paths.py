@@ -162,6 +162,43 @@ def test_nd(prec, dims):
     assert rel_l2(yb, n * x.astype(np.complex128)) <= 2 * tol
 
 
+@pytest.mark.parametrize("dims", [(64, 16), (128, 48), (256, 256, 3), (16, 256, 4), (64, 32, 48), (256, 17, 16)])
+def test_lean_rows_and_columns(dims):
+    """FP32 plans whose row pass runs on the lean kernel with rows in and out
+    (groups of 16 rows) and whose strided axes run its column form (16
+    adjacent columns), against the oracle, both directions and in place."""
+    n = int(np.prod(dims))
+    x = inputs_c(n + 3, n, "f32").reshape(tuple(reversed(dims)))
+    xd = to_dev(x)
+    p = F.plan_fft(xd)
+    assert "kernel=lean" in p.describe()
+    ref = O.ndfft(x.astype(np.complex128).ravel(), list(dims)).reshape(x.shape)
+    y = to_host(p * xd)
+    tol = 200 * np.finfo(np.float32).eps
+    assert rel_l2(y, ref) <= tol
+    pi = F.plan_ifft(xd)
+    back = to_host(pi * to_dev(y))
+    assert rel_l2(back, x) <= 2 * tol
+    z = to_dev(x)
+    p(z, z)
+    assert np.array_equal(to_host(z), y)
+
+
+@pytest.mark.parametrize("n,batch", [(64, 16 * 37), (128, 16 * 9), (256, 16 * 5), (256, 65536)])
+def test_lean_row_batches(n, batch):
+    x = inputs_c(n * 7 + batch, n * batch, "f32")
+    xd = to_dev(x)
+    p = F.plan_fft(torch.float32, n, batch=batch)
+    assert "rows_in rows_out" in p.describe()
+    y = to_host(p * xd)
+    rows = list(range(0, batch, max(1, batch // 64))) + [batch - 1]
+    xr = x.reshape(batch, n)
+    ref = O.fft(xr[rows].astype(np.complex128), -1, 1.0, "f64")
+    assert rel_l2(y.reshape(batch, n)[rows], ref) <= tol_for("f32", n)
+    back = to_host(F.plan_ifft(torch.float32, n, batch=batch) * to_dev(y))
+    assert rel_l2(back, x) <= 2 * tol_for("f32", n)
+
+
 # N-d real plans (runtests.jl:358-370 on the same shapes, plus odd, ragged,
 # non-smooth and long halved / strided axes): torch shape reversed(dims),
 # halved along the last axis

@@ -108,6 +108,26 @@ def test_schedule_config3():
     assert len(s) == 3 and "in_stride=1 " in s[0] and "in_stride=256 " in s[1] and "in_stride=65536 " in s[2]
 
 
+def test_lean_rows_and_columns_schedule():
+    # FP32 rows of 64/128/256 points in groups of 16 take the lean kernel with
+    # both edges transposed through LDS; N-d axes whose column blocks hold a
+    # multiple of 16 columns take its column form; the rest keep pow2
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [256, 256, 256]))
+    assert "kernel=lean rows_in rows_out" in s[0] and "in_stride=1 out_stride=1" in s[0]
+    assert all("kernel=lean" in p and "rows_" not in p for p in s[1:])
+    assert "in_stride=256 " in s[1] and "in_stride=65536 " in s[2]
+    for n, batch in ((64, 48), (128, 4096), (256, 16)):
+        s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [n], batch=batch))
+        assert len(s) == 1 and "kernel=lean rows_in rows_out" in s[0] and f"groups={batch // 16} " in s[0]
+    for n, batch in ((128, 40), (512, 64), (256, 1)):
+        s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [n], batch=batch))
+        assert "kernel=lean" not in s[0]
+    # FP64 and column blocks of 129 columns keep the power-of-two kernel
+    assert "kernel=lean" not in _lib.describe_request(_lib.KIND_C2C, _lib.F64, [256], batch=64)
+    s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F32, [256, 256, 256]))
+    assert "kernel=pow2" in s[1] and "in_stride=129 " in s[1] and "kernel=lean" in s[2]
+
+
 def test_stream_hints_rule():
     # footprints of 384 MiB and up: the pass reading x streams its loads, the
     # pass writing y last streams its stores; smaller plans keep the caches
