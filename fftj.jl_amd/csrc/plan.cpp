@@ -2070,8 +2070,8 @@ static Layout layout_ragged(int dtype, int64_t count, const int64_t* sizes, cons
         ragged_two_level(C, it, n);
         if (as_pass) {
           const int32_t idx = static_cast<int32_t>(C.items.size());
-          it.woff = C.ws_total;
-          C.ws_total += n;
+          it.woff = (C.ws_total + 15) & ~int64_t(15);  // each region starts on a 128-byte line
+          C.ws_total = it.woff + n;
           for (int c0 = 0; c0 < it.n1; c0 += it.cA) C.tiles_a.push_back({idx, c0});
           for (int r0 = 0; r0 < it.n2; r0 += it.cB) C.tiles_b.push_back({idx, r0});
         }
@@ -2122,6 +2122,39 @@ static Layout layout_ragged(int dtype, int64_t count, const int64_t* sizes, cons
   if (!R3->items.empty()) st.rg3 = R3;
   L.steps.push_back(st);
   return L;
+}
+
+// XCD-local tile order for the tile-granular passes.  Workgroups are
+// dispatched to the 8 XCDs round-robin by index, and each XCD has its own
+// L2; packed items start anywhere, so a 16-column run of one tile shares its
+// first and last 128-byte lines with the neighbouring tiles.  Laying each
+// item's tiles out at indices x, x + 8, x + 16, ... (bin x, items assigned to
+// bins longest first onto the least-filled) keeps them on one XCD, close in
+// time, so the shared lines are fetched once.  Empty slots (unequal bins)
+// hold item -1.
+static std::vector<RaggedTile> xcd_tile_order(const std::vector<RaggedTile>& tiles) {
+  constexpr int kX = 8;
+  std::vector<std::pair<size_t, size_t>> runs;  // [begin, end) per item
+  for (size_t i = 0; i < tiles.size();) {
+    size_t j = i;
+    while (j < tiles.size() && tiles[j].item == tiles[i].item) ++j;
+    runs.push_back({i, j});
+    i = j;
+  }
+  std::stable_sort(runs.begin(), runs.end(),
+                   [](const auto& a, const auto& b) { return a.second - a.first > b.second - b.first; });
+  std::vector<std::vector<RaggedTile>> bins(kX);
+  std::vector<size_t> fill(kX, 0);
+  for (const auto& r : runs) {
+    const int x = static_cast<int>(std::min_element(fill.begin(), fill.end()) - fill.begin());
+    for (size_t t = r.first; t < r.second; ++t) bins[x].push_back(tiles[t]);
+    fill[x] += r.second - r.first;
+  }
+  const size_t slots = *std::max_element(fill.begin(), fill.end());
+  std::vector<RaggedTile> out(slots * kX, RaggedTile{-1, 0});
+  for (int x = 0; x < kX; ++x)
+    for (size_t s = 0; s < bins[x].size(); ++s) out[s * kX + x] = bins[x][s];
+  return out;
 }
 
 // longest-predicted-first onto the least-loaded workgroup (deterministic)
@@ -2215,9 +2248,13 @@ static std::string describe(const Layout& L) {
       if (B)
         o << " large_transforms=" << B->items.size() << " large_tile=" << B->tile_max << " large_direct=" << B->counts[0]
           << " large_two_level=" << B->counts[1] + B->counts[3];
-      if (Pp)
-        o << " pass_transforms=" << Pp->items.size() << " pass_tiles=" << Pp->tiles_a.size() << "+"
-          << Pp->tiles_b.size();
+      if (Pp) {
+        auto live = [](const std::vector<RaggedTile>& v) {
+          return std::count_if(v.begin(), v.end(), [](const RaggedTile& t) { return t.item >= 0; });
+        };
+        o << " pass_transforms=" << Pp->items.size() << " pass_tiles=" << live(Pp->tiles_a) << "+"
+          << live(Pp->tiles_b);
+      }
     } else if (s.kind == ST_HVDH_PERMUTE || s.kind == ST_HVDH_SPREAD || s.kind == ST_HVDH_SOLVE) {
       o << " n=" << s.hv.n << " t=" << s.hv.t << " taps=" << 2 * s.hv.h << " band=" << s.hv.B << " halo=" << s.hv.H
         << " batch=" << s.hv.batch;
@@ -2415,6 +2452,14 @@ static void materialize(fftjh_plan_s& P) {
         if (bytes) hip_check(hipMemcpy(b->p, src, bytes, hipMemcpyHostToDevice), "ragged table upload");
         return b;
       };
+      static const bool xcd_env = [] {
+        const char* e = std::getenv("FFTJH_RAGGED_XCD");
+        return !e || std::atoi(e) != 0;
+      }();
+      if (xcd_env) {
+        R.tiles_a = xcd_tile_order(R.tiles_a);
+        R.tiles_b = xcd_tile_order(R.tiles_b);
+      }
       if (R.tiles_a.size() > (size_t(1) << 31) - 1 || R.tiles_b.size() > (size_t(1) << 31) - 1)
         fail(FFTJH_ERR_UNSUPPORTED, "too many ragged pass tiles");
       R.d_items = upload(R.items.data(), R.items.size() * sizeof(RaggedItem));

@@ -403,6 +403,7 @@ k_ragged_pass(const RaggedItem* __restrict__ items, const RaggedTile* __restrict
   const TwPtr<T> t1 = as_tw<T>(tw);
   const int tid = static_cast<int>(threadIdx.x);
   const RaggedTile tl = tiles[blockIdx.x];
+  if (tl.item < 0) return;  // an empty slot of the XCD-local order
   const RaggedItem& it = items[tl.item];
   const int n1 = it.n1, n2 = it.n2;
   V2<T>* mid = ws + it.woff;
