@@ -1905,6 +1905,29 @@ static std::vector<int> radices_pref(int64_t n, int pref) {
   return r;
 }
 
+static int ragged_tile_C(const RaggedData& R, int64_t L, const std::vector<int>& rad);
+
+// The stage chain of a tile length.  The tile-granular passes take the
+// shortest chain with the composite radices 6..15 (140 = 14 x 10 instead of
+// 4 x 5 x 7: fewer LDS round trips per tile) where it fits as many columns
+// per tile as the prime chain; FFTJH_RAGGED_COMPACT=0 keeps the prime chains.
+static std::vector<int> ragged_radices(const RaggedData& R, int64_t L) {
+  static const bool compact_env = [] {
+    const char* e = std::getenv("FFTJH_RAGGED_COMPACT");
+    return !e || std::atoi(e) != 0;
+  }();
+  std::vector<int> rad = radices_pref(L, R.radix_pref);
+  if (!compact_env || R.big != 2 || rad.size() < 3 || L > R.tile_max) return rad;
+  std::vector<int> rc = compact_radices(L);
+  if (rc.empty() || rc.size() >= rad.size()) return rad;
+  int cp = 0, cc = 0;
+  for (int C = 64; C >= 1 && !cp; C >>= 1)
+    if (C * L <= R.tile_max && pass_threads(static_cast<int>(L), C, rad) <= R.threads) cp = C;
+  for (int C = 64; C >= 1 && !cc; C >>= 1)
+    if (C * L <= R.tile_max && pass_threads(static_cast<int>(L), C, rc) <= R.threads) cc = C;
+  return cc >= cp ? rc : rad;
+}
+
 // largest power-of-two number of length-L columns one tile holds
 static int ragged_tile_C(const RaggedData& R, int64_t L, const std::vector<int>& rad) {
   for (int C = 64; C >= 1; C >>= 1) {
@@ -1918,7 +1941,7 @@ static int ragged_tile_C(const RaggedData& R, int64_t L, const std::vector<int>&
 static bool ragged_direct_ok(const RaggedData& R, int64_t L) {
   if (L == 1) return true;  // no stages: a scaled copy
   if (L > R.tile_max) return false;
-  auto rad = radices_pref(L, R.radix_pref);
+  auto rad = ragged_radices(R, L);
   if (rad.empty() || static_cast<int>(rad.size()) > kMaxStages) return false;
   return ragged_tile_C(R, L, rad) >= 1;
 }
@@ -1928,7 +1951,7 @@ static int ragged_sched(RaggedData& R, int64_t L) {
   if (it != R.sched_of.end()) return it->second;
   RaggedSched sc{};
   sc.n = static_cast<int>(L);
-  auto rad = radices_pref(L, R.radix_pref);
+  auto rad = ragged_radices(R, L);
   sc.nstages = static_cast<int>(rad.size());
   int64_t ns = 1;
   for (size_t i = 0; i < rad.size(); ++i) {
@@ -1964,7 +1987,7 @@ static void ragged_two_level(RaggedData& R, RaggedItem& it, int64_t L) {
   it.n1 = static_cast<int>(L / best_n2);
   it.sA = ragged_sched(R, it.n2);
   it.sB = ragged_sched(R, it.n1);
-  auto radA = radices_pref(it.n2, R.radix_pref), radB = radices_pref(it.n1, R.radix_pref);
+  auto radA = ragged_radices(R, it.n2), radB = ragged_radices(R, it.n1);
   it.cA = ragged_tile_C(R, it.n2, radA);
   it.cB = ragged_tile_C(R, it.n1, radB);
   it.logcA = 0;

@@ -9,7 +9,7 @@ namespace fftjh {
 constexpr int kRaggedThreads = 256;  // workgroup size of the small-item launch
 constexpr int kRaggedTile = 4096;    // complex elements one LDS tile holds (small items)
 // smooth items longer than this run as two tile-granular passes (default)
-constexpr int kRaggedPassMin = 16384;
+constexpr int kRaggedPassMin = 4096;
 // the large-item launch: one workgroup per CU whose LDS tile holds 16384
 // complex FP32 (8192 FP64) elements, so lengths up to that run in one tile
 // and longer ones take 64-column tiles in their two phases

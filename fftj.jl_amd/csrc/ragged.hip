@@ -296,6 +296,12 @@ __device__ __forceinline__ void tile_first(const int radix, const int n, const i
     case 3: FFTJH_RF(3); break;
     case 4: FFTJH_RF(4); break;
     case 5: FFTJH_RF(5); break;
+    case 6: FFTJH_RF(6); break;
+    case 9: FFTJH_RF(9); break;
+    case 10: FFTJH_RF(10); break;
+    case 12: FFTJH_RF(12); break;
+    case 14: FFTJH_RF(14); break;
+    case 15: FFTJH_RF(15); break;
     case 7: FFTJH_RF(7); break;
     case 8: FFTJH_RF(8); break;
     case 11: FFTJH_RF(11); break;
@@ -318,6 +324,12 @@ __device__ __forceinline__ void tile_last(const StageDesc& sl, const int n, cons
     case 3: FFTJH_RL(3); break;
     case 4: FFTJH_RL(4); break;
     case 5: FFTJH_RL(5); break;
+    case 6: FFTJH_RL(6); break;
+    case 9: FFTJH_RL(9); break;
+    case 10: FFTJH_RL(10); break;
+    case 12: FFTJH_RL(12); break;
+    case 14: FFTJH_RL(14); break;
+    case 15: FFTJH_RL(15); break;
     case 7: FFTJH_RL(7); break;
     case 8: FFTJH_RL(8); break;
     case 11: FFTJH_RL(11); break;
@@ -339,6 +351,12 @@ __device__ void tile_mid(LdsPtr<T> lds, const RaggedSched& sc, int C, int logC, 
       case 3: run_stage<T, DIR, 3>(sc.n, C, st, lds, tw, c, jt, jstep); break;
       case 4: run_stage<T, DIR, 4>(sc.n, C, st, lds, tw, c, jt, jstep); break;
       case 5: run_stage<T, DIR, 5>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 6: run_stage<T, DIR, 6>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 9: run_stage<T, DIR, 9>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 10: run_stage<T, DIR, 10>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 12: run_stage<T, DIR, 12>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 14: run_stage<T, DIR, 14>(sc.n, C, st, lds, tw, c, jt, jstep); break;
+      case 15: run_stage<T, DIR, 15>(sc.n, C, st, lds, tw, c, jt, jstep); break;
       case 7: run_stage<T, DIR, 7>(sc.n, C, st, lds, tw, c, jt, jstep); break;
       case 8: run_stage<T, DIR, 8>(sc.n, C, st, lds, tw, c, jt, jstep); break;
       case 11: run_stage<T, DIR, 11>(sc.n, C, st, lds, tw, c, jt, jstep); break;
