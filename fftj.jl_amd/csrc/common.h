@@ -131,6 +131,9 @@ struct LeanDesc {
   int32_t groups;
   int32_t D0, D1;
   int32_t rin;      // inputs as contiguous rows at stride istride (an LDS transposition)
+  int32_t ncol;     // columns per block (column form): group columns 16*d0 + c >= ncol are
+                    // absent (no load, no store); 0: every group is whole
+  int32_t xcd;      // 1: XCD-local group order (workgroup b runs group start(b % 8) + b / 8)
   int64_t in_s0, in_s1, in_s2, istride;
   int64_t out_s0, out_s1, out_s2, ostride;
   int32_t psh;

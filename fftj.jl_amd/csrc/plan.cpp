@@ -2791,14 +2791,15 @@ static void lean_rows(Layout& L) {
     if (s.kind != ST_PASS || !s.pow2 || s.mat) continue;
     const PassDesc& d = s.d;
     if (!lean_length(d.n) || d.real_io || d.rs_io || d.tw_period || d.B2 != 1) continue;
-    if (d.batch % 16 || d.batch / 16 >= (int64_t(1) << 31)) continue;
     // column blocks of an N-d axis (column b % Bi of block b / Bi, element j
     // at j*Bi), 16 adjacent columns per group: the lean column kernel
     auto colblock = [&](const Geometry& g) {
       return g.stride == d.Bi && g.s_in == 1 && g.s_mid == 0 && g.s_out == d.Bi * d.n;
     };
-    if (cols_env && d.Bi > 1 && d.Bi % 16 == 0 && colblock(d.in) && colblock(d.out) &&
-        d.batch / d.Bi < (int64_t(1) << 31)) {
+    // (a block of columns that is not a multiple of 16 ends in a partial group)
+    const int64_t D0 = (d.Bi + 15) / 16;
+    if (cols_env && d.Bi >= 16 && d.Bi < (int64_t(1) << 31) && colblock(d.in) && colblock(d.out) &&
+        D0 * (d.batch / d.Bi) < (int64_t(1) << 31)) {
       Step st;
       st.kind = ST_LEAN;
       st.dir = s.dir;
@@ -2810,9 +2811,10 @@ static void lean_rows(Layout& L) {
       st.batch = d.batch;
       LeanDesc& ld = st.ld;
       ld.n = d.n;
-      ld.groups = static_cast<int32_t>(d.batch / 16);
-      ld.D0 = static_cast<int32_t>(d.Bi / 16);
+      ld.D0 = static_cast<int32_t>(D0);
       ld.D1 = static_cast<int32_t>(d.batch / d.Bi);
+      ld.groups = static_cast<int32_t>(D0 * (d.batch / d.Bi));
+      ld.ncol = d.Bi % 16 ? static_cast<int32_t>(d.Bi) : 0;
       ld.in_s0 = ld.out_s0 = 16;
       ld.in_s1 = ld.out_s1 = d.Bi * d.n;
       ld.istride = ld.ostride = d.Bi;
@@ -2820,7 +2822,7 @@ static void lean_rows(Layout& L) {
       s = st;
       continue;
     }
-    if (d.Bi != 1) continue;
+    if (d.Bi != 1 || d.batch % 16 || d.batch / 16 >= (int64_t(1) << 31)) continue;
     if (d.in.stride != 1 || d.out.stride != 1 || d.in.s_out < d.n || d.out.s_out < d.n) continue;
     Step st;
     st.kind = ST_LEAN;
@@ -2847,11 +2849,29 @@ static void lean_rows(Layout& L) {
   }
 }
 
+// Every lean launch takes the XCD-local group order (neighbouring column
+// groups on one XCD).  Measured on one MI355X, forward, ms (off / only the
+// launches with partial column groups / every launch): 256^3 R2C 0.1076 /
+// 0.0844 / 0.0855, C2R 0.1069 / 0.0927 / 0.0879 (its 129-column half
+// spectrum: every 128-byte run straddles two lines), 256^3 C2C 0.1257 /
+// 0.1259 / 0.1250, 2^24 0.1353 / 0.1366 / 0.1341; 2^24 forward + inverse
+// 0.2676 / 0.2670 / 0.2660.  FFTJH_LEAN_XCD: 1 (default) every launch, 2
+// only partial groups, 0 none.
+static void lean_xcd(Layout& L) {
+  static const int env = [] {
+    const char* e = std::getenv("FFTJH_LEAN_XCD");
+    return e ? std::atoi(e) : 1;
+  }();
+  for (Step& s : L.steps)
+    if (s.kind == ST_LEAN) s.ld.xcd = env == 1 || (env == 2 && s.ld.ncol) ? 1 : 0;
+}
+
 static int make_plan(fftjh_plan* out, Layout&& L) {
   if (!out) fail(FFTJH_ERR_ARGUMENT, "plan output pointer is NULL");
   auto P = std::make_unique<fftjh_plan_s>();
   P->L = std::move(L);
   lean_rows(P->L);
+  lean_xcd(P->L);
   stream_hints(P->L);
   materialize(*P);
   *out = P.release();

@@ -53,6 +53,17 @@ namespace {
 
 __device__ __forceinline__ int lp(int f) { return f + (f >> 4); }
 
+// Group of workgroup b.  Workgroups are dispatched to the 8 XCDs round-robin
+// by index and each XCD has its own L2; with `xcd` set, XCD x runs the
+// contiguous groups [start(x), start(x + 1)), so neighbouring column groups
+// (whose 128-byte runs share lines when the rows are not line-aligned) run on
+// one XCD at nearly the same time.  A bijection on [0, groups).
+__device__ __forceinline__ uint32_t lean_group(uint32_t b, uint32_t groups, int xcd) {
+  if (!xcd) return b;
+  const uint32_t q = groups >> 3, r = groups & 7u, x = b & 7u, s = b >> 3;
+  return x * q + (x < r ? x : r) + s;
+}
+
 // non-temporal (streaming) element load, and the store of an output
 // (non-temporal when bit 1 of NT is set)
 __device__ __forceinline__ float2 ld_nt2(const float2* p) {
@@ -162,15 +173,17 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
       return;
     }
     const int lo = c + jt * static_cast<int>(d.istride);
+    // (a partial column group: the absent columns load zeros)
+    const bool here = !d.ncol || 16 * static_cast<int>(G.d0) + c < d.ncol;
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
       const float2* p = xb + static_cast<int64_t>(R2) * u * d.istride + lo;
-      v[u] = (NT & 1) ? ld_nt2(p) : *p;
+      v[u] = !here ? make_float2(0.f, 0.f) : (NT & 1) ? ld_nt2(p) : *p;
     }
   };
 
-  const uint32_t g = blockIdx.x;
-  if (static_cast<int>(g) >= groups) return;
+  if (static_cast<int>(blockIdx.x) >= groups) return;
+  const uint32_t g = lean_group(blockIdx.x, static_cast<uint32_t>(groups), d.xcd);
   const float2 tw_own = tws[tid];
   const Grp G = group(g);
   float2 v[16];
@@ -314,6 +327,7 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
     } else {
       float2* yb = y + G.ob;
       const int lo = c + jt * static_cast<int>(d.ostride);
+      if (d.ncol && 16 * static_cast<int>(G.d0) + c >= d.ncol) return;  // an absent column
       if (d.psh) {
         // e_u = d1*(a0*(16*d0 + c) + b*(jt + R2*u)) = e_0 + u*d1*b*R2
         const float2 ws = scaled ? cscale<float>(w0, scale) : w0;
@@ -364,8 +378,8 @@ __global__ void __launch_bounds__(256) k_lean_mat(LeanDesc d, const float2* __re
   const int lane = tid & 63, wave = tid >> 6;
   const int g4 = lane >> 4, col = lane & 15;
   const int jt = tid >> 4;  // the VALU stage's thread role (column col)
-  const uint32_t g = blockIdx.x;
-  if (static_cast<int>(g) >= d.groups) return;
+  if (static_cast<int>(blockIdx.x) >= d.groups) return;
+  const uint32_t g = lean_group(blockIdx.x, static_cast<uint32_t>(d.groups), d.xcd);
 
   const double2* ptw = static_cast<const double2*>(d.ptw);
   const uint32_t pmask = d.psh >= 32 ? 0xffffffffu : ((1u << d.psh) - 1u);

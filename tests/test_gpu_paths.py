@@ -162,7 +162,8 @@ def test_nd(prec, dims):
     assert rel_l2(yb, n * x.astype(np.complex128)) <= 2 * tol
 
 
-@pytest.mark.parametrize("dims", [(64, 16), (128, 48), (256, 256, 3), (16, 256, 4), (64, 32, 48), (256, 17, 16)])
+@pytest.mark.parametrize("dims", [(64, 16), (128, 48), (256, 256, 3), (16, 256, 4), (64, 32, 48), (256, 17, 16),
+                                  (33, 64, 5), (17, 256, 3), (129, 128, 2), (20, 64, 64)])
 def test_lean_rows_and_columns(dims):
     """FP32 plans whose row pass runs on the lean kernel with rows in and out
     (groups of 16 rows) and whose strided axes run its column form (16

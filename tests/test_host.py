@@ -122,10 +122,13 @@ def test_lean_rows_and_columns_schedule():
     for n, batch in ((128, 40), (512, 64), (256, 1)):
         s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [n], batch=batch))
         assert "kernel=lean" not in s[0]
-    # FP64 and column blocks of 129 columns keep the power-of-two kernel
+    # FP64 keeps the power-of-two kernel; a block of 129 columns ends in a
+    # partial group of one column (9 groups per block)
     assert "kernel=lean" not in _lib.describe_request(_lib.KIND_C2C, _lib.F64, [256], batch=64)
     s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F32, [256, 256, 256]))
-    assert "kernel=pow2" in s[1] and "in_stride=129 " in s[1] and "kernel=lean" in s[2]
+    assert "kernel=lean" in s[1] and "in_stride=129 " in s[1] and "groups=2304 " in s[1] and "kernel=lean" in s[2]
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [33, 64, 5]))
+    assert "n=64 " in s[1] and "kernel=lean" in s[1] and "groups=15 " in s[1]
 
 
 def test_stream_hints_rule():
