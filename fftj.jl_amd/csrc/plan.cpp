@@ -1890,7 +1890,11 @@ struct RaggedData {
   std::vector<double> cost;     // per item
   int64_t counts[4] = {0, 0, 0, 0};
   std::vector<RaggedTile> tiles_a, tiles_b;  // tile-granular passes: phase A and phase B tiles
-  DevBufPtr d_items, d_wgptr, d_wgitems, d_scheds, d_tw, d_corr, d_ws, d_wgws, d_tiles_a, d_tiles_b;
+  // tile-granular passes: FP64 two-level tables of w_n per length (hi, lo offsets in double2)
+  std::map<int64_t, std::pair<int64_t, int64_t>> t64_of;
+  int64_t tw64_total = 0;
+  std::vector<PowJob> pow64_jobs;
+  DevBufPtr d_items, d_wgptr, d_wgitems, d_scheds, d_tw, d_corr, d_ws, d_wgws, d_tiles_a, d_tiles_b, d_tw64;
 };
 
 static std::vector<int> radices_pref(int64_t n, int pref) {
@@ -2004,6 +2008,19 @@ static void ragged_two_level(RaggedData& R, RaggedItem& it, int64_t L) {
   it.tw_hi = R.tw_total;
   R.pow_jobs.push_back({R.tw_total, nh, K, L});
   R.tw_total += nh;
+  if (R.big == 2) {
+    // the same split in FP64 for the tile passes' twiddle start and steps
+    auto t = R.t64_of.find(L);
+    if (t == R.t64_of.end()) {
+      const int64_t lo = R.tw64_total;
+      R.pow64_jobs.push_back({lo, K, 1, L});
+      R.pow64_jobs.push_back({lo + K, nh, K, L});
+      R.tw64_total += K + nh;
+      t = R.t64_of.emplace(L, std::make_pair(lo + K, lo)).first;
+    }
+    it.t64_hi = t->second.first;
+    it.t64_lo = t->second.second;
+  }
 }
 
 static double ragged_cost(const RaggedItem& it) {
@@ -2495,6 +2512,11 @@ static void materialize(fftjh_plan_s& P) {
       for (const auto& j : R.pow_jobs)
         hip_check(launch_tw_pow(dt, static_cast<char*>(R.d_tw->p) + j.off * es, j.count, j.mult, j.L, R.dir, 0),
                   "twiddles");
+      R.d_tw64 = std::make_shared<DevBuf>(std::max<int64_t>(R.tw64_total, 1) * 16);
+      for (const auto& j : R.pow64_jobs)
+        hip_check(launch_tw_pow(FFTJH_F64, static_cast<char*>(R.d_tw64->p) + j.off * 16, j.count, j.mult, j.L, R.dir,
+                                0),
+                  "twiddles");
       hip_check(hipDeviceSynchronize(), "ragged tables");
       R.d_ws = std::make_shared<DevBuf>(std::max<int64_t>(R.ws_total, 1) * es);
      }
@@ -2672,6 +2694,7 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
           pl.tile_list = ph ? R.d_tiles_b->p : R.d_tiles_a->p;
           pl.scheds = R.d_scheds->p;
           pl.tw = R.d_tw->p;
+          pl.tw64 = R.d_tw64->p;
           pl.x = ptr[s.src];
           pl.y = ptr[s.dst];
           pl.ws = ws ? ws : R.d_ws->p;

@@ -41,6 +41,7 @@ struct RaggedItem {
   double beta;
   double scale;
   int64_t woff;      // tile-granular passes: the item's region of the workspace (n elements)
+  int64_t t64_hi, t64_lo;  // tile-granular passes: FP64 two-level tables of w_n (shared per length)
 };
 
 // One tile of a tile-granular pass over long items: phase A tiles are
@@ -58,6 +59,7 @@ struct RaggedPassLaunch {
   const void* tile_list;
   const void* scheds;
   const void* tw;
+  const void* tw64;  // FP64 two-level tables (double2) of the phase-A twiddle
   const void* x;
   void* y;
   void* ws;

@@ -401,19 +401,19 @@ __device__ __noinline__ void tile_load_rows(GPtr<T> src_, const int n1_, const i
   }
 }
 
-// w_n^e = exp(DIR*2*pi*i*e/n) in FP64, e reduced exactly mod n
-template <int DIR>
-__device__ __forceinline__ double2 ragged_w(uint64_t e, int n) {
-  e %= static_cast<uint64_t>(n);
-  double sv, cv;
-  sincospi(2.0 * static_cast<double>(e) / static_cast<double>(n), &sv, &cv);
-  return make_double2(cv, DIR * sv);
+// w_n^e (e < 2^32) from the item's FP64 two-level tables: w_n^(i*2^lo_bits)
+// at hi[i], w_n^i at lo[i]; e reduced exactly mod n
+__device__ __forceinline__ double2 ragged_w(const double2* __restrict__ t64, const RaggedItem& it, uint32_t e) {
+  e %= static_cast<uint32_t>(it.n);
+  const uint32_t mask = (1u << it.lo_bits) - 1u;
+  return cmul64(t64[it.t64_hi + (e >> it.lo_bits)], t64[it.t64_lo + (e & mask)]);
 }
 
 template <typename T, int DIR, int PH>
 __global__ void __launch_bounds__(kRaggedThreads) __attribute__((amdgpu_waves_per_eu(FFTJH_RAGGED_PASS_WAVES)))
 k_ragged_pass(const RaggedItem* __restrict__ items, const RaggedTile* __restrict__ tiles,
-              const RaggedSched* __restrict__ scheds, const V2<T>* tw, const V2<T>* x, V2<T>* y, V2<T>* ws) {
+              const RaggedSched* __restrict__ scheds, const V2<T>* tw, const double2* __restrict__ tw64,
+              const V2<T>* x, V2<T>* y, V2<T>* ws) {
   constexpr int kRT = kRaggedThreads, kTile = kRaggedTile;
   extern __shared__ __align__(16) unsigned char smem[];
   V2<T>* lds = reinterpret_cast<V2<T>*>(smem);
@@ -444,11 +444,10 @@ k_ragged_pass(const RaggedItem* __restrict__ items, const RaggedTile* __restrict
     tile_mid<T, DIR, kRT>(l3, sc, C, logC, t1);
     // outputs k2 = j + r*ns of column j1 = c0 + c times w_n^(j1*k2)
     const StageDesc& sl = sc.st[sc.nstages - 1];
-    const int n = it.n;
-    const uint64_t j1 = static_cast<uint64_t>(c0 + c);
-    const double2 w0 = ragged_w<DIR>(j1 * static_cast<uint64_t>(jt), n);
-    const double2 Sr = ragged_w<DIR>(j1 * static_cast<uint64_t>(sl.ns), n);
-    const double2 Sb = ragged_w<DIR>(j1 * static_cast<uint64_t>(jstep), n);
+    const uint32_t j1 = static_cast<uint32_t>(c0 + c);  // (j1 < n1 <= 4096, jt, ns, jstep < 4096)
+    const double2 w0 = ragged_w(tw64, it, j1 * static_cast<uint32_t>(jt));
+    const double2 Sr = ragged_w(tw64, it, j1 * static_cast<uint32_t>(sl.ns));
+    const double2 Sb = ragged_w(tw64, it, j1 * static_cast<uint32_t>(jstep));
     tile_last<T, DIR>(sl, n2, C, l3, t1, as_g<T>(mid + c0 + c), n1, valid, T(1), true, w0, Sr, Sb, c, jt, jstep);
   } else {
     const int C = it.cB, logC = it.logcB;
@@ -491,7 +490,8 @@ static hipError_t launch_pass_t(const RaggedPassLaunch& L, hipStream_t st) {
   hipLaunchKernelGGL((k_ragged_pass<T, DIR, PH>), dim3(L.tiles), dim3(kRaggedThreads), lds, st,
                      static_cast<const RaggedItem*>(L.items), static_cast<const RaggedTile*>(L.tile_list),
                      static_cast<const RaggedSched*>(L.scheds), static_cast<const V2<T>*>(L.tw),
-                     static_cast<const V2<T>*>(L.x), static_cast<V2<T>*>(L.y), static_cast<V2<T>*>(L.ws));
+                     static_cast<const double2*>(L.tw64), static_cast<const V2<T>*>(L.x), static_cast<V2<T>*>(L.y),
+                     static_cast<V2<T>*>(L.ws));
   return hipGetLastError();
 }
 
