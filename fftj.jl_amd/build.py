@@ -26,7 +26,7 @@ FLAGS = [
 ]
 SOURCES = ["stockham.hip", "stockham_pow2.hip", "stockham_pow2_f32_fwd.hip", "stockham_pow2_f32_inv.hip",
            "stockham_pow2_f64_fwd.hip", "stockham_pow2_f64_inv.hip", "twiddle.hip", "real.hip", "resample.hip",
-           "ragged.hip", "mlp.hip", "stockham_mat.hip", "stockham_lean.hip", "hvdh.hip",
+           "ragged.hip", "mlp.hip", "stockham_mat.hip", "stockham_lean.hip", "stockham_lean_sq.hip", "hvdh.hip",
            "plan.cpp"]
 
 

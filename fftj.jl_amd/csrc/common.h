@@ -145,6 +145,12 @@ struct LeanDesc {
   int32_t mat;      // split-precision MFMA mode of the radix-16 stages (kMatNone: VALU)
   int32_t nt;       // bit 0: non-temporal loads of x, bit 1: non-temporal stores of y
   const void* matp; // that mode's operand table for the pass's direction
+  // square lengths N = R*R (k_lean_sq): columns per group, and the twiddle
+  // period when it is not a power of two (0: P = 2^psh) with 1/P
+  int32_t cols;
+  int32_t pad2_;
+  int64_t pmod;
+  double pinv;
 };
 
 // HvdH route (hvdh.hip) of `batch` transforms of odd non-smooth n, FP32:

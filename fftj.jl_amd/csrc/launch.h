@@ -28,6 +28,10 @@ int pow2_max_threads(int dtype);
 // lean column-pass kernel (FP32, n = 64, 128, 256); tws: the table
 // w_n^i, i < n, of the pass's direction
 bool lean_length(int n);
+// square lengths on the lean kernel (stockham_lean_sq.hip): columns per
+// group for length n (0: none)
+int lean_sq_cols(int n);
+hipError_t launch_lean_sq(int dir, const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t stream);
 hipError_t launch_lean(int dir, const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t stream);
 
 // HvdH route kernels (FP32): the Delta permutation dst[Delta*l mod n] =

@@ -1113,6 +1113,17 @@ static bool lean3_ok(const Layout& L, const std::vector<int64_t>& F, int64_t n, 
       dst != RY)
     return false;
   if (F.size() != 3) return false;
+  // square factors (100 = 10 x 10) on the square kernel: all three equal, C
+  // columns per group dividing every pass's column count
+  const int sq = lean_sq_cols(static_cast<int>(F[0]));
+  if (sq && !L.mat && F[1] == F[0] && F[2] == F[0]) {
+    static const bool sq_env = [] {
+      const char* e = std::getenv("FFTJH_LEAN_SQ");
+      return !e || std::atoi(e) != 0;
+    }();
+    return sq_env && F[0] % sq == 0 && (F[1] * F[2]) % sq == 0 && n < (int64_t(1) << 31) &&
+           batch * (n / (F[0] * sq)) < (int64_t(1) << 31);
+  }
   for (int64_t f : F)
     if (!lean_length(static_cast<int>(f))) return false;
   // 32-bit lane offsets (< n/16 + 16) and group counts (< 2^31)
@@ -1127,11 +1138,17 @@ static void lean_twiddle(LeanDesc& d, int64_t P, int64_t a0, int64_t b) {
   d.ptw_lo = ((P - 1) >> d.ptw_sh) + 1;
   d.tw_a0 = a0;
   d.tw_b = b;
+  // a period that is not a power of two (the square kernel): reduced mod P
+  d.pmod = (P & (P - 1)) ? P : 0;
+  d.pinv = 1.0 / static_cast<double>(P);
 }
 
 static void add_lean3(Layout& L, const std::vector<int64_t>& F, int64_t n, int64_t batch, int dir, double scale,
                       int src, int dst, int z) {
   const int64_t N1 = F[0], N2 = F[1], N3 = F[2], N23 = N2 * N3;
+  // columns per group: 16 (powers of two), or the square kernel's
+  const int64_t CW = lean_sq_cols(static_cast<int>(N1)) && N2 == N1 && N3 == N1 ? lean_sq_cols(static_cast<int>(N1))
+                                                                                  : 16;
   need_scratch(L, z, n * batch);  // the x == y variant
   auto mk = [&](int N, int rows, int64_t D0, int64_t D1, int s_, int d_, int as_, int ad_) {
     Step st;
@@ -1145,31 +1162,32 @@ static void add_lean3(Layout& L, const std::vector<int64_t>& F, int64_t n, int64
     st.ld.rows = rows;
     st.ld.D0 = static_cast<int32_t>(D0);
     st.ld.D1 = static_cast<int32_t>(D1);
-    st.ld.groups = static_cast<int32_t>(batch * (n / (16 * N)));
+    st.ld.groups = static_cast<int32_t>(batch * (n / (CW * N)));
     st.ld.scale = 1.0;
     st.ld.mat = L.mat;
+    st.ld.cols = CW == 16 ? 0 : static_cast<int32_t>(CW);
     st.n = N;
     st.batch = batch * (n / N);
     return st;
   };
-  Step a = mk(static_cast<int>(N3), 1, N1 / 16, N2, src, dst, src, z);
-  a.ld.in_s0 = 16;
+  Step a = mk(static_cast<int>(N3), 1, N1 / CW, N2, src, dst, src, z);
+  a.ld.in_s0 = CW;
   a.ld.in_s1 = N1;
   a.ld.in_s2 = n;
   a.ld.istride = N1 * N2;
-  a.ld.out_s0 = 16 * N23;
+  a.ld.out_s0 = CW * N23;
   a.ld.out_s1 = N3;
   a.ld.out_s2 = n;
   a.ld.ostride = N23;
   lean_twiddle(a.ld, N23, 0, 1);
-  Step b = mk(static_cast<int>(N2), 0, N3 / 16, N1, dst, dst, z, z);
-  b.ld.in_s0 = b.ld.out_s0 = 16;
+  Step b = mk(static_cast<int>(N2), 0, N3 / CW, N1, dst, dst, z, z);
+  b.ld.in_s0 = b.ld.out_s0 = CW;
   b.ld.in_s1 = b.ld.out_s1 = N23;
   b.ld.in_s2 = b.ld.out_s2 = n;
   b.ld.istride = b.ld.ostride = N3;
   lean_twiddle(b.ld, n, 1, N3);
-  Step c = mk(static_cast<int>(N1), 0, N23 / 16, batch, dst, dst, z, dst);
-  c.ld.in_s0 = c.ld.out_s0 = 16;
+  Step c = mk(static_cast<int>(N1), 0, N23 / CW, batch, dst, dst, z, dst);
+  c.ld.in_s0 = c.ld.out_s0 = CW;
   c.ld.in_s1 = c.ld.out_s1 = n;
   c.ld.istride = c.ld.ostride = N23;
   c.ld.scale = scale;
@@ -1185,7 +1203,7 @@ static void add_lean3(Layout& L, const std::vector<int64_t>& F, int64_t n, int64
     const char* e = std::getenv("FFTJH_LEAN_NT");
     return e ? std::atoi(e) : -1;
   }();
-  if (!L.mat) {
+  if (!L.mat && CW == 16) {
     const int nt = lean_nt >= 0 ? lean_nt : (16 * n * batch >= (int64_t(256) << 20) ? 1 : 0);
     if (nt & 1) a.ld.nt = 1;
     if (nt & 2) c.ld.nt = 2;
@@ -2242,12 +2260,15 @@ static std::string describe(const Layout& L) {
     if (s.kind == ST_LEAN) {
       // a pass line (the planners' features read pass lines)
       const LeanDesc& d = s.ld;
-      o << "  pass " << roles[s.src] << "->" << roles[s.dst] << " n=" << d.n << " radices=16," << d.n / 16
+      const int sqr = d.cols ? static_cast<int>(std::lround(std::sqrt(static_cast<double>(d.n)))) : 0;
+      o << "  pass " << roles[s.src] << "->" << roles[s.dst] << " n=" << d.n << " radices="
+        << (sqr ? sqr : 16) << "," << (sqr ? sqr : d.n / 16)
         << " kernel=lean" << (d.mat == kMatF16 ? " kernel=mfma_f16" : d.mat == kMatBF16 ? " kernel=mfma_bf16"
                               : d.mat == kMatBF16x3 ? " kernel=mfma_bf16x3" : "")
-        << (d.rin ? " rows_in" : "") << (d.rows ? " rows_out" : "") << " batch=" << s.batch << " C=16 threads=" << d.n
+        << (d.rin ? " rows_in" : "") << (d.rows ? " rows_out" : "") << " batch=" << s.batch << " C="
+        << (d.cols ? d.cols : 16) << " threads=" << (d.cols ? d.cols * sqr : d.n)
         << " groups=" << d.groups << " in_stride=" << (d.rin ? 1 : d.istride) << " out_stride=" << (d.rows ? 1 : d.ostride);
-      if (d.psh) o << " twiddle=w_" << (int64_t(1) << d.psh);
+      if (d.psh) o << " twiddle=w_" << (d.pmod ? d.pmod : int64_t(1) << d.psh);
       if (d.scale != 1.0) o << " scale=" << d.scale;
       if (d.nt) o << " stream=" << ((d.nt & 1) ? "in" : "") << ((d.nt & 2) ? "out" : "");
       const int as = s.alias_src >= 0 ? s.alias_src : s.src, ad = s.alias_dst >= 0 ? s.alias_dst : s.dst;
@@ -2550,7 +2571,8 @@ static void materialize(fftjh_plan_s& P) {
       s.tw = cached(key_of("lean", {dev, double(dir), double(N)}), int64_t(N) * 8,
                     [&](void* p) { hip_check(launch_tw_pow(FFTJH_F32, p, N, 1, N, dir, 0), "twiddle kernel"); });
       if (s.ld.psh) {
-        const int64_t P = int64_t(1) << s.ld.psh, hi = s.ld.ptw_lo, lo = int64_t(1) << s.ld.ptw_sh;
+        const int64_t P = s.ld.pmod ? s.ld.pmod : int64_t(1) << s.ld.psh, hi = s.ld.ptw_lo,
+                      lo = int64_t(1) << s.ld.ptw_sh;
         std::ostringstream pk;
         pk << "ptw:" << dev << ":" << dir << ":" << P;
         s.ptw = cached(pk.str(), (hi + lo) * 16, [&](void* p) {

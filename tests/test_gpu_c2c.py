@@ -134,6 +134,27 @@ def test_config2c_1e6x8():
     assert rel_l2(y.reshape(batch, n)[[0, 7]], ref) <= tol_for("f32", n)
 
 
+@pytest.mark.parametrize("batch", [1, 3])
+def test_lean_square_1e6(batch):
+    """10^6 = 100^3 on the square lean kernel (100 = 10 x 10, 50 columns per
+    group, twiddle periods 10^4 and 10^6): forward vs the oracle, the 1/n
+    inverse back, and the in-place call (x == y through the scratch)."""
+    n = 10**6
+    x = inputs_c(9 + batch, n * batch, "f32").reshape(batch, n)
+    p = F.plan_fft(torch.float32, n, batch=batch, planner="heuristic")
+    assert p.describe().count("radices=10,10 kernel=lean") == 3
+    xd = to_dev(x)
+    y = to_host(p * xd).reshape(batch, n)
+    rows = [0, batch - 1] if batch > 1 else [0]
+    assert rel_l2(y[rows], O.fft(x[rows].astype(np.complex128))) <= tol_for("f32", n)
+    pi = F.plan_ifft(torch.float32, n, batch=batch, planner="heuristic")
+    back = to_host(pi * to_dev(y))
+    assert rel_l2(back.reshape(batch, n), x) <= 2 * tol_for("f32", n)
+    z = to_dev(x)
+    p(z, z)
+    assert np.array_equal(to_host(z).reshape(batch, n), y)
+
+
 # composite-radix butterflies (prime-factor 6, 10, 12, 14, 15 and 3 x 3 = 9):
 # each forced as a stage of an explicit chain, first, middle and last
 # position, both directions (the fused generic kernel and the plain one)

@@ -131,6 +131,17 @@ def test_lean_rows_and_columns_schedule():
     assert "n=64 " in s[1] and "kernel=lean" in s[1] and "groups=15 " in s[1]
 
 
+def test_lean_square_schedule():
+    # 10^6 = 100^3 (FP32): the lean layout on the square kernel, 100 = 10 x 10,
+    # 50 columns per group of 500 threads; twiddle periods 10^4 and 10^6
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [10**6], batch=8))
+    assert len(s) == 3 and all("n=100 radices=10,10 kernel=lean" in p and "C=50 threads=500" in p for p in s)
+    assert "rows_out" in s[0] and "twiddle=w_10000 " in s[0] and "twiddle=w_1000000 " in s[1]
+    assert "twiddle" not in s[2] and s[2].endswith("aliased=s0->y")
+    # FP64 keeps the fused chain kernel
+    assert "kernel=lean" not in _lib.describe_request(_lib.KIND_C2C, _lib.F64, [10**6], batch=8)
+
+
 def test_stream_hints_rule():
     # footprints of 384 MiB and up: the pass reading x streams its loads, the
     # pass writing y last streams its stores; smaller plans keep the caches
@@ -144,11 +155,15 @@ def test_stream_hints_rule():
 
 
 def test_fixed_chain_schedule():
-    # 10^6 = 100^3: FP32 passes of 100 = 10 x 10 on the compile-time chain
-    # kernel (10 elements per thread, 32 transforms per workgroup); FP64's
-    # 320 threads would exceed its limit, so it keeps the runtime chain
-    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [10**6], batch=8))
+    # passes of 100 = 10 x 10 outside the lean layout (a tuned 10^6 split,
+    # the axes of a 100^3 array) run on the compile-time chain kernel (10
+    # elements per thread, 32 transforms per workgroup); FP64's 320 threads
+    # would exceed its limit, so it keeps the runtime chain
+    t = _lib.make_tuning([100, 100, 100], (), 0)
+    s = _passes(_lib.describe_tuned(_lib.F32, [10**6], batch=8, tuning=t))
     assert len(s) == 3 and all("n=100 radices=10,10 kernel=chain" in p and "C=32 threads=320" in p for p in s)
+    s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [100, 100, 100]))
+    assert len(s) == 3 and all("n=100 radices=10,10 kernel=chain" in p for p in s)
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F64, [10**6], batch=2))
     assert len(s) == 3 and not any("kernel=chain" in p for p in s)
 
