@@ -148,9 +148,15 @@ struct LeanDesc {
   // square lengths N = R*R (k_lean_sq): columns per group, and the twiddle
   // period when it is not a power of two (0: P = 2^psh) with 1/P
   int32_t cols;
-  int32_t pad2_;
+  // rs = 1 (rows-out passes): the pass reads the m-point grid of a resampled
+  // length straight from rows of rs_n inputs through the deconvolution,
+  // g[j] = x[j]*corr[j] (j < (rs_n+1)/2), x[j-(rs_m-rs_n)]*corr[j-(rs_m-rs_n)]
+  // (j >= rs_m - rs_n + (rs_n+1)/2), 0 between; transform d2 at x + d2*rs_n
+  int32_t rs;
   int64_t pmod;
   double pinv;
+  int64_t rs_n, rs_m;
+  const void* rs_corr;
 };
 
 // HvdH route (hvdh.hip) of `batch` transforms of odd non-smooth n, FP32:

@@ -1459,6 +1459,51 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
   const int g = free[0];
   need_scratch(L, g, rp.m * batch);
   {
+    // FP32 grids of three lean factors: the lean layout from x into the grid
+    // (pass A reads x through the deconvolution, passes B and C in place on
+    // the grid), then the interpolation.  FFTJH_RS_LEAN=0: the pass kernel.
+    static const bool rs_lean = [] {
+      const char* e = std::getenv("FFTJH_RS_LEAN");
+      return !e || std::atoi(e) != 0;
+    }();
+    fftjh_tuning t = L.tune;
+    t.flags &= ~(FFTJH_FLAG_MFMA_F16 | FFTJH_FLAG_MFMA_BF16 | FFTJH_FLAG_MFMA_BF16X3);
+    const std::vector<int64_t> F = rp.m > 16384 ? choose_factors(L.dtype, rp.m, default_max_factor(L.dtype))
+                                                : std::vector<int64_t>{};
+    bool lean_ok = rs_lean && L.dtype == FFTJH_F32 && !L.mat && tune_is_default(t) && is_rows(gin, n, Bi) &&
+                   gin.stride == 1 && rp.m < (int64_t(1) << 31) && F.size() == 3 &&
+                   batch * (rp.m / 1024) < (int64_t(1) << 31);
+    for (int64_t f : F) lean_ok = lean_ok && lean_length(static_cast<int>(f));
+    if (lean_ok) {
+      const size_t first = L.steps.size();
+      add_lean3(L, F, rp.m, batch, dir, 1.0, src, g, g);
+      Step& a = L.steps[first];
+      a.ld.rs = 1;
+      a.ld.rs_n = n;
+      a.ld.rs_m = rp.m;
+      a.ld.nt = 0;
+      a.n = n;  // the correction table's length and parameters
+      a.m = rp.m;
+      a.w = rp.w;
+      a.beta = rp.beta;
+      a.scale = scale;
+      Step ip;
+      ip.kind = ST_RS_INTERP;
+      ip.src = g;
+      ip.dst = dst;
+      ip.dir = dir;
+      ip.n = n;
+      ip.m = rp.m;
+      ip.batch = batch;
+      ip.w = rp.w;
+      ip.beta = rp.beta;
+      ip.Bi = Bi;
+      ip.gb = gout;
+      L.steps.push_back(ip);
+      return;
+    }
+  }
+  {
     // the deconvolution at the load of the m-point transform's first pass,
     // when that is a power-of-two pass over the grid rows and x is rows
     // (FFTJH_RS_FUSE=0: its own launch)
@@ -2271,6 +2316,7 @@ static std::string describe(const Layout& L) {
       if (d.psh) o << " twiddle=w_" << (d.pmod ? d.pmod : int64_t(1) << d.psh);
       if (d.scale != 1.0) o << " scale=" << d.scale;
       if (d.nt) o << " stream=" << ((d.nt & 1) ? "in" : "") << ((d.nt & 2) ? "out" : "");
+      if (d.rs) o << " deconv n=" << d.rs_n << " m=" << d.rs_m << " w=" << s.w;
       const int as = s.alias_src >= 0 ? s.alias_src : s.src, ad = s.alias_dst >= 0 ? s.alias_dst : s.dst;
       if (as != s.src || ad != s.dst) o << " aliased=" << roles[as] << "->" << roles[ad];
       o << "\n";
@@ -2568,6 +2614,10 @@ static void materialize(fftjh_plan_s& P) {
     } else if (s.kind == ST_LEAN) {
       // stage table w_n^i (i < n) and the multi-pass twiddle's FP64 tables
       const int N = s.ld.n, dir = s.dir;
+      if (s.ld.rs) {
+        s.aux = rs_corr(s);
+        s.ld.rs_corr = s.aux->p;
+      }
       s.tw = cached(key_of("lean", {dev, double(dir), double(N)}), int64_t(N) * 8,
                     [&](void* p) { hip_check(launch_tw_pow(FFTJH_F32, p, N, 1, N, dir, 0), "twiddle kernel"); });
       if (s.ld.psh) {

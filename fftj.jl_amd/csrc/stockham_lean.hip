@@ -103,7 +103,10 @@ template <int NT> __device__ __forceinline__ void st_out(float2* p, const float2
 // istride*i from the group's base): thread t loads element t of every row
 // (whole 256-byte-and-up runs per wave) and an LDS transposition hands
 // thread (c, jt) its elements jt + R2*u of row c.
-template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0, int NT = 0, int RIN = 0>
+// RS = 1: the loads go through the deconvolution of a resampled length
+// (LeanDesc::rs): grid element j of transform d2 is x[d2*rs_n + j'] *
+// corr[j'] on the two kept ranges, zero between.
+template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0, int NT = 0, int RIN = 0, int RS = 0>
 __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restrict__ x, float2* y,
                                               const float2* __restrict__ tws) {
   constexpr int R2 = N / 16;   // threads per column
@@ -134,7 +137,7 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
     return cmul64(ptw[e >> d.ptw_sh], ptw[d.ptw_lo + (e & lmask)]);
   };
   struct Grp {
-    uint32_t d0, d1;
+    uint32_t d0, d1, d2;
     int64_t ib, ob;
   };
   auto group = [&](uint32_t g) {
@@ -142,9 +145,9 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
     r.d0 = g % static_cast<uint32_t>(d.D0);
     const uint32_t gr = g / static_cast<uint32_t>(d.D0);
     r.d1 = gr % static_cast<uint32_t>(d.D1);
-    const uint32_t d2 = gr / static_cast<uint32_t>(d.D1);
-    r.ib = r.d0 * d.in_s0 + r.d1 * d.in_s1 + d2 * d.in_s2;
-    r.ob = r.d0 * d.out_s0 + r.d1 * d.out_s1 + d2 * d.out_s2;
+    r.d2 = gr / static_cast<uint32_t>(d.D1);
+    r.ib = r.d0 * d.in_s0 + r.d1 * d.in_s1 + r.d2 * d.in_s2;
+    r.ob = r.d0 * d.out_s0 + r.d1 * d.out_s1 + r.d2 * d.out_s2;
     return r;
   };
   // twiddle gathers first, then the data loads (waits on the gathers then
@@ -161,6 +164,25 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
         w0d = tw_at(G.d1 * (a + bb * static_cast<uint32_t>(jt)));
         if (tid < 16) sud = tw_at(G.d1 * bb * static_cast<uint32_t>(R2) * static_cast<uint32_t>(tid));
       }
+    }
+    if constexpr (RS) {
+      // grid indices in 32 bits (the planner keeps m below 2^31)
+      const int j0 = static_cast<int>(G.d0 * d.in_s0 + G.d1 * d.in_s1) + c + jt * static_cast<int>(d.istride);
+      const float2* xr = x + static_cast<int64_t>(G.d2) * d.rs_n;
+      const float* cr = static_cast<const float*>(d.rs_corr);
+      const int hp = static_cast<int>((d.rs_n + 1) / 2), shift = static_cast<int>(d.rs_m - d.rs_n);
+      const int hi0 = shift + hp;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int j = j0 + R2 * u * static_cast<int>(d.istride);
+        v[u] = make_float2(0.f, 0.f);
+        if (j < hp) {
+          v[u] = cscale<float>(xr[j], cr[j]);
+        } else if (j >= hi0) {
+          v[u] = cscale<float>(xr[j - shift], cr[j - shift]);
+        }
+      }
+      return;
     }
     const float2* xb = x + G.ib;
     if constexpr (RIN) {
@@ -627,6 +649,13 @@ hipError_t launch_k(const LeanDesc& d, const void* x, void* y, const void* tws, 
 
 template <int DIR, int N>
 hipError_t launch_n(const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t st) {
+  if (d.rs) {
+    // the first pass of a resampled length's grid transform (rows out)
+    if (!d.rows) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_lean<DIR, N, 1, 1, 0, 0, 0, 1>), dim3(d.groups), dim3(N), 0, st, d,
+                       static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+    return hipGetLastError();
+  }
   if (d.rin) {
     // rows in, rows out (the split exchange; non-temporal loads when asked)
     if (!d.rows) return hipErrorInvalidValue;

@@ -92,6 +92,10 @@ def test_schedule_config3():
     assert s[-1].startswith("rs_interp")
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1000003], batch=4))
     assert "deconv n=1000003 m=2097152" in s[0]
+    # a grid of three lean factors (2^21 = 128^3) runs the lean layout from x
+    # into the grid, the first pass loading through the deconvolution
+    assert len(s) == 4 and all("kernel=lean" in p for p in s[:3]) and "rows_out" in s[0]
+    assert s[0].startswith("pass x->s0") and s[2].startswith("pass s0->s0") and s[3].startswith("rs_interp s0->y")
     # a generic first pass keeps the deconvolution as a launch of its own
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F64, [10007], batch=3))
     assert s[0].startswith("rs_deconv") and "deconv" not in s[1]
