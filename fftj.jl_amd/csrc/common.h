@@ -154,6 +154,11 @@ struct LeanDesc {
   // (j >= rs_m - rs_n + (rs_n+1)/2), 0 between; transform d2 at x + d2*rs_n
   int32_t rs;
   int64_t pmod;
+  // twq = 1 (columns out): the multi-pass twiddle's multiplier is the column,
+  // e = (16*d0 + c)*(tw_a0*d1 + tw_b*k) mod 2^psh (the generic passes' middle
+  // form, the column's digit times the group's)
+  int32_t twq;
+  int32_t pad3_;
   double pinv;
   int64_t rs_n, rs_m;
   const void* rs_corr;

@@ -2885,7 +2885,54 @@ static void lean_rows(Layout& L) {
   for (Step& s : L.steps) {
     if (s.kind != ST_PASS || !s.pow2 || s.mat) continue;
     const PassDesc& d = s.d;
-    if (!lean_length(d.n) || d.real_io || d.rs_io || d.tw_period || d.B2 != 1) continue;
+    if (!lean_length(d.n) || d.real_io || d.rs_io) continue;
+    // a generic layout's column passes with a power-of-two multi-pass
+    // twiddle (columns b % Bi contiguous, element stride and the middle and
+    // outer indices as the geometry says, the same on both sides):
+    //   qdiv a multiple of 16, B2 = 1: the twiddle's multiplier q = column /
+    //   qdiv is a group digit (d1), e = d1*bcoef*k;
+    //   qdiv = 1: the multiplier is the column, e = col*(acoef*bm + bcoef*k),
+    //   bm the group digit d1 (TWQ form)
+    if (cols_env && d.tw_period && (d.tw_period & (d.tw_period - 1)) == 0 && d.tw_period <= (int64_t(1) << 32) &&
+        d.Bi % 16 == 0 && d.in.s_in == 1 && d.in.stride == d.out.stride && d.in.s_in == d.out.s_in &&
+        d.in.s_mid == d.out.s_mid && d.in.s_out == d.out.s_out && !d.nt_load && !d.nt_store) {
+      const bool digit = d.B2 == 1 && d.tw_qdiv % 16 == 0 && d.Bi % d.tw_qdiv == 0;
+      const bool colq = d.tw_qdiv == 1;
+      const int64_t outer = d.batch / (d.Bi * d.B2);
+      if ((digit || colq) && outer * d.Bi * d.B2 == d.batch && d.batch / 16 < (int64_t(1) << 31)) {
+        Step st;
+        st.kind = ST_LEAN;
+        st.dir = s.dir;
+        st.src = s.src;
+        st.dst = s.dst;
+        st.alias_src = s.alias_src;
+        st.alias_dst = s.alias_dst;
+        st.n = d.n;
+        st.batch = d.batch;
+        LeanDesc& ld = st.ld;
+        ld.n = d.n;
+        ld.groups = static_cast<int32_t>(d.batch / 16);
+        ld.in_s0 = ld.out_s0 = 16;
+        ld.istride = ld.ostride = d.in.stride;
+        ld.in_s2 = ld.out_s2 = d.in.s_out;
+        if (digit) {
+          ld.D0 = static_cast<int32_t>(d.tw_qdiv / 16);
+          ld.D1 = static_cast<int32_t>(d.Bi / d.tw_qdiv);
+          ld.in_s1 = ld.out_s1 = d.tw_qdiv;
+          lean_twiddle(ld, d.tw_period, 0, d.tw_bcoef % d.tw_period);
+        } else {
+          ld.D0 = static_cast<int32_t>(d.Bi / 16);
+          ld.D1 = static_cast<int32_t>(d.B2);
+          ld.in_s1 = ld.out_s1 = d.in.s_mid;
+          lean_twiddle(ld, d.tw_period, d.tw_acoef % d.tw_period, d.tw_bcoef % d.tw_period);
+          ld.twq = 1;
+        }
+        ld.scale = d.scale;
+        s = st;
+        continue;
+      }
+    }
+    if (d.tw_period || d.B2 != 1) continue;
     // column blocks of an N-d axis (column b % Bi of block b / Bi, element j
     // at j*Bi), 16 adjacent columns per group: the lean column kernel
     auto colblock = [&](const Geometry& g) {

@@ -106,7 +106,11 @@ template <int NT> __device__ __forceinline__ void st_out(float2* p, const float2
 // RS = 1: the loads go through the deconvolution of a resampled length
 // (LeanDesc::rs): grid element j of transform d2 is x[d2*rs_n + j'] *
 // corr[j'] on the two kept ranges, zero between.
-template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0, int NT = 0, int RIN = 0, int RS = 0>
+// TWQ = 1 (columns out): the multi-pass twiddle with the column as the
+// multiplier (LeanDesc::twq); each lane's start value w_P^(col*(a0*d1 +
+// b*jt)) times its column's step powers S_col^u, S_col = w_P^(col*b*R2),
+// a 16 x 16 table in LDS.
+template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0, int NT = 0, int RIN = 0, int RS = 0, int TWQ = 0>
 __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restrict__ x, float2* y,
                                               const float2* __restrict__ tws) {
   constexpr int R2 = N / 16;   // threads per column
@@ -119,6 +123,7 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
   __shared__ __align__(16) float lf[SPLIT ? LDS_ELEMS : 2 * LDS_ELEMS];
   __shared__ float2 ltw[N];     // the stage table w_N^i
   __shared__ float2 lsu[16];    // the group's twiddle steps S^u (columns out)
+  __shared__ float2 lsq[TWQ ? 256 : 1];  // TWQ: S_col^u at [u*16 + c]
   float2* lds = reinterpret_cast<float2*>(lf);
   const int tid = static_cast<int>(threadIdx.x);
   const int c = tid & 15, jt = tid >> 4;
@@ -157,7 +162,10 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
     sud = make_double2(1.0, 0.0);
     if (d.psh) {
       const uint32_t bb = static_cast<uint32_t>(d.tw_b);
-      if constexpr (ROWS) {
+      if constexpr (TWQ) {
+        const uint32_t col = 16u * G.d0 + static_cast<uint32_t>(c);
+        w0d = tw_at(col * (static_cast<uint32_t>(d.tw_a0) * G.d1 + bb * static_cast<uint32_t>(jt)));
+      } else if constexpr (ROWS) {
         w0d = tw_at(G.d1 * bb * static_cast<uint32_t>(tid));
       } else {
         const uint32_t a = static_cast<uint32_t>(d.tw_a0) * (16u * G.d0 + static_cast<uint32_t>(c));
@@ -211,6 +219,16 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
   float2 v[16];
   double2 w0d, sud;
   fetch(G, v, w0d, sud);
+  if constexpr (TWQ) {
+    if (d.psh) {
+      const uint32_t bb = static_cast<uint32_t>(d.tw_b);
+      for (int i = tid; i < 256; i += N) {
+        const uint32_t coli = 16u * G.d0 + static_cast<uint32_t>(i & 15);
+        const double2 sq = tw_at(coli * bb * static_cast<uint32_t>(R2) * static_cast<uint32_t>(i >> 4));
+        lsq[i] = make_float2(static_cast<float>(sq.x), static_cast<float>(sq.y));
+      }
+    }
+  }
   if constexpr (RIN) {
     static_assert(SPLIT == 1, "the rows-in transposition uses the float exchange buffer");
     // v[i] = element tid of row i -> v[u] = element jt + R2*u of row c
@@ -350,7 +368,15 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
       float2* yb = y + G.ob;
       const int lo = c + jt * static_cast<int>(d.ostride);
       if (d.ncol && 16 * static_cast<int>(G.d0) + c >= d.ncol) return;  // an absent column
-      if (d.psh) {
+      if (TWQ && d.psh) {
+        // e_u = col*(a0*d1 + b*(jt + R2*u)) = e_0 + u*col*b*R2
+        const float2 ws = scaled ? cscale<float>(w0, scale) : w0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const float2 val = cmul<float>(t[u], u ? cmul<float>(ws, lsq[u * 16 + c]) : ws);
+          st_out<NT>(yb + static_cast<int64_t>(R2) * u * d.ostride + lo, val);
+        }
+      } else if (d.psh) {
         // e_u = d1*(a0*(16*d0 + c) + b*(jt + R2*u)) = e_0 + u*d1*b*R2
         const float2 ws = scaled ? cscale<float>(w0, scale) : w0;
 #pragma unroll
@@ -649,6 +675,14 @@ hipError_t launch_k(const LeanDesc& d, const void* x, void* y, const void* tws, 
 
 template <int DIR, int N>
 hipError_t launch_n(const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t st) {
+  if (d.twq) {
+    // a generic layout's middle pass (columns out, the column as the
+    // twiddle's multiplier)
+    if (d.rows) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_lean<DIR, N, 0, 1, 0, 0, 0, 0, 1>), dim3(d.groups), dim3(N), 0, st, d,
+                       static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+    return hipGetLastError();
+  }
   if (d.rs) {
     // the first pass of a resampled length's grid transform (rows out)
     if (!d.rows) return hipErrorInvalidValue;
