@@ -2932,6 +2932,39 @@ static void lean_rows(Layout& L) {
         continue;
       }
     }
+    // a generic layout's last pass: rows in (16 consecutive transforms at
+    // row stride s_in), columns out (consecutive transforms adjacent), no
+    // twiddle: the rows-in transposition with the column store
+    if (cols_env && !d.tw_period && d.Bi % 16 == 0 && d.in.stride == 1 && d.in.s_in >= d.n && d.out.s_in == 1 &&
+        d.out.stride >= d.Bi && !d.nt_load && !d.nt_store && d.batch % (d.Bi * d.B2) == 0 &&
+        d.batch / 16 < (int64_t(1) << 31)) {
+      Step st;
+      st.kind = ST_LEAN;
+      st.dir = s.dir;
+      st.src = s.src;
+      st.dst = s.dst;
+      st.alias_src = s.alias_src;
+      st.alias_dst = s.alias_dst;
+      st.n = d.n;
+      st.batch = d.batch;
+      LeanDesc& ld = st.ld;
+      ld.n = d.n;
+      ld.rin = 1;
+      ld.groups = static_cast<int32_t>(d.batch / 16);
+      ld.D0 = static_cast<int32_t>(d.Bi / 16);
+      ld.D1 = static_cast<int32_t>(d.B2);
+      ld.in_s0 = 16 * d.in.s_in;
+      ld.in_s1 = d.in.s_mid;
+      ld.in_s2 = d.in.s_out;
+      ld.istride = d.in.s_in;
+      ld.out_s0 = 16;
+      ld.out_s1 = d.out.s_mid;
+      ld.out_s2 = d.out.s_out;
+      ld.ostride = d.out.stride;
+      ld.scale = d.scale;
+      s = st;
+      continue;
+    }
     if (d.tw_period || d.B2 != 1) continue;
     // column blocks of an N-d axis (column b % Bi of block b / Bi, element j
     // at j*Bi), 16 adjacent columns per group: the lean column kernel

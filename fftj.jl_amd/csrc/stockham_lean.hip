@@ -691,8 +691,13 @@ hipError_t launch_n(const LeanDesc& d, const void* x, void* y, const void* tws, 
     return hipGetLastError();
   }
   if (d.rin) {
-    // rows in, rows out (the split exchange; non-temporal loads when asked)
-    if (!d.rows) return hipErrorInvalidValue;
+    // rows in, rows out (the split exchange; non-temporal loads when asked),
+    // or rows in, columns out (a generic layout's last pass)
+    if (!d.rows) {
+      hipLaunchKernelGGL((k_lean<DIR, N, 0, 1, 0, 0, 1>), dim3(d.groups), dim3(N), 0, st, d,
+                         static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+      return hipGetLastError();
+    }
     if (d.nt == 1) {
       hipLaunchKernelGGL((k_lean<DIR, N, 1, 1, 0, 1, 1>), dim3(d.groups), dim3(N), 0, st, d,
                          static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
