@@ -158,7 +158,11 @@ struct LeanDesc {
   // e = (16*d0 + c)*(tw_a0*d1 + tw_b*k) mod 2^psh (the generic passes' middle
   // form, the column's digit times the group's)
   int32_t twq;
-  int32_t pad3_;
+  // real = 1 / 2 (rows in and out, half length N): the R2C twist at the
+  // rows-out store (N + 1 bins per row) / the C2R pre-pass at the rows-in
+  // load (N + 1 bins per row in); rtw = w_k = exp(dir*2*pi*i*k/(2N)), k <= N
+  int32_t real;
+  const void* rtw;
   double pinv;
   int64_t rs_n, rs_m;
   const void* rs_corr;

@@ -2317,6 +2317,7 @@ static std::string describe(const Layout& L) {
       if (d.scale != 1.0) o << " scale=" << d.scale;
       if (d.nt) o << " stream=" << ((d.nt & 1) ? "in" : "") << ((d.nt & 2) ? "out" : "");
       if (d.rs) o << " deconv n=" << d.rs_n << " m=" << d.rs_m << " w=" << s.w;
+      if (d.real) o << (d.real == 1 ? " real=r2c" : " real=c2r");
       const int as = s.alias_src >= 0 ? s.alias_src : s.src, ad = s.alias_dst >= 0 ? s.alias_dst : s.dst;
       if (as != s.src || ad != s.dst) o << " aliased=" << roles[as] << "->" << roles[ad];
       o << "\n";
@@ -2618,6 +2619,13 @@ static void materialize(fftjh_plan_s& P) {
         s.aux = rs_corr(s);
         s.ld.rs_corr = s.aux->p;
       }
+      if (s.ld.real) {
+        // w_k = exp(dir*2*pi*i*k/(2N)), k = 0..N (real.jl:26-39)
+        const int64_t n2 = 2 * int64_t(N), cnt = N + 1;
+        s.aux = cached(key_of("real", {dev, double(dt), double(dir), double(n2)}), cnt * es,
+                       [&](void* p) { hip_check(launch_tw_pow(dt, p, cnt, 1, n2, dir, 0), "twiddle kernel"); });
+        s.ld.rtw = s.aux->p;
+      }
       s.tw = cached(key_of("lean", {dev, double(dir), double(N)}), int64_t(N) * 8,
                     [&](void* p) { hip_check(launch_tw_pow(FFTJH_F32, p, N, 1, N, dir, 0), "twiddle kernel"); });
       if (s.ld.psh) {
@@ -2885,6 +2893,41 @@ static void lean_rows(Layout& L) {
   for (Step& s : L.steps) {
     if (s.kind != ST_PASS || !s.pow2 || s.mat) continue;
     const PassDesc& d = s.d;
+    // rows of 2N reals with the real edge fused (R2C twist at the store, C2R
+    // pre-pass at the load): rows in and out through LDS, the edge there
+    static const bool real_env = [] {
+      const char* e = std::getenv("FFTJH_LEAN_REAL");
+      return !e || std::atoi(e) != 0;
+    }();
+    if (real_env && lean_length(d.n) && (d.real_io == 1 || d.real_io == 2) && !d.rs_io && !d.tw_period &&
+        d.Bi == 1 && d.B2 == 1 && d.in.stride == 1 && d.out.stride == 1 && d.batch % 16 == 0 &&
+        d.batch / 16 < (int64_t(1) << 31) && !d.nt_load && !d.nt_store &&
+        d.in.s_out >= d.n + (d.real_io == 2 ? 1 : 0) && d.out.s_out >= d.n + (d.real_io == 1 ? 1 : 0)) {
+      Step st;
+      st.kind = ST_LEAN;
+      st.dir = s.dir;
+      st.src = s.src;
+      st.dst = s.dst;
+      st.alias_src = s.alias_src;
+      st.alias_dst = s.alias_dst;
+      st.n = d.n;
+      st.batch = d.batch;
+      LeanDesc& ld = st.ld;
+      ld.n = d.n;
+      ld.rows = 1;
+      ld.rin = 1;
+      ld.real = d.real_io;
+      ld.groups = static_cast<int32_t>(d.batch / 16);
+      ld.D0 = ld.groups;
+      ld.D1 = 1;
+      ld.in_s0 = 16 * d.in.s_out;
+      ld.istride = d.in.s_out;
+      ld.out_s0 = 16 * d.out.s_out;
+      ld.ostride = d.out.s_out;
+      ld.scale = d.scale;
+      s = st;
+      continue;
+    }
     if (!lean_length(d.n) || d.real_io || d.rs_io) continue;
     // a generic layout's column passes with a power-of-two multi-pass
     // twiddle (columns b % Bi contiguous, element stride and the middle and

@@ -110,7 +110,16 @@ template <int NT> __device__ __forceinline__ void st_out(float2* p, const float2
 // multiplier (LeanDesc::twq); each lane's start value w_P^(col*(a0*d1 +
 // b*jt)) times its column's step powers S_col^u, S_col = w_P^(col*b*R2),
 // a 16 x 16 table in LDS.
-template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0, int NT = 0, int RIN = 0, int RS = 0, int TWQ = 0>
+// REAL = 1 / 2 (with RIN and ROWS): the real edges of an even-length real
+// transform of 2N reals whose half-length transform is this pass, as
+// LeanDesc::real.  R2C: y_0 = Re Z_0 + Im Z_0, y_N = Re Z_0 - Im Z_0,
+// y_k = xe + w_k*(-i*xo), xe, xo = (Z_k +- conj(Z_{N-k}))/2, read from the
+// rows-out transposition (both partners are in LDS).  C2R: Z_k = xe +
+// i*(w_k*xd), xe, xd = X_k +- conj(X_{N-k}), formed through LDS from the
+// rows-in loads (X_N by thread 0) before the rows-in transposition; the 1/2
+// and 1/N ride on the pass's scale.
+template <int DIR, int N, int ROWS, int SPLIT, int XLANE = 0, int NT = 0, int RIN = 0, int RS = 0, int TWQ = 0,
+          int REAL = 0>
 __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restrict__ x, float2* y,
                                               const float2* __restrict__ tws) {
   constexpr int R2 = N / 16;   // threads per column
@@ -229,6 +238,45 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
       }
     }
   }
+  if constexpr (REAL == 2) {
+    // C2R pre-pass: thread tid holds X_i[tid] of the 16 rows; X_i[N] (for
+    // Z_0) comes from thread 0; the partners X_i[N - tid] through LDS
+    float2 xn[16];
+    if (tid == 0) {
+      const float2* xb = x + G.ib;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) xn[i] = xb[static_cast<int64_t>(i) * d.istride + N];
+    }
+    const float2 w = static_cast<const float2*>(d.rtw)[tid];
+    float br[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lf[i * CS + tid] = v[i].x;
+    if (tid == 0) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) lf[i * CS + N] = xn[i].x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) br[i] = lf[i * CS + N - tid];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) lf[i * CS + tid] = v[i].y;
+    if (tid == 0) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) lf[i * CS + N] = xn[i].y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const float2 a = v[i];
+      const float2 c = make_float2(br[i], -lf[i * CS + N - tid]);  // conj(X_{N-k})
+      const float2 xe = make_float2(a.x + c.x, a.y + c.y);
+      const float2 xd = make_float2(a.x - c.x, a.y - c.y);
+      const float2 t2 = cmul<float>(xd, w);
+      v[i] = make_float2(xe.x - t2.y, xe.y + t2.x);  // xe + i*(w*xd)
+    }
+    __syncthreads();
+  }
   if constexpr (RIN) {
     static_assert(SPLIT == 1, "the rows-in transposition uses the float exchange buffer");
     // v[i] = element tid of row i -> v[u] = element jt + R2*u of row c
@@ -331,7 +379,47 @@ __global__ void __launch_bounds__(256) k_lean(LeanDesc d, const float2* __restri
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) t[b + BPT * k2] = a[k2];
     }
-    if constexpr (ROWS) {
+    if constexpr (REAL == 1) {
+      // R2C: the rows-out transposition reads Z_k and Z_{N-k} of every row
+      static_assert(ROWS == 1 && SPLIT == 1, "the twist rides on the rows-out transposition");
+      float ar[16], br[16];
+      const int mk = (N - tid) & (N - 1);
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < 16; ++u) lf[c * CS + jt + R2 * u] = t[u].x;
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        ar[i] = lf[i * CS + tid];
+        br[i] = lf[i * CS + mk];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < 16; ++u) lf[c * CS + jt + R2 * u] = t[u].y;
+      __syncthreads();
+      const float2 w = static_cast<const float2*>(d.rtw)[tid];
+      float2* yb = y + G.ob;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const float2 a = make_float2(ar[i], lf[i * CS + tid]);
+        float2 out;
+        if (tid == 0) {
+          out = make_float2(a.x + a.y, 0.f);
+          float2 last = make_float2(a.x - a.y, 0.f);
+          if (scaled) last = cscale<float>(last, scale);
+          yb[static_cast<int64_t>(i) * d.ostride + N] = last;
+        } else {
+          const float2 cc = make_float2(br[i], -lf[i * CS + mk]);  // conj(Z_{N-k})
+          const float2 xe = make_float2(0.5f * (a.x + cc.x), 0.5f * (a.y + cc.y));
+          const float2 xo = make_float2(0.5f * (a.x - cc.x), 0.5f * (a.y - cc.y));
+          const float2 mixo = make_float2(xo.y, -xo.x);  // -i * xo
+          const float2 m = cmul<float>(mixo, w);
+          out = make_float2(xe.x + m.x, xe.y + m.y);
+        }
+        if (scaled) out = cscale<float>(out, scale);
+        yb[static_cast<int64_t>(i) * d.ostride + tid] = out;
+      }
+    } else if constexpr (ROWS) {
       // transpose through LDS: column c, element k = jt + R2*u at [c][k];
       // thread t then holds element t of every column i
       float2 o[16];
@@ -688,6 +776,17 @@ hipError_t launch_n(const LeanDesc& d, const void* x, void* y, const void* tws, 
     if (!d.rows) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_lean<DIR, N, 1, 1, 0, 0, 0, 1>), dim3(d.groups), dim3(N), 0, st, d,
                        static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+    return hipGetLastError();
+  }
+  if (d.real) {
+    // the real edges of rows of 2N reals (rows in and out)
+    if (!d.rin || !d.rows) return hipErrorInvalidValue;
+    if (d.real == 1)
+      hipLaunchKernelGGL((k_lean<DIR, N, 1, 1, 0, 0, 1, 0, 0, 1>), dim3(d.groups), dim3(N), 0, st, d,
+                         static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
+    else
+      hipLaunchKernelGGL((k_lean<DIR, N, 1, 1, 0, 0, 1, 0, 0, 2>), dim3(d.groups), dim3(N), 0, st, d,
+                         static_cast<const float2*>(x), static_cast<float2*>(y), static_cast<const float2*>(tws));
     return hipGetLastError();
   }
   if (d.rin) {

@@ -98,6 +98,36 @@ PAIRED = [("f32", 2**16, 2), ("f32", 2**17, 3), ("f32", 2**18, 1), ("f32", 3 * 2
           ("f32", 5 * 2**15, 2), ("f64", 2**16, 2), ("f64", 2**19, 1), ("f64", 2**21, 2)]
 
 
+@pytest.mark.parametrize("n,batch", [(128, 16), (256, 48), (512, 32), (256, 4096)])
+def test_lean_real_rows(n, batch):
+    """FP32 real rows of 2N = 128/256/512 reals in groups of 16 on the lean
+    kernel: the R2C twist at its rows-out store, the C2R pre-pass at its
+    rows-in load; every row against the oracle, an arbitrary spectrum
+    (imaginary DC and Nyquist parts) through both inverses."""
+    h = n // 2
+    x = F.rng.normal(n + batch, n * batch).astype(np.float32).reshape(batch, n)
+    p = F.plan_rfft(torch.float32, n, batch=batch)
+    pi = F.plan_irfft(torch.float32, n, batch=batch)
+    pb = F.plan_brfft(torch.float32, n, batch=batch)
+    assert "kernel=lean rows_in rows_out" in p.describe() and "real=r2c" in p.describe()
+    assert "real=c2r" in pi.describe() and "kernel=lean" in pi.describe()
+    Xd = p(to_dev(x))
+    X = to_host(Xd).reshape(batch, h + 1)
+    ref = O.rfft(x.astype(np.float64))
+    for b in range(0, batch, max(1, batch // 64)):
+        assert rel_l2(X[b], ref[b]) <= tol_for("f32", n), b
+    assert rel_l2(to_host(pi(Xd)).reshape(batch, n), x) <= 2 * tol_for("f32", n)
+    Z = F.rng.uniform_complex(n + 11, (h + 1) * batch).astype(np.complex64).reshape(batch, h + 1)
+    Zd = to_dev(Z)
+    yi = to_host(pi(Zd)).reshape(batch, n)
+    yb = to_host(pb(Zd)).reshape(batch, n)
+    for b in range(0, batch, max(1, batch // 64)):
+        zr = Z[b].astype(np.complex128)
+        assert rel_l2(yi[b], O.irfft(zr, n)) <= tol_for("f32", n), b
+        assert rel_l2(yb[b], O.irfft(zr, n, normalized=False)) <= tol_for("f32", n), b
+    assert np.array_equal(to_host(Zd), Z)
+
+
 @pytest.mark.parametrize("prec,n,batch", PAIRED)
 def test_real_multipass_paired(prec, n, batch):
     T = torch_dtype(prec)
