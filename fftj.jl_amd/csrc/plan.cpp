@@ -1275,6 +1275,25 @@ static void build_c2c(Layout& L, int64_t n, int64_t batch, int64_t Bi, Geometry 
         double score = 1e300;
         factor_search(L.dtype, n, 2, 512, cur, F, score);
       }
+      // small footprints (the array inside the L2s, 8 MiB and less) above that
+      // length: two passes where three run latency bound, the column pass a
+      // power of two of 64-256 points (the power-of-two kernel, and a
+      // resampled grid's fused deconvolution), the row pass up to 4096.
+      // Measured on the 147,456-point grid x 4 of 65,537: three passes
+      // (64 x 48 x 48) 24.2 us, 2304 x 64 19.6 us, 1152 x 128 20.1 us,
+      // 576 x 256 20.6 us, the balanced 384 x 384 20.9 us; F64 2^19 x 1
+      // 27.9 -> 23.1 us.  FP32 powers of two keep the three lean passes
+      // (2^18 x 1 13.4 us against 16.9, 2^20 x 1 24.1 against 25.3).
+      static const bool small_env = [] {
+        const char* e = std::getenv("FFTJH_SMALL_TWO_PASS");
+        return !(e && std::atoi(e) == 0);
+      }();
+      const bool lean_pow2 = L.dtype == FFTJH_F32 && (n & (n - 1)) == 0;
+      const bool small = small_env && !lean_pow2 && n * batch * esize(L.dtype) <= (int64_t(8) << 20);
+      if (!mf_env && F.empty() && L.two_pass512 && !colblock && small && n > two_pass_max) {
+        for (int64_t c = 64; c <= 256 && F.empty(); c *= 2)
+          if (n % c == 0 && n / c <= 4096 && single_pass_ok(L.dtype, n / c)) F = {n / c, c};
+      }
       if (F.empty()) F = choose_factors(L.dtype, n, default_max_factor(L.dtype));
       // FP32 plans of 128 MiB and more per array: column passes on the 81
       // and 100 chains where the balanced split leaves one to the runtime

@@ -86,9 +86,11 @@ def test_schedule_config2():
 
 def test_schedule_config3():
     # the deconvolution rides on the first (power-of-two) pass of the m-point
-    # transform; the interpolation is its own launch
+    # transform; the interpolation is its own launch.  The 147,456-point grid
+    # x 4 (4.5 MiB) is a small footprint: two passes, 64 x 2304
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [65537], batch=4))
-    assert len(s) == 4 and s[0].startswith("pass x->") and "deconv n=65537 m=147456" in s[0]
+    assert len(s) == 3 and s[0].startswith("pass x->") and "deconv n=65537 m=147456" in s[0]
+    assert " n=64 " in s[0] and " n=2304 " in s[1]
     assert s[-1].startswith("rs_interp")
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [1000003], batch=4))
     assert "deconv n=1000003 m=2097152" in s[0]
