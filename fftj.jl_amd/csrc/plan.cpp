@@ -2405,6 +2405,26 @@ struct fftjh_plan_s {
   fftjh::Layout L;
   fftjh::DevBufPtr scratch[4];
   int device = 0;
+  // ragged plans with long items: a second stream on which the short-item
+  // launches run beside the long items' tile passes, forked from and joined
+  // back to the caller's stream through two events
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  mutable std::mutex side_mu;  // one fork/join sequence at a time (calls from several threads)
+  fftjh_plan_s() = default;
+  fftjh_plan_s(const fftjh_plan_s&) = delete;
+  fftjh_plan_s& operator=(const fftjh_plan_s&) = delete;
+  ~fftjh_plan_s() {
+    if (!side) return;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return;
+    if (cur != device) (void)hipSetDevice(device);
+    (void)hipStreamSynchronize(side);
+    (void)hipEventDestroy(fork);
+    (void)hipEventDestroy(join);
+    (void)hipStreamDestroy(side);
+    if (cur != device) (void)hipSetDevice(cur);
+  }
 };
 
 namespace fftjh {
@@ -2606,6 +2626,15 @@ static void materialize(fftjh_plan_s& P) {
                   "twiddles");
       hip_check(hipDeviceSynchronize(), "ragged tables");
       R.d_ws = std::make_shared<DevBuf>(std::max<int64_t>(R.ws_total, 1) * es);
+      static const bool fork_env = [] {
+        const char* e = std::getenv("FFTJH_RAGGED_FORK");
+        return !e || std::atoi(e) != 0;
+      }();
+      if (fork_env && !P.side) {
+        hip_check(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        hip_check(hipEventCreateWithFlags(&P.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+        hip_check(hipEventCreateWithFlags(&P.join, hipEventDisableTiming), "hipEventCreateWithFlags");
+      }
      }
     } else if (s.kind == ST_R2C_TWIST || s.kind == ST_C2R_PRE) {
       // w_k = exp(dir*2*pi*i*k/n), k = 0..h (real.jl:26-39)
@@ -2762,7 +2791,20 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
       return launch_strided_copy(dt, ptr[s.src], ptr[s.dst], s.n, s.batch, s.Bi, s.ga, s.gb, st);
     case ST_RAGGED: {
       // the small-item launch, then the large-item one (same stream: they
-      // may share a caller's workspace)
+      // may share a caller's workspace).  With the plan's own workspaces
+      // (disjoint) and long items, the two run on the plan's side stream
+      // beside the long items' passes.
+      const bool small = (s.rg && !s.rg->items.empty()) || (s.rg2 && !s.rg2->items.empty());
+      const bool fork = !ws && P.side && s.rg3 && small;
+      hipStream_t sst = st;
+      std::unique_lock<std::mutex> lock(P.side_mu, std::defer_lock);
+      if (fork) {
+        lock.lock();
+        hipError_t e = hipEventRecord(P.fork, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(P.side, P.fork, 0);
+        if (e != hipSuccess) return e;
+        sst = P.side;
+      }
       for (const RaggedData* Rp : {s.rg.get(), s.rg2.get()}) {
         if (!Rp || Rp->items.empty()) continue;
         const RaggedData& R = *Rp;
@@ -2779,7 +2821,7 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
         rl.y = ptr[s.dst];
         rl.ws = ws ? ws : R.d_ws->p;
         rl.wg_ws = R.d_wgws->p;
-        const hipError_t e = launch_ragged(dt, s.dir, rl, st);
+        const hipError_t e = launch_ragged(dt, s.dir, rl, sst);
         if (e != hipSuccess) return e;
       }
       if (s.rg3) {
@@ -2800,6 +2842,11 @@ static hipError_t launch_step(const fftjh_plan_s& P, const Step& s, void* const*
           const hipError_t e = launch_ragged_pass(dt, s.dir, pl, st);
           if (e != hipSuccess) return e;
         }
+      }
+      if (fork) {
+        const hipError_t e = hipEventRecord(P.join, P.side);
+        if (e != hipSuccess) return e;
+        return hipStreamWaitEvent(st, P.join, 0);
       }
       return hipSuccess;
     }
