@@ -26,8 +26,8 @@ FLAGS = [
 ]
 SOURCES = ["stockham.hip", "stockham_pow2.hip", "stockham_pow2_f32_fwd.hip", "stockham_pow2_f32_inv.hip",
            "stockham_pow2_f64_fwd.hip", "stockham_pow2_f64_inv.hip", "twiddle.hip", "real.hip", "resample.hip",
-           "ragged.hip", "mlp.hip", "stockham_mat.hip", "stockham_lean.hip", "stockham_lean_sq.hip", "hvdh.hip",
-           "plan.cpp"]
+           "ragged.hip", "mlp.hip", "stockham_mat.hip", "stockham_lean.hip", "stockham_lean_sq.hip",
+           "stockham_lean_pair.hip", "hvdh.hip", "plan.cpp"]
 
 
 def _compile(src):

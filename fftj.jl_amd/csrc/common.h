@@ -166,6 +166,15 @@ struct LeanDesc {
   double pinv;
   int64_t rs_n, rs_m;
   const void* rs_corr;
+  // real = 3 / 4 (k_lean_pair): the paired edges of a multi-pass
+  // half-length transform of real_h = n*pm points per row, pm transforms T
+  // per row; transform T of row r at (T % pbi)*s0 + ((T / pbi) % pb2)*s1 +
+  // r*s2 on each side (pbi*pb2 = pm), elements at istride / ostride; rtw the
+  // FP64 two-level table of w_{2h} (rtw_sh, rtw_hi as the pass kernel's);
+  // the C2R pass's twiddle multiplier q = (T % pbi) / pqdiv
+  int32_t pm;
+  int32_t rtw_sh;
+  int64_t pbi, pb2, pqdiv, real_h, rtw_hi;
 };
 
 // HvdH route (hvdh.hip) of `batch` transforms of odd non-smooth n, FP32:

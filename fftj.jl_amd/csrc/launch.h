@@ -33,6 +33,8 @@ bool lean_length(int n);
 int lean_sq_cols(int n);
 hipError_t launch_lean_sq(int dir, const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t stream);
 hipError_t launch_lean(int dir, const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t stream);
+// the paired real edges of a multi-pass half-length transform (stockham_lean_pair.hip)
+hipError_t launch_lean_pair(int dir, const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t stream);
 
 // HvdH route kernels (FP32): the Delta permutation dst[Delta*l mod n] =
 // scale*src[l], the spread of the permuted input onto grid rows of t, the

@@ -2330,13 +2330,13 @@ static std::string describe(const Layout& L) {
         << " kernel=lean" << (d.mat == kMatF16 ? " kernel=mfma_f16" : d.mat == kMatBF16 ? " kernel=mfma_bf16"
                               : d.mat == kMatBF16x3 ? " kernel=mfma_bf16x3" : "")
         << (d.rin ? " rows_in" : "") << (d.rows ? " rows_out" : "") << " batch=" << s.batch << " C="
-        << (d.cols ? d.cols : 16) << " threads=" << (d.cols ? d.cols * sqr : d.n)
+        << (d.cols ? d.cols : d.real >= 3 ? 32 : 16) << " threads=" << (d.cols ? d.cols * sqr : d.real >= 3 ? 2 * d.n : d.n)
         << " groups=" << d.groups << " in_stride=" << (d.rin ? 1 : d.istride) << " out_stride=" << (d.rows ? 1 : d.ostride);
       if (d.psh) o << " twiddle=w_" << (d.pmod ? d.pmod : int64_t(1) << d.psh);
       if (d.scale != 1.0) o << " scale=" << d.scale;
       if (d.nt) o << " stream=" << ((d.nt & 1) ? "in" : "") << ((d.nt & 2) ? "out" : "");
       if (d.rs) o << " deconv n=" << d.rs_n << " m=" << d.rs_m << " w=" << s.w;
-      if (d.real) o << (d.real == 1 ? " real=r2c" : " real=c2r");
+      if (d.real) o << (d.real == 1 ? " real=r2c" : d.real == 2 ? " real=c2r" : d.real == 3 ? " real=r2c-paired" : " real=c2r-paired");
       const int as = s.alias_src >= 0 ? s.alias_src : s.src, ad = s.alias_dst >= 0 ? s.alias_dst : s.dst;
       if (as != s.src || ad != s.dst) o << " aliased=" << roles[as] << "->" << roles[ad];
       o << "\n";
@@ -2667,7 +2667,18 @@ static void materialize(fftjh_plan_s& P) {
         s.aux = rs_corr(s);
         s.ld.rs_corr = s.aux->p;
       }
-      if (s.ld.real) {
+      if (s.ld.real >= 3) {
+        // w_{2h}^e (real.jl:26-39) from an FP64 two-level table, the same
+        // table as the power-of-two kernel's paired passes
+        const int64_t P = 2 * s.ld.real_h, hi = s.ld.rtw_hi, lo = int64_t(1) << s.ld.rtw_sh;
+        std::ostringstream rk;
+        rk << "ptw:" << dev << ":" << dir << ":" << P;
+        s.aux = cached(rk.str(), (hi + lo) * 16, [&](void* p) {
+          hip_check(launch_tw_pow(FFTJH_F64, p, hi, lo, P, dir, 0), "twiddle kernel");
+          hip_check(launch_tw_pow(FFTJH_F64, static_cast<char*>(p) + hi * 16, lo, 1, P, dir, 0), "twiddle kernel");
+        });
+        s.ld.rtw = s.aux->p;
+      } else if (s.ld.real) {
         // w_k = exp(dir*2*pi*i*k/(2N)), k = 0..N (real.jl:26-39)
         const int64_t n2 = 2 * int64_t(N), cnt = N + 1;
         s.aux = cached(key_of("real", {dev, double(dt), double(dir), double(n2)}), cnt * es,
@@ -2990,6 +3001,56 @@ static void lean_rows(Layout& L) {
       ld.istride = d.in.s_out;
       ld.out_s0 = 16 * d.out.s_out;
       ld.ostride = d.out.s_out;
+      ld.scale = d.scale;
+      s = st;
+      continue;
+    }
+    // the paired real edges of a multi-pass half-length transform (R2C last
+    // pass, rows in; C2R first pass, columns in), 64 or 128 points: two
+    // 16-column sub-groups (a column block and its mirror) per workgroup
+    static const bool pair_env = [] {
+      const char* e = std::getenv("FFTJH_LEAN_PAIRED");
+      return !e || std::atoi(e) != 0;
+    }();
+    if (pair_env && (d.n == 64 || d.n == 128) && (d.real_io == 3 || d.real_io == 4) && !d.rs_io &&
+        d.pair_M % 32 == 0 && d.Bi * d.B2 == d.pair_M && d.batch % d.pair_M == 0 && d.batch / 32 < (int64_t(1) << 31) &&
+        d.Bi < (int64_t(1) << 31) && d.B2 < (int64_t(1) << 31) && !d.nt_load && !d.nt_store &&
+        (d.real_io == 4 || (d.in.stride == 1 && !d.tw_period)) &&
+        (!d.tw_period || ((d.tw_period & (d.tw_period - 1)) == 0 && d.tw_period <= (int64_t(1) << 32) &&
+                          d.tw_qdiv >= 1 && d.tw_qdiv < (int64_t(1) << 31)))) {
+      Step st;
+      st.kind = ST_LEAN;
+      st.dir = s.dir;
+      st.src = s.src;
+      st.dst = s.dst;
+      st.alias_src = s.alias_src;
+      st.alias_dst = s.alias_dst;
+      st.n = d.n;
+      st.batch = d.batch;
+      LeanDesc& ld = st.ld;
+      ld.n = d.n;
+      ld.real = d.real_io;
+      ld.groups = static_cast<int32_t>(d.batch / 32);
+      ld.D0 = ld.groups;
+      ld.D1 = 1;
+      ld.pm = d.pair_M;
+      ld.pbi = d.Bi;
+      ld.pb2 = d.B2;
+      ld.in_s0 = d.in.s_in;
+      ld.in_s1 = d.in.s_mid;
+      ld.in_s2 = d.in.s_out;
+      ld.istride = d.in.stride;
+      ld.out_s0 = d.out.s_in;
+      ld.out_s1 = d.out.s_mid;
+      ld.out_s2 = d.out.s_out;
+      ld.ostride = d.out.stride;
+      ld.real_h = d.real_h;
+      ld.rtw_sh = d.rtw_sh;
+      ld.rtw_hi = d.rtw_hi;
+      if (d.tw_period) {
+        lean_twiddle(ld, d.tw_period, d.tw_acoef % d.tw_period, d.tw_bcoef % d.tw_period);
+        ld.pqdiv = d.tw_qdiv;
+      }
       ld.scale = d.scale;
       s = st;
       continue;

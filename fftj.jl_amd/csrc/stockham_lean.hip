@@ -860,6 +860,7 @@ bool lean_length(int n) { return n == 64 || n == 128 || n == 256; }
 
 hipError_t launch_lean(int dir, const LeanDesc& d, const void* x, void* y, const void* tws, hipStream_t st) {
   if (d.cols) return launch_lean_sq(dir, d, x, y, tws, st);  // square lengths (stockham_lean_sq.hip)
+  if (d.real >= 3) return launch_lean_pair(dir, d, x, y, tws, st);  // paired real edges
   return dir < 0 ? launch_dir<-1>(d, x, y, tws, st) : launch_dir<1>(d, x, y, tws, st);
 }
 

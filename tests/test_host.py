@@ -114,6 +114,24 @@ def test_schedule_config3():
     assert len(s) == 3 and "in_stride=1 " in s[0] and "in_stride=256 " in s[1] and "in_stride=65536 " in s[2]
 
 
+def test_lean_paired_real_schedule():
+    # the paired real edges of 64- and 128-point multi-pass passes run on the
+    # lean pair kernel (two 16-column sub-groups, 2n threads); 256-point ones
+    # and C2R passes whose twiddle period is not a power of two keep pow2
+    for n, batch, kind, edge in ((1 << 22, 4, _lib.KIND_R2C, -1), (1 << 22, 4, _lib.KIND_C2R, 0),
+                                 (1 << 20, 16, _lib.KIND_R2C, -1), (1 << 20, 16, _lib.KIND_C2R, 0),
+                                 (786432, 3, _lib.KIND_R2C, -1)):
+        s = _passes(_lib.describe_request(kind, _lib.F32, [n], batch=batch))
+        p = s[edge]
+        assert "kernel=lean" in p and "paired" in p and "C=32 " in p, p
+        m = int(p.split(" n=")[1].split()[0])
+        assert f"threads={2 * m} " in p and m in (64, 128)
+    s = _passes(_lib.describe_request(_lib.KIND_R2C, _lib.F32, [1 << 18], batch=64))
+    assert "kernel=pow2 real=r2c-paired" in s[-1]
+    s = _passes(_lib.describe_request(_lib.KIND_C2R, _lib.F32, [196608], batch=3))
+    assert "kernel=pow2 real=c2r-paired" in s[0] and "twiddle=w_3072" in s[0]
+
+
 def test_lean_rows_and_columns_schedule():
     # FP32 rows of 64/128/256 points in groups of 16 take the lean kernel with
     # both edges transposed through LDS; N-d axes whose column blocks hold a
