@@ -161,3 +161,42 @@ def test_per_size_baseline_matches_ragged():
     offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
     for n, o in zip(sizes.tolist(), offs.tolist()):
         assert rel_l2(yp[o:o + n], yr[o:o + n]) <= 2 * tol_for("f32", n), n
+
+
+def test_ragged_side_stream_eager_and_graph():
+    """Short items (one launch), resampled primes on the large-tile launch
+    and long items as two tile passes: the first two run on the plan's side
+    stream beside the passes, forked from and joined back to the caller's
+    stream.  Calls on another stream, and a captured graph's replays with
+    eager work between them, give the same bits as the first eager call."""
+    sizes = np.array([64, 100, 1000, 4096, 257, 8191, 20000, 32768, 65536, 6000, 9973], dtype=np.int64)
+    x = to_dev(inputs_c(17, int(sizes.sum()), "f32"))
+    p = F.plan_ragged(torch.float32, sizes)
+    d = p.describe()
+    assert " pass_transforms=" in d and " large_transforms=" in d and " direct=" in d
+    ref = to_host(p(x))
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    xh = to_host(x)
+    for n, o in zip(sizes.tolist(), offs.tolist()):
+        assert rel_l2(ref[o:o + n], O.fft(xh[o:o + n].astype(np.complex128), sign=-1)) <= tol_for("f32", n), n
+    y = torch.empty_like(x)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(2):
+            p(y, x)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(y), ref)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        p(y, x)
+    z = torch.empty_like(x)
+    for i in range(3):
+        y.zero_()
+        p(z, x)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(to_host(y), ref), i
+        assert np.array_equal(to_host(z), ref), i
