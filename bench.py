@@ -485,8 +485,8 @@ def main():
         print(json.dumps({"cpu_configs": cpu_configs_leg(__graft_entry__.load_package())}), flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this library runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "a design, not built)"}))
         sys.exit(2)
     import numpy as np
     import torch

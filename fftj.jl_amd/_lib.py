@@ -79,7 +79,7 @@ class HipError(RuntimeError):
 
 
 class UnsupportedError(NotImplementedError):
-    """A valid request this build does not plan."""
+    """A valid request this library does not plan."""
 
 
 _ERRS = {ERR_ARGUMENT: ArgumentError, ERR_DIMENSION: DimensionMismatch, ERR_HIP: HipError,

@@ -37,7 +37,7 @@ extern "C" {
 #define FFTJH_ERR_ARGUMENT 1    /* Julia ArgumentError */
 #define FFTJH_ERR_DIMENSION 2   /* Julia DimensionMismatch */
 #define FFTJH_ERR_HIP 3         /* a HIP runtime call failed */
-#define FFTJH_ERR_UNSUPPORTED 4 /* valid request outside what this build plans */
+#define FFTJH_ERR_UNSUPPORTED 4 /* valid request outside what this library plans */
 
 #define FFTJH_F32 0 /* Float32 / ComplexF32 */
 #define FFTJH_F64 1 /* Float64 / ComplexF64 */

@@ -1,6 +1,6 @@
 """GPU parity: 1-D complex plans (direct chain, four-step, batched) against
 the FP64 oracle on the same seeded inputs (the reference's test matrix,
-runtests.jl:6-56, 171-231), plus the BASELINE configurations at full size."""
+runtests.jl:6-56, 171-231), plus the benchmark's 1-D shapes at full size."""
 import numpy as np
 import pytest
 import torch
@@ -88,7 +88,7 @@ def test_bfft_is_n_times_ifft():
     assert rel_l2(yb, n * yi) < 1e-14
 
 
-# ---- BASELINE configurations at full size ------------------------------------
+# ---- the benchmark's 1-D shapes at full size ------------------------------------
 
 def test_config1_f64_1024(golden):
     x = golden["cfg1_x"]

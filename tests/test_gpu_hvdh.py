@@ -2,7 +2,7 @@
 Gaussian spread onto a smooth t coprime to n, the t-point transform, chunked
 banded solves) for odd non-smooth FP32 lengths: against the FP64 DFT and the
 oracle's restatement of HvdhPlan on the same seeded inputs, both directions,
-in place, batched -- the BASELINE configuration-3 primes at full size."""
+in place, batched -- the primes 65,537 x 4 and 1,000,003 x 4 at full size."""
 import numpy as np
 import pytest
 import torch

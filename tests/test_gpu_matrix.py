@@ -1,7 +1,7 @@
 """GPU parity on the rest of the reference's threaded test matrix and on the
 plans the other suites do not reach: the 1-D lengths of threading.jl:80-81
 and the both-axes-non-smooth shape of threading.jl:124, in both precisions;
-C2R at the BASELINE size compared row by row with the oracle's irfft; the
+C2R 2^22 x 4 compared row by row with the oracle's irfft; the
 2^24 transform forced onto the 4096 x 4096 two-pass split; FP64 plans large
 enough for the streaming-hint kernel variants."""
 import numpy as np
@@ -47,7 +47,7 @@ def test_nd_both_axes_nonsmooth(prec):
 
 
 def test_config3_c2r_2pow22_x4_rows():
-    # C2R at the BASELINE size: every row against the oracle's irfft (and
+    # C2R 2^22 x 4: every row against the oracle's irfft (and
     # brfft), on a spectrum that is not the transform of real data
     n, batch = 1 << 22, 4
     h = n // 2 + 1
@@ -64,7 +64,7 @@ def test_config3_c2r_2pow22_x4_rows():
 
 
 def test_2pow24_forced_4096x4096_split():
-    # the four-step split the BASELINE configuration names (4096 x 4096),
+    # the four-step split of 2^24 into 4096 x 4096,
     # forced through the tuned entry point, against the oracle
     n = 1 << 24
     print("c2c f32 n=2^24 splits=4096x4096", flush=True)

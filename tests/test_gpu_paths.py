@@ -1,6 +1,6 @@
 """GPU parity: prime (resampling) plans, real plans, N-d plans, against the
 FP64 oracle on the same seeded inputs (runtests.jl:57-83, 233-262, 330-389),
-plus the BASELINE configuration 3 shapes at full size."""
+plus the benchmark's prime, real and 3-D shapes at full size."""
 import numpy as np
 import pytest
 import torch

@@ -74,7 +74,7 @@ def test_invariants(n):
 # ------------------------------------------------------- reference KATs (A1)
 
 @pytest.mark.parametrize("n,W,chain", [
-    (1024, 8, [4, 4, 8, 8]),                 # F64 1024
+    (1024, 8, [4, 4, 8, 8]),                 # F64 1024 (the plain radix-2/4 case)
     (4096, 16, [4, 4, 4, 8, 8]),             # F32 4096
     (10**6, 16, [4, 4, 5, 5, 5, 5, 5, 5, 4]),  # F32 10^6 serial chain
     (256, 16, [4, 4, 4, 4]),
