@@ -183,3 +183,20 @@ def test_composite_radix_chains(prec, chain):
 def test_default_composite_plans(prec, n):
     # lengths whose default chains take composite radices
     _check_c2c(prec, n, batch=2)
+
+
+@pytest.mark.parametrize("prec,n,batch,launches", [("f32", 147456, 4, 2), ("f32", 3 * 2**17, 2, 2),
+                                                   ("f64", 2**19, 1, 2), ("f32", 2**18, 2, 3)])
+def test_small_footprint_two_passes(prec, n, batch, launches):
+    # arrays of 8 MiB and less above the two-pass limit: two passes with a
+    # power-of-two column pass (FP32 powers of two keep the three lean passes)
+    T = torch_dtype(prec)
+    x = inputs_c(n % 97 + batch, n * batch, prec).reshape(batch, n)
+    for sign in (-1, 1):
+        p = F.plan_fft(T, n, batch=batch) if sign < 0 else F.plan_ifft(T, n, batch=batch)
+        d = p.describe()
+        assert f"launches={launches}" in d, d
+        y = to_host(p(to_dev(x))).reshape(batch, n)
+        ref = O.fft(x.astype(np.complex128), sign=sign, scale=1.0 if sign < 0 else 1.0 / n)
+        for b in range(batch):
+            assert rel_l2(y[b], ref[b]) <= tol_for(prec, n), (sign, b)
