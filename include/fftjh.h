@@ -120,7 +120,13 @@ int fftjh_plan_c2c_tuned(fftjh_plan* plan, int dtype, int rank, const int64_t* d
 
 /* Ragged batch: `count` independent complex transforms of lengths sizes[i]
  * at complex offsets offsets[i] of x and y (NULL = packed back to back), run
- * by one persistent kernel launch.  sign -1 forward; +1 inverse, scaled by
+ * by at most four launches: the short and the resampled transforms on
+ * persistent item-granular launches, the long ones as two tile-granular
+ * passes.  The item-granular launches run on a stream the plan owns, forked
+ * from the execute call's stream by an event and joined back to it by
+ * another, so the call stays ordered on that stream (and capturable in a
+ * graph); fftjh_execute_static keeps all four on the caller's stream.
+ * sign -1 forward; +1 inverse, scaled by
  * 1/sizes[i] when normalized != 0.  Lengths up to 65536 (FP32) / 32768
  * (FP64).  No reference counterpart: FFTJ has no batch or ragged API, and a
  * caller loops over 1-D plans (plan.jl:207-213); this stands in for that loop. */
