@@ -565,6 +565,25 @@ static void add_pass(Layout& L, int dir, int n, int64_t batch, int64_t Bi, Geome
     }
   }
   if (!bestC) fail(FFTJH_ERR_UNSUPPORTED, "pass does not fit one workgroup");
+  // small strided passes (8 MiB and less) whose groups would not cover the
+  // 256 CUs: fewer transforms per workgroup, more workgroups (latency bound).
+  // Measured (us, off -> on): 147,456 x 4 grid 19.7 -> 16.0 (65,537 x 4 28.4
+  // -> 24.9), 100,003 29.1 -> 22.1, 3^9 x 2 14.0 -> 11.7, 54,675 x 2 14.4 ->
+  // 13.2, F64 10,007 x 3 even; single-pass contiguous rows lose (1000 x 100
+  // 7.1 -> 7.7), so they keep their group size
+  static const bool spread_env = [] {
+    const char* e = std::getenv("FFTJH_SMALL_SPREAD");
+    return !e || std::atoi(e) != 0;
+  }();
+  if (spread_env && colmode && !(pt && pt->C > 0) && !real_io && !s.pow2 &&
+      n * batch * esize(L.dtype) <= (int64_t(8) << 20)) {
+    while (bestC > 1 && (batch + bestC - 1) / bestC < 256) {
+      int t;
+      if (!fits(bestC / 2, &t)) break;
+      bestC /= 2;
+      bestT = t;
+    }
+  }
   d.C = bestC;
   d.logC = 0;
   while ((1 << d.logC) < bestC) ++d.logC;

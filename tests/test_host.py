@@ -210,11 +210,12 @@ def test_chain_column_split():
 def test_fixed_chain_gives_way_to_generic():
     # the 81 = 9 x 9 chain runs three threads per transform, so only 64
     # columns fill its waves; 64 columns take it, 40 fall back to the generic
-    # kernel with its own group search (16 columns) and compact radices 9,9
+    # kernel with compact radices 9,9 (and, a small pass whose groups would
+    # not cover the CUs, one column per group)
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [64, 81]))
     assert "n=81 radices=9,9 kernel=chain" in s[1] and "C=64 " in s[1]
     s = _passes(_lib.describe_request(_lib.KIND_C2C, _lib.F32, [40, 81]))
-    assert "n=81 radices=9,9" in s[1] and "kernel=chain" not in s[1] and "C=16 " in s[1]
+    assert "n=81 radices=9,9" in s[1] and "kernel=chain" not in s[1] and "C=1 " in s[1] and "groups=40 " in s[1]
     # a tuned group size the chain cannot take runs the generic kernel
     # instead of refusing the plan; one that fits no kernel is still refused
     t = _lib.make_tuning((81, 6561), [{}, {"C": 4}])
