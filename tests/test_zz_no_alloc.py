@@ -100,6 +100,38 @@ def test_static_scratch_ragged_workspace_twice_smaller():
             assert lib.fftjh_plan_destroy(h) == L.OK
 
 
+def test_static_scratch_real_paired_passes():
+    # R2C / C2R on the paired lean passes (2^20 x 2: three passes through
+    # scratch); static scratch against fftjh_execute, bit for bit, and the
+    # forward against the oracle
+    lib = _lib()
+    n, batch, h1 = 1 << 20, 2, (1 << 19) + 1
+    print(f"execute_static r2c/c2r f32 n={n} batch={batch}", flush=True)
+    x = F.rng.normal(31, n * batch).astype(np.float32)
+    xd = to_dev(x)
+    hr, hc = ctypes.c_void_p(), ctypes.c_void_p()
+    assert lib.fftjh_plan_r2c(ctypes.byref(hr), L.F32, n, batch, 0) == L.OK
+    assert lib.fftjh_plan_c2r(ctypes.byref(hc), L.F32, n, batch, 1, 0) == L.OK
+    try:
+        X1 = torch.full((batch * h1,), 3.0, dtype=torch.complex64, device="cuda")
+        X2 = torch.full_like(X1, -5.0)
+        assert lib.fftjh_execute(hr, _ptr(X1), _ptr(xd), _stream()) == L.OK
+        assert lib.fftjh_execute_static(hr, _ptr(X2), _ptr(xd), _stream()) == L.OK, lib.fftjh_last_error().decode()
+        a, b = to_host(X1), to_host(X2)
+        assert np.array_equal(a, b)
+        ref = O.rfft(x.reshape(batch, n).astype(np.float64))
+        assert rel_l2(b.reshape(batch, h1)[batch - 1], ref[batch - 1]) <= tol_for("f32", n)
+        z1 = torch.full((batch * n,), 3.0, dtype=torch.float32, device="cuda")
+        z2 = torch.full_like(z1, -5.0)
+        assert lib.fftjh_execute(hc, _ptr(z1), _ptr(X1), _stream()) == L.OK
+        assert lib.fftjh_execute_static(hc, _ptr(z2), _ptr(X1), _stream()) == L.OK, lib.fftjh_last_error().decode()
+        assert np.array_equal(to_host(z1), to_host(z2))
+        assert rel_l2(to_host(z2), x) <= 2 * tol_for("f32", n)
+    finally:
+        assert lib.fftjh_plan_destroy(hr) == L.OK
+        assert lib.fftjh_plan_destroy(hc) == L.OK
+
+
 def test_static_scratch_refuses_plans_beyond_its_size():
     lib = _lib()
     print("execute_static refusal: c2c f32 n=2^24 (128 MiB intermediate)", flush=True)
